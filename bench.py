@@ -1,0 +1,375 @@
+"""Throughput of the D2Q9 MRT lattice update on one MI355X.
+
+Headline (``value``): GLUPS of the FP32 fused MRT step on the cylinder-wake
+lattice, 800x300, circle r=30 at (200,150), Re=100 (tau=0.68), u=0.1, seed-42
+perturbation, Mei interpolated bounce-back, Zou/He inlet, outflow copy,
+periodic y -- the reference's semantics (src/main.cpp, src/lbm.hpp).  Inputs are
+resident in HBM; the timed region is K steps replayed from hipGraphs of 100.
+
+Extra forms on the same JSON line (``forms``): FP32 and FP16-shifted storage on
+the 6400x2400 LES cylinder, the 256-lattice batched ensemble against the same
+lattices stepped one launch each, and the 32768x16384 domain when
+``--large`` is given.
+
+``roofline`` is for the dominant kernel (the fused step kernel of the headline
+workload): algorithmic bytes per launch = N_fluid*18*s + N_cells*1 (s = bytes per
+stored population), divided by the average launch duration measured here with
+HIP events on the library's stream, against 8 TB/s.
+
+``cpu_baseline`` times the reference itself (oracle/_ref/libref_fast.so, built
+from the reference sources) or, when that is absent, the FP64 oracle, on this
+host's cores on a bounded sample of the same workload, in a child process.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "lbm-2d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cylinder_lattice(nx, ny, cx, cy, r, tau, u, storage, stream, les=False, cs=0.1, perturb=True):
+    import lbm2d as G
+    d = G.desc_for_case(G.CYLINDER, nx, ny, tau=tau, u_in=u, les=les, cs=cs)
+    ens = G.Ensemble([d], storage=storage, stream=stream)
+    mask = G.circle_mask(nx, ny, cx, cy, r)
+    ens.set_obstacle(0, mask)
+    ens.set_bb_circle(0, cx, cy, r, G.BB_MEI)
+    f = G.equilibrium(np.ones((ny, nx)), np.full((ny, nx), u), np.zeros((ny, nx)))
+    if perturb:
+        # seed-42 transverse perturbation of the cylinder driver (src/main.cpp:68-83)
+        import oracle as O
+        g = O.MT19937(42)
+        for x in range(cx + 5, min(nx, cx + 60)):
+            for y in range(ny):
+                if mask[y, x]:
+                    continue
+                dv = g.uniform(-1e-4, 1e-4)
+                fr = f[y, x]
+                rho = fr.sum()
+                uu = (fr * G.CXV).sum() / rho
+                vv = (fr * G.CYV).sum() / rho
+                f[y, x] = G.equilibrium(np.array(rho), np.array(uu), np.array(vv + dv))
+    ens.upload(0, f)
+    return ens, mask
+
+
+def algorithmic_bytes(info, les_bytes):
+    s = info.bytes_per_pop
+    return info.fluid_cells * 18 * s + info.cells * 1 + info.les_cells * les_bytes
+
+
+def time_steps(ens, steps, warmup, torch):
+    ens.step(warmup)
+    ens.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_launch_ms(ens, torch, n=200):
+    """Average duration of one step-kernel launch, HIP events on the library's stream."""
+    stream = torch.cuda.current_stream()
+    ens.step_lattice(0, 10)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    for i in range(n):
+        starts[i].record(stream)
+        ens.step_lattice(0, 1)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+
+
+def graph_step_ms(ens, torch, replays=20, graph_steps=100):
+    """Average time per step inside hipGraph replays (events around each replay)."""
+    stream = torch.cuda.current_stream()
+    ens.step(graph_steps)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(replays + 1)]
+    evs[0].record(stream)
+    for i in range(replays):
+        ens.step(graph_steps)
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    return float(np.mean([evs[i].elapsed_time(evs[i + 1]) for i in range(replays)])) / graph_steps
+
+
+def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1):
+    ens, mask = cylinder_lattice(nx, ny, nx // 4, ny // 2, r, tau, 0.1, storage, stream,
+                                 les=les, cs=cs, perturb=False)
+    bx, rows, _ = ens.autotune()
+    info = ens.info()
+    t = time_steps(ens, steps, warmup, torch)
+    kms = kernel_launch_ms(ens, torch, n=50)
+    b = algorithmic_bytes(info, 2)
+    m0 = ens.mass(0)
+    out = {
+        "lattice": f"{nx}x{ny}", "storage": storage, "les": les,
+        "GLUPS": info.cells * steps / t / 1e9,
+        "ms_per_step": 1e3 * t / steps,
+        "kernel_ms": kms,
+        "bytes_per_step": b,
+        "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "launch": [bx, rows],
+        "mass": m0,
+    }
+    ens.close()
+    return out
+
+
+def ensemble_descs(seed=42, n=256):
+    """Batched-ensemble lattices: NX~U[256,1024], NY~U[96,384], 1-4 circles r~U[8,40],
+    centres uniform with a 2r margin, non-overlapping; Re log-uniform in [50,400];
+    tau = 0.5 + 3*0.1*D/Re with D the largest diameter.  numpy Generator(PCG64(seed))."""
+    import lbm2d as G
+    rng = np.random.default_rng(seed)
+    lats = []
+    for _ in range(n):
+        nx = int(rng.integers(256, 1025))
+        ny = int(rng.integers(96, 385))
+        k = int(rng.integers(1, 5))
+        circles = []
+        tries = 0
+        while len(circles) < k and tries < 200:
+            tries += 1
+            r = int(rng.integers(8, 41))
+            if 4 * r >= ny - 2 or 4 * r >= nx - 2:
+                continue
+            cx = int(rng.integers(2 * r, nx - 2 * r))
+            cy = int(rng.integers(2 * r, ny - 2 * r))
+            if all((cx - a) ** 2 + (cy - b) ** 2 > (r + c + 2) ** 2 for a, b, c in circles):
+                circles.append((cx, cy, r))
+        re = float(np.exp(rng.uniform(np.log(50), np.log(400))))
+        dmax = 2 * max(c[2] for c in circles)
+        tau = 0.5 + 3 * 0.1 * dmax / re
+        d = G.desc_for_case(G.CYLINDER, nx, ny, tau=tau, u_in=0.1)
+        mask = np.zeros((ny, nx), np.uint8)
+        for cx, cy, r in circles:
+            mask |= G.circle_mask(nx, ny, cx, cy, r)
+        lats.append((d, mask, circles))
+    return lats
+
+
+def ensemble_links(mask, circles, nx, ny):
+    """Per-obstacle exact-circle q for every link (nearest circle owns the link)."""
+    import lbm2d as G
+    links = []
+    ys, xs = np.nonzero(mask == 0)
+    solid = mask.astype(bool)
+    for i in range(1, 9):
+        tx = xs + G.CXV[i]
+        ty = (ys + G.CYV[i]) % ny
+        ok = (tx >= 0) & (tx < nx)
+        sel = np.nonzero(ok)[0]
+        hit = sel[solid[ty[sel], tx[sel]]]
+        for k in hit:
+            x, y = int(xs[k]), int(ys[k])
+            sx, sy = x + G.CXV[i], y + G.CYV[i]
+            c = min(circles, key=lambda c: (sx - c[0]) ** 2 + (sy - c[1]) ** 2)
+            q = _q_circle(c[0], c[1], c[2], x, y, i)
+            links.append((y * nx + x, i, G.BB_MEI, 0, q, 0.0, 0.0))
+    return links
+
+
+def _q_circle(ccx, ccy, r, xf, yf, i):
+    import lbm2d as G
+    dx, dy = xf - ccx, yf - ccy
+    ex, ey = float(G.CXV[i]), float(G.CYV[i])
+    a = ex * ex + ey * ey
+    b = 2.0 * (dx * ex + dy * ey)
+    c = dx * dx + dy * dy - r * r
+    disc = b * b - 4.0 * a * c
+    if disc < 0.0:
+        return 1.0
+    sd = np.sqrt(disc)
+    t1, t2 = (-b - sd) / (2 * a), (-b + sd) / (2 * a)
+    if 0.0 < t1 <= 1.0:
+        return float(t1)
+    if 0.0 < t2 <= 1.0:
+        return float(t2)
+    return 1.0
+
+
+def bench_ensemble(steps, warmup, torch, stream, n=256):
+    import lbm2d as G
+    lats = ensemble_descs(n=n)
+    ens = G.Ensemble([d for d, _, _ in lats], storage="f32", stream=stream)
+    for l, (d, mask, circles) in enumerate(lats):
+        ens.set_obstacle(l, mask)
+        ens.set_links(l, ensemble_links(mask, circles, d.nx, d.ny))
+        f = G.equilibrium(np.ones((d.ny, d.nx)), np.full((d.ny, d.nx), 0.1), np.zeros((d.ny, d.nx)))
+        ens.upload(l, f)
+    bx, rows, _ = ens.autotune()
+    info = ens.info()
+    t = time_steps(ens, steps, warmup, torch)
+    # the same lattices, one launch per lattice per step, no graph
+    plain_steps = max(2, steps // 20)
+    for l in range(n):
+        ens.step_lattice(l, 1)
+    ens.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for l in range(n):
+            ens.step_lattice(l, 1)
+    ens.sync()
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    b = algorithmic_bytes(info, 0)
+    glups = info.cells * steps / t / 1e9
+    glups_plain = info.cells * plain_steps / tp / 1e9
+    fx, _ = ens.forces(0)
+    out = {
+        "lattices": n, "cells": info.cells, "storage": "f32",
+        "GLUPS": glups, "ms_per_step": 1e3 * t / steps,
+        "hbm_frac": b / (t / steps) / 1e9 / HBM_PEAK_GBS,
+        "plain_per_lattice_GLUPS": glups_plain,
+        "speedup_vs_plain": glups / glups_plain,
+        "launch": [bx, rows],
+        "cd_lattice0_last": float(2 * fx[-1, 0] / (2 * max(c[2] for c in lats[0][2]) * 0.01)),
+    }
+    ens.close()
+    return out
+
+
+def cpu_baseline(seconds=12.0):
+    """Child-process timing of the reference (or the oracle) on the cylinder workload."""
+    code = f"""
+import json, os, sys, time
+sys.path.insert(0, {ROOT!r})
+import oracle as O
+threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count()))
+kind = 'reference' if O.ref_available(fast=True) else 'port'
+nx, ny, tau, u = 800, 300, 0.68, 0.1
+if kind == 'reference':
+    s = O.RefLattice(nx, ny, O.CYLINDER, fast=True)
+else:
+    s = O.Oracle(nx, ny, O.CYLINDER, faithful=True)
+s.place_cylinder(200, 150, 30)
+s.init_equilibrium(1.0, u, 0.0)
+s.perturb_seed42(205, 260)
+s.step(tau, u, 2)
+n, total = 0, 0.0
+chunk = 20
+while total < {seconds}:
+    t0 = time.perf_counter(); s.step(tau, u, chunk); total += time.perf_counter() - t0; n += chunk
+print(json.dumps({{'kind': kind, 'steps': n, 'seconds': total, 'threads': threads,
+                  'mlups': nx * ny * n / total / 1e6}}))
+"""
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(os.cpu_count()))
+    env["OMP_PROC_BIND"] = "close"
+    env["OMP_PLACES"] = "cores"
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                         timeout=300)
+    if out.returncode != 0:
+        return {"error": out.stderr[-400:]}
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["mlups"] / 1e3, "unit": "GLUPS", "cores": r["threads"], "kind": r["kind"],
+            "sample": f"800x300 cylinder MRT, {r['steps']} steps in {r['seconds']:.1f} s, "
+                      f"FP64 AoS, OpenMP {r['threads']} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--large", action="store_true", help="also time the 32768x16384 domain")
+    ap.add_argument("--no-forms", action="store_true", help="headline workload only")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the lattice update is measured on a single MI355X).", file=sys.stderr)
+        sys.exit(2)
+    if args.steps % 100 != 0:
+        args.steps = max(100, args.steps // 100 * 100)
+
+    import torch
+    import lbm2d as G
+    torch.cuda.init()
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ens, mask = cylinder_lattice(800, 300, 200, 150, 30, 0.68, 0.1, "f32", stream.cuda_stream)
+        bx, rows, _ = ens.autotune()
+        ens.set_launch(graph_steps=100)
+        info = ens.info()
+        t = time_steps(ens, args.steps, args.warmup, torch)
+        glups = info.cells * args.steps / t / 1e9
+        kms = kernel_launch_ms(ens, torch)
+        gms = graph_step_ms(ens, torch)
+        bytes_launch = algorithmic_bytes(info, 0)
+        fx, fy = ens.forces(0)
+        cd_last = 2.0 * fx[-1, 0] / (60 * 0.01)
+        ens.close()
+
+        forms = {}
+        if not args.no_forms:
+            big_steps = 200
+            forms["fp32_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "f32", big_steps, 20,
+                                                     torch, stream.cuda_stream, les=True)
+            forms["fp16s_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "f16s", big_steps, 20,
+                                                      torch, stream.cuda_stream, les=True)
+            forms["bf16s_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "bf16s", big_steps, 20,
+                                                      torch, stream.cuda_stream, les=True)
+            forms["fp16s_over_fp32"] = forms["fp16s_6400x2400_les"]["GLUPS"] / forms["fp32_6400x2400_les"]["GLUPS"]
+            forms["batched_256"] = bench_ensemble(1000, 100, torch, stream.cuda_stream)
+            if args.large:
+                forms["fp32_32768x16384"] = bench_form(32768, 16384, 60, 0.505, "f32", 20, 2, torch,
+                                                       stream.cuda_stream, les=True, cs=0.12)
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    achieved = bytes_launch / (kms * 1e-3) / 1e9
+    line = {
+        "metric": "GLUPS + % HBM peak, D2Q9 MRT step: FP32 plain vs FP16-store vs batched",
+        "value": glups,
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "cylinder wake 800x300, D2Q9 MRT, FP32 storage (f-w) and compute, "
+                        "r=30 at (200,150), Re=100 tau=0.68 u=0.1, Mei interpolated bounce-back, "
+                        "Zou/He inlet, outflow copy, periodic y, seed-42 perturbation, "
+                        "hipGraph of 100 steps",
+            "cells": info.cells, "fluid_cells": info.fluid_cells, "launch": [bx, rows],
+            "graph_step_ms": gms, "cd_last_step": cd_last,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel_ms": kms,
+            "bytes_per_launch": bytes_launch,
+        },
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

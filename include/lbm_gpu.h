@@ -1,0 +1,175 @@
+/*
+ * lbm_gpu.h -- C ABI of the MI355X (gfx950) D2Q9 lattice-update library.
+ *
+ * The reference has no FFI for this path: its boundary is the header-only C++
+ * call  execute_time_step(LBMCapabilities&, double tau, double u_inflow)
+ * (src/lbm.hpp:293) operating on the process-global lattice of
+ * src/lbm_types.hpp:15-32 and 187-224.  The entry points below replace that call
+ * and the state it touches, with the reference's implicit inputs made explicit
+ * per lattice so that many lattices of different sizes can live in one handle.
+ *
+ * Conventions
+ *   - Every function returns LBM_OK (0) or a nonzero status; lbm_last_error()
+ *     then holds a message (per host thread).  No exception crosses the ABI.
+ *   - Host buffers belong to the caller; device memory belongs to the handle.
+ *   - Populations cross the boundary in the reference's layout: AoS FP64,
+ *     f[(y*nx + x)*9 + i], directions as src/lbm_types.hpp:41-42.
+ *   - A handle is used from one host thread at a time; its kernels run in
+ *     order on one HIP stream (the caller's, or the null stream).
+ *   - The state visible after lbm_step is the reference's sys.f after
+ *     execute_time_step returns: post-stream, post-boundary, pre-collision.
+ */
+#ifndef LBM_GPU_H
+#define LBM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_INVALID 1 /* bad argument or call order */
+#define LBM_ERR_DEVICE 2  /* HIP runtime error */
+#define LBM_ERR_NOMEM 3   /* device or host allocation failed */
+
+typedef struct lbm_ensemble* lbm_handle;
+
+/* Population storage in HBM.  All forms store the deviation h_i = (f_i - w_i) * 2^k;
+ * F64 and F32 use k = 0, the 16-bit forms use the handle's scale exponent. */
+enum lbm_storage { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16S = 2, LBM_STORE_BF16S = 3 };
+
+/* CollisionType, src/lbm_types.hpp:26. */
+enum lbm_collision { LBM_MRT = 0, LBM_BGK = 1 };
+
+/* Treatment of each lattice edge (src/lbm.hpp:408-484): populations leaving through
+ * an OPEN edge are dropped (zero) and an inlet/outlet may overwrite them; a WALL
+ * edge bounces them back into the same node; PERIODIC wraps. */
+enum lbm_edge { LBM_EDGE_PERIODIC = 0, LBM_EDGE_OPEN = 1, LBM_EDGE_WALL = 2 };
+
+/* Velocity inlet at x = 0: Zou/He on every row (src/lbm.hpp:21-34) or the parabolic
+ * profile on rows y >= inlet_y0 of fluid nodes (src/lbm.hpp:40-58). */
+enum lbm_inlet { LBM_INLET_NONE = 0, LBM_INLET_ZOU_HE = 1, LBM_INLET_PARABOLIC = 2 };
+
+/* Boundary-link schemes (src/lbm.hpp:226-288). */
+enum lbm_bb { LBM_BB_SIMPLE = 0, LBM_BB_MEI = 1, LBM_BB_BOUZIDI = 2 };
+
+/* One lattice.  Replaces the globals NX, NY, g_case, g_collision, g_use_les, g_cs
+ * (src/lbm_types.hpp:15-32) and LBMCapabilities::body_force_x (:207). */
+typedef struct lbm_desc {
+    int32_t nx, ny;
+    int32_t collision;        /* enum lbm_collision */
+    int32_t les;              /* 1: Smagorinsky + Van Driest (src/lbm.hpp:155-182) */
+    double cs;                /* Smagorinsky constant (g_cs) */
+    int32_t x_edge, y_edge;   /* enum lbm_edge */
+    int32_t inlet;            /* enum lbm_inlet */
+    int32_t inlet_y0;         /* parabolic inlet: first row */
+    int32_t inlet_h;          /* parabolic inlet: profile height */
+    int32_t outlet;           /* 1: zero-gradient copy x=nx-2 -> nx-1 (src/lbm.hpp:63-73) */
+    int32_t lid;              /* 1: row ny-1 := feq(rho, u_in, 0) (src/lbm.hpp:110-120) */
+    int32_t forces;           /* 1: momentum exchange every step (src/lbm.hpp:542-581) */
+    double body_force_x;      /* LBMCapabilities::body_force_x */
+    double tau;               /* relaxation time */
+    double u_in;              /* inlet / lid velocity (u_inflow) */
+} lbm_desc;
+
+/* One boundary link: fluid node `cell` (= y*nx + x) whose neighbour cell + e_dir is
+ * solid.  Replaces the per-step q evaluation of BounceBackGeometry
+ * (src/lbm_types.hpp:116-176) by a precomputed entry. */
+typedef struct lbm_link {
+    int32_t cell;
+    int32_t dir;              /* 1..8 */
+    int32_t scheme;           /* enum lbm_bb */
+    int32_t slot;             /* force slot of this lattice, -1 = not counted */
+    double q;                 /* wall distance fraction along the link, (0,1] */
+    double uwx, uwy;          /* wall velocity at the solid node */
+} lbm_link;
+
+typedef struct lbm_info {
+    int64_t cells;            /* sum of nx*ny over the lattices */
+    int64_t fluid_cells;
+    int64_t link_cells;       /* fluid nodes with at least one boundary link */
+    int64_t les_cells;        /* cells of lattices with LES on */
+    int32_t bytes_per_pop;    /* 8, 4 or 2 */
+    int32_t n_lattices;
+    int64_t device_bytes;     /* device memory held by the handle */
+    int32_t block_x, rows;    /* launch shape chosen for the step kernel */
+} lbm_info;
+
+/* Message of the last failing call on this host thread. */
+const char* lbm_last_error(void);
+
+/* Fills *out for one of the reference's CaseType values (src/lbm_types.hpp:20-23),
+ * reproducing the branch structure of execute_time_step: stream edges
+ * (src/lbm.hpp:408-484), boundary conditions (:530-539) and forces (:543). */
+int lbm_desc_for_case(int case_type, int32_t nx, int32_t ny, lbm_desc* out);
+
+/* Creates a handle holding n lattices with one storage form.  stream may be NULL
+ * (null stream) or a hipStream_t owned by the caller.  scale_exp is the k of the
+ * 16-bit forms (ignored for F64/F32). */
+int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_exp,
+               void* stream, lbm_handle* out);
+int lbm_destroy(lbm_handle h);
+
+/* Geometry.  lbm_set_obstacle takes nx*ny bytes (nonzero = solid), the
+ * LBMCapabilities::obstacle array; it resets every link to simple bounce-back
+ * counted in slot 0, the reference's behaviour without bb_geom. */
+int lbm_set_obstacle(lbm_handle h, int32_t lattice, const uint8_t* mask);
+/* The reference's single interpolated-BB geometry (BounceBackGeometry with a circle
+ * or polygon, src/lbm_types.hpp:90-176): q of every link from that one shape, 1.0
+ * where the link misses it; scheme LBM_BB_MEI (the reference default,
+ * src/lbm_types.hpp:181) or LBM_BB_BOUZIDI. */
+int lbm_set_bb_circle(lbm_handle h, int32_t lattice, double cx, double cy, double r, int32_t scheme);
+int lbm_set_bb_polygon(lbm_handle h, int32_t lattice, const double* xy, int32_t n_vertices,
+                       int32_t scheme);
+/* Rotating wall (BounceBackGeometry::compute_wall_velocity, src/lbm_types.hpp:106-114). */
+int lbm_set_moving_wall(lbm_handle h, int32_t lattice, double omega, double rot_cx, double rot_cy);
+/* Counts only links whose solid node lies within r of (cx,cy) (src/lbm.hpp:569-573). */
+int lbm_set_force_radius(lbm_handle h, int32_t lattice, double cx, double cy, double r);
+/* Explicit per-link entries (per-obstacle q, scheme, wall velocity, slot).  Every
+ * link must exist in the obstacle mask. */
+int lbm_set_links(lbm_handle h, int32_t lattice, const lbm_link* links, int32_t n);
+/* Number of force slots of a lattice (default 1). */
+int lbm_set_force_slots(lbm_handle h, int32_t lattice, int32_t n_slots);
+
+/* Relaxation time and inlet/lid velocity for the following steps. */
+int lbm_set_params(lbm_handle h, int32_t lattice, double tau, double u_in);
+
+/* State transfer in the reference layout (AoS FP64, nx*ny*9 doubles). */
+int lbm_upload(lbm_handle h, int32_t lattice, const double* f_aos);
+int lbm_download(lbm_handle h, int32_t lattice, double* f_aos);
+
+/* Advances every lattice by n steps: one launch per step covering all lattices,
+ * replayed from a hipGraph of graph_steps launches when n allows. */
+int lbm_step(lbm_handle h, int32_t n);
+/* Advances one lattice by n steps with one launch per step and no graph. */
+int lbm_step_lattice(lbm_handle h, int32_t lattice, int32_t n);
+/* Launch shape / graph length; 0 = keep current.  Valid block_x: 64,128,256,512;
+ * rows: 1,2,4,8. */
+int lbm_set_launch(lbm_handle h, int32_t block_x, int32_t rows, int32_t graph_steps);
+/* Times every launch shape on the current state and keeps the fastest.  The
+ * lattice state and force history are unchanged. */
+int lbm_autotune(lbm_handle h, int32_t* best_block_x, int32_t* best_rows, float* best_ms);
+
+/* Total force per step, per slot, since the previous call (FP64), in step order.
+ * fx/fy receive up to max_steps*n_slots values (step-major); *n_steps the count. */
+int lbm_forces(lbm_handle h, int32_t lattice, double* fx, double* fy, int32_t max_steps,
+               int32_t* n_steps);
+/* FP64 mass: sum of all populations over non-solid nodes of the current state. */
+int lbm_mass(lbm_handle h, int32_t lattice, double* mass);
+/* Nonzero when a non-finite density appeared since the last call. */
+int lbm_nan_flag(lbm_handle h, int32_t lattice, int32_t* flag);
+/* Steps taken by a lattice since its last upload. */
+int lbm_steps_done(lbm_handle h, int32_t lattice, int64_t* steps);
+int lbm_get_info(lbm_handle h, lbm_info* out);
+/* Waits for all work of the handle. */
+int lbm_sync(lbm_handle h);
+
+/* Wall distance used by the LES (src/lbm_types.hpp:231-264), computed once per
+ * geometry: out[nx*ny] in lattice units, 1e9 where no solid is reachable. */
+int lbm_wall_distance(lbm_handle h, int32_t lattice, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_GPU_H */

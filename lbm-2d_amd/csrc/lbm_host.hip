@@ -1,0 +1,1226 @@
+// lbm_host.hip -- host runtime behind the C ABI of include/lbm_gpu.h.
+//
+// Owns the device pools of an ensemble of lattices:
+//   populations  two buffers (A-B), SoA planes of pitch-aligned rows per lattice
+//   flags        one byte per node (solid / special / links / write)
+//   link_slot    int32 per node: index of the node's boundary-link record
+//   links        one LinkRec per fluid node with links (q, scheme, wall velocity, slot)
+//   wdist        uint16 wall distance per node (LES lattices)
+//   forces       FP64 accumulators [2][slots] and a per-slot history ring
+// and builds them from the reference's obstacle/geometry description
+// (src/lbm_types.hpp:90-182, src/lbm.hpp:79-87, 931-964).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_gpu.h"
+#include "lbm_internal.h"
+#include "lbm_kernels.h"
+
+namespace lbm {
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+#define LBM_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(LBM_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr double W[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
+                         1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+constexpr int kHistoryRows = 4096;
+constexpr int64_t kStagingBytes = 256ll << 20;
+
+// q along link i from fluid node (xf,yf) to a circle: first root t in (0,1] of the
+// ray/circle intersection, 1.0 when the ray misses (src/lbm_types.hpp:118-138).
+double q_circle(double ccx, double ccy, double r, double xf, double yf, int i)
+{
+    const double dx = xf - ccx, dy = yf - ccy;
+    const double ex = CX[i], ey = CY[i];
+    const double a = ex * ex + ey * ey;
+    const double b = 2.0 * (dx * ex + dy * ey);
+    const double c = dx * dx + dy * dy - r * r;
+    const double disc = b * b - 4.0 * a * c;
+    if (disc < 0.0) return 1.0;
+    const double sd = std::sqrt(disc);
+    const double t1 = (-b - sd) / (2.0 * a), t2 = (-b + sd) / (2.0 * a);
+    if (t1 > 0.0 && t1 <= 1.0) return t1;
+    if (t2 > 0.0 && t2 <= 1.0) return t2;
+    return 1.0;
+}
+
+// q to a closed polygon: smallest t in (0,1] over the edges crossed by the link
+// segment, near-parallel edges skipped (src/lbm_types.hpp:143-170).
+double q_polygon(const std::vector<double>& xy, double xf, double yf, int i)
+{
+    const int n = static_cast<int>(xy.size() / 2);
+    if (n < 3) return 1.0;
+    const double ex = CX[i], ey = CY[i];
+    double best = 1.0;
+    for (int v = 0; v < n; ++v) {
+        const int w = (v + 1) % n;
+        const double x1 = xy[2 * v], y1 = xy[2 * v + 1];
+        const double sx = xy[2 * w] - x1, sy = xy[2 * w + 1] - y1;
+        const double den = ex * sy - ey * sx;
+        if (std::fabs(den) < 1e-15) continue;
+        const double t = ((x1 - xf) * sy - (y1 - yf) * sx) / den;
+        const double s = ((x1 - xf) * ey - (y1 - yf) * ex) / den;
+        if (t > 0.0 && t <= best && s >= 0.0 && s <= 1.0) best = t;
+    }
+    return best;
+}
+
+struct HostLattice {
+    lbm_desc d{};
+    int32_t pitch = 0;
+    int64_t f_off = 0, cell_off = 0;
+    int32_t link_off = 0, slot_base = 0, n_slots = 1;
+    // geometry description
+    std::vector<uint8_t> obstacle;
+    int bb_kind = 0;  // 0 none, 1 circle, 2 polygon
+    double bcx = 0, bcy = 0, br = 0;
+    std::vector<double> poly;
+    int bb_scheme = 1;
+    bool moving = false;
+    double omega = 0, rcx = 0, rcy = 0;
+    bool frad = false;
+    double fcx = 0, fcy = 0, fr = 0;
+    std::vector<lbm_link> explicit_links;
+    bool geom_dirty = true;
+    // built geometry
+    std::vector<LinkRec> recs;
+    std::vector<uint8_t> flags;      // pitch layout
+    std::vector<int32_t> lslot;      // pitch layout
+    std::vector<uint16_t> wdist;     // pitch layout (LES only)
+    std::vector<double> wall_dist;   // nx*ny (LES only)
+    int64_t fluid = 0;
+    bool uses_bouzidi = false;
+    // run state
+    bool uploaded = false;
+    int64_t steps = 0;
+    int32_t par = 0;
+    int64_t read_pos = 0;
+    std::vector<double> pend_fx, pend_fy;
+};
+
+}  // namespace
+
+struct Ensemble {
+    int storage = ST_F32;
+    int scale_exp = 0;
+    float scale = 1.f, inv_scale = 1.f;
+    int bpp = 4;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<HostLattice> lat;
+    int64_t pool_elems = 0, cell_elems = 0;
+    void* buf[2] = {nullptr, nullptr};
+    uint8_t* d_flags = nullptr;
+    int32_t* d_lslot = nullptr;
+    uint16_t* d_wdist = nullptr;
+    LinkRec* d_links = nullptr;
+    size_t links_cap = 0;
+    LatticeDev* d_lat = nullptr;
+    Tile* d_tiles = nullptr;
+    size_t tiles_cap = 0;
+    int n_tiles = 0;
+    int64_t* d_steps = nullptr;
+    int32_t* d_par = nullptr;
+    double* d_acc = nullptr;
+    double* d_hist = nullptr;
+    double* d_dummy_acc = nullptr;
+    int32_t* d_nan = nullptr;
+    int32_t* d_dummy_nan = nullptr;
+    double* d_mass = nullptr;
+    double* d_stage = nullptr;
+    int total_slots = 0;
+    int R = kHistoryRows;
+    int bx = 256, rows = 1, graph_steps = 100;
+    bool geom_dirty = true, desc_dirty = true, tiles_dirty = true, counters_stale = false;
+    hipGraphExec_t graph = nullptr;
+    bool graph_dirty = true;
+    int64_t device_bytes = 0;
+};
+
+namespace {
+
+int lattice_ok(Ensemble* E, int l)
+{
+    if (!E) return fail(LBM_ERR_INVALID, "null handle");
+    if (l < 0 || l >= static_cast<int>(E->lat.size())) return fail(LBM_ERR_INVALID, "lattice index out of range");
+    return LBM_OK;
+}
+
+template <typename T> int dev_alloc(Ensemble* E, T** p, size_t count)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LBM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    E->device_bytes += static_cast<int64_t>(count * sizeof(T));
+    return LBM_OK;
+}
+
+// Host-to-device copy ordered on the handle's stream, complete on return.
+int h2d(Ensemble* E, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return LBM_OK;
+    LBM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, E->stream));
+    LBM_HIP(hipStreamSynchronize(E->stream));
+    return LBM_OK;
+}
+
+int d2h(Ensemble* E, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return LBM_OK;
+    LBM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, E->stream));
+    LBM_HIP(hipStreamSynchronize(E->stream));
+    return LBM_OK;
+}
+
+// Multi-source BFS over the 4-neighbour graph from every solid node, non-periodic
+// (the fixpoint of the relaxation in src/lbm_types.hpp:231-264); 1e9 = unreachable.
+void wall_distance(const std::vector<uint8_t>& obs, int nx, int ny, std::vector<double>& d)
+{
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    d.assign(n, 1e9);
+    std::vector<int64_t> q;
+    q.reserve(n);
+    for (int64_t c = 0; c < n; ++c)
+        if (obs[c]) {
+            d[c] = 0.0;
+            q.push_back(c);
+        }
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int64_t c = q[h];
+        const int x = static_cast<int>(c % nx), y = static_cast<int>(c / nx);
+        const int nbx[4] = {x - 1, x + 1, x, x}, nby[4] = {y, y, y - 1, y + 1};
+        for (int k = 0; k < 4; ++k) {
+            if (nbx[k] < 0 || nbx[k] >= nx || nby[k] < 0 || nby[k] >= ny) continue;
+            const int64_t m = static_cast<int64_t>(nby[k]) * nx + nbx[k];
+            if (d[m] > d[c] + 1.0) {
+                d[m] = d[c] + 1.0;
+                q.push_back(m);
+            }
+        }
+    }
+}
+
+// Neighbour of (x,y) along e_i under the lattice's edge rules.  Returns false when
+// the population leaves through an OPEN or WALL edge (no boundary link then).
+bool neighbour(const lbm_desc& d, int x, int y, int i, int& tx, int& ty)
+{
+    tx = x + CX[i];
+    ty = y + CY[i];
+    if (tx < 0 || tx >= d.nx) {
+        if (d.x_edge != LBM_EDGE_PERIODIC) return false;
+        tx = tx < 0 ? tx + d.nx : tx - d.nx;
+    }
+    if (ty < 0 || ty >= d.ny) {
+        if (d.y_edge != LBM_EDGE_PERIODIC) return false;
+        ty = ty < 0 ? ty + d.ny : ty - d.ny;
+    }
+    return true;
+}
+
+// Flags, link records and wall distance of one lattice from its description.
+int build_geometry(HostLattice& H)
+{
+    const lbm_desc& d = H.d;
+    const int nx = d.nx, ny = d.ny;
+    if (H.obstacle.empty()) H.obstacle.assign(static_cast<size_t>(nx) * ny, 0);
+    H.flags.assign(static_cast<size_t>(ny) * H.pitch, 0);
+    H.lslot.assign(static_cast<size_t>(ny) * H.pitch, -1);
+    H.recs.clear();
+    H.fluid = 0;
+    std::vector<int32_t> rec_of(static_cast<size_t>(nx) * ny, -1);
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            const int64_t c = static_cast<int64_t>(y) * nx + x;
+            const int64_t p = static_cast<int64_t>(y) * H.pitch + x;
+            uint8_t fl = 0;
+            const bool edge = x == 0 || x == nx - 1 || y == 0 || y == ny - 1;
+            if (H.obstacle[c]) {
+                fl = CELL_SOLID | CELL_SPECIAL;
+            } else {
+                ++H.fluid;
+                uint32_t mask = 0;
+                for (int i = 1; i < 9; ++i) {
+                    int tx, ty;
+                    if (neighbour(d, x, y, i, tx, ty) && H.obstacle[static_cast<int64_t>(ty) * nx + tx])
+                        mask |= 1u << i;
+                }
+                if (edge) fl |= CELL_SPECIAL;
+                if (mask) {
+                    fl |= CELL_SPECIAL | CELL_LINKS;
+                    LinkRec r{};
+                    r.mask = mask;
+                    r.scheme = 0;
+                    for (int i = 0; i < 9; ++i) {
+                        r.slot[i] = 0;
+                        r.q[i] = 0.5;
+                        r.uwx[i] = r.uwy[i] = 0.0;
+                    }
+                    rec_of[c] = static_cast<int32_t>(H.recs.size());
+                    H.lslot[p] = rec_of[c];
+                    H.recs.push_back(r);
+                }
+            }
+            H.flags[p] = fl;
+        }
+    auto each_link = [&](auto&& fn) {
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const int32_t ri = rec_of[static_cast<int64_t>(y) * nx + x];
+                if (ri < 0) continue;
+                LinkRec& r = H.recs[ri];
+                for (int i = 1; i < 9; ++i)
+                    if (r.mask & (1u << i)) fn(x, y, i, r);
+            }
+    };
+    if (H.bb_kind != 0) {
+        each_link([&](int x, int y, int i, LinkRec& r) {
+            r.q[i] = H.bb_kind == 1 ? q_circle(H.bcx, H.bcy, H.br, x, y, i) : q_polygon(H.poly, x, y, i);
+            r.scheme = (r.scheme & ~(3u << (2 * i))) | (static_cast<uint32_t>(H.bb_scheme) << (2 * i));
+        });
+    }
+    if (H.moving) {
+        // compute_wall_velocity at the (unwrapped) solid node (src/lbm_types.hpp:106-114, lbm.hpp:244)
+        each_link([&](int x, int y, int i, LinkRec& r) {
+            const double px = x + CX[i], py = y + CY[i];
+            r.uwx[i] = -H.omega * (py - H.rcy);
+            r.uwy[i] = H.omega * (px - H.rcx);
+        });
+    }
+    if (H.frad) {
+        // keep links whose (wrapped) solid node is within r of the centre (src/lbm.hpp:569-573)
+        each_link([&](int x, int y, int i, LinkRec& r) {
+            int tx, ty;
+            neighbour(d, x, y, i, tx, ty);
+            const double dx = tx - H.fcx, dy = ty - H.fcy;
+            if (std::sqrt(dx * dx + dy * dy) > H.fr) r.slot[i] = -1;
+        });
+    }
+    for (const lbm_link& k : H.explicit_links) {
+        if (k.cell < 0 || k.cell >= static_cast<int64_t>(nx) * ny || k.dir < 1 || k.dir > 8)
+            return fail(LBM_ERR_INVALID, "link cell/dir out of range");
+        const int32_t ri = rec_of[k.cell];
+        if (ri < 0 || !(H.recs[ri].mask & (1u << k.dir)))
+            return fail(LBM_ERR_INVALID, "link " + std::to_string(k.cell) + "/" + std::to_string(k.dir) +
+                                             " is not a fluid-to-solid link of the obstacle mask");
+        if (k.scheme < 0 || k.scheme > 2) return fail(LBM_ERR_INVALID, "bad link scheme");
+        if (k.slot >= H.n_slots) return fail(LBM_ERR_INVALID, "link slot beyond the lattice's force slots");
+        LinkRec& r = H.recs[ri];
+        r.q[k.dir] = k.q;
+        r.uwx[k.dir] = k.uwx;
+        r.uwy[k.dir] = k.uwy;
+        r.slot[k.dir] = static_cast<int16_t>(k.slot);
+        r.scheme = (r.scheme & ~(3u << (2 * k.dir))) | (static_cast<uint32_t>(k.scheme) << (2 * k.dir));
+    }
+    H.uses_bouzidi = false;
+    for (const LinkRec& r : H.recs)
+        for (int i = 1; i < 9; ++i)
+            if ((r.mask & (1u << i)) && ((r.scheme >> (2 * i)) & 3) == LBM_BB_BOUZIDI) H.uses_bouzidi = true;
+    if (H.uses_bouzidi) {
+        // Bouzidi reads the stored value of upstream nodes, which may be solid
+        for (size_t p = 0; p < H.flags.size(); ++p)
+            if (H.flags[p] & CELL_SOLID) H.flags[p] |= CELL_WRITE;
+    }
+    if (d.les) {
+        wall_distance(H.obstacle, nx, ny, H.wall_dist);
+        H.wdist.assign(static_cast<size_t>(ny) * H.pitch, 0xffff);
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const double v = H.wall_dist[static_cast<int64_t>(y) * nx + x];
+                H.wdist[static_cast<int64_t>(y) * H.pitch + x] =
+                    v >= 65535.0 ? 0xffff : static_cast<uint16_t>(v);
+            }
+    } else {
+        H.wdist.clear();
+        H.wall_dist.clear();
+    }
+    H.geom_dirty = false;
+    return LBM_OK;
+}
+
+LatticeDev make_dev(const HostLattice& H, int index)
+{
+    LatticeDev L{};
+    const lbm_desc& d = H.d;
+    L.nx = d.nx;
+    L.ny = d.ny;
+    L.pitch = H.pitch;
+    L.lattice = index;
+    L.f_off = H.f_off;
+    L.plane = static_cast<int64_t>(d.ny) * H.pitch;
+    L.cell_off = H.cell_off;
+    L.link_off = H.link_off;
+    L.slot_base = H.slot_base;
+    L.n_slots = H.n_slots;
+    L.collision = d.collision;
+    L.les = d.les;
+    L.x_edge = d.x_edge;
+    L.y_edge = d.y_edge;
+    L.inlet = d.inlet;
+    L.inlet_y0 = d.inlet_y0;
+    L.inlet_h = d.inlet_h;
+    L.outlet = d.outlet;
+    L.lid = d.lid;
+    L.forces = d.forces;
+    // MRTParams::from_tau (src/lbm_types.hpp:72-76)
+    double s = 1.0 / d.tau;
+    L.s_shear = s < 0.5 ? 0.5 : (s > 1.99 ? 1.99 : s);
+    L.s_bulk = 1.2;
+    L.s_normal = 1.0;
+    L.tau = d.tau;
+    L.inv_tau = 1.0 / d.tau;
+    L.u_in = d.u_in;
+    L.cs2 = d.les ? d.cs * d.cs : 0.0;
+    L.nu = (d.tau - 0.5) / 3.0;
+    // Fscale (src/lbm.hpp:296-297)
+    L.fscale = d.body_force_x != 0.0 ? (1.0 - 1.0 / (2.0 * d.tau)) * 3.0 * d.body_force_x : 0.0;
+    return L;
+}
+
+// Rebuilds the geometry pools, slot layout and descriptors that changed.
+int commit(Ensemble* E)
+{
+    int rc;
+    if (E->geom_dirty) {
+        bool any = false;
+        for (auto& H : E->lat)
+            if (H.geom_dirty) {
+                if ((rc = build_geometry(H)) != LBM_OK) return rc;
+                any = true;
+            }
+        if (any) {
+            size_t nrec = 0;
+            int slots = 0;
+            for (auto& H : E->lat) {
+                H.link_off = static_cast<int32_t>(nrec);
+                nrec += H.recs.size();
+                H.slot_base = slots;
+                slots += H.n_slots;
+            }
+            if (nrec > E->links_cap) {
+                if ((rc = dev_alloc(E, &E->d_links, nrec)) != LBM_OK) return rc;
+                E->links_cap = nrec;
+                E->graph_dirty = true;
+            }
+            std::vector<LinkRec> all;
+            all.reserve(nrec);
+            std::vector<uint8_t> flags(E->cell_elems, 0);
+            std::vector<int32_t> lslot(E->cell_elems, -1);
+            std::vector<uint16_t> wd(E->cell_elems, 0xffff);
+            for (auto& H : E->lat) {
+                all.insert(all.end(), H.recs.begin(), H.recs.end());
+                std::copy(H.flags.begin(), H.flags.end(), flags.begin() + H.cell_off);
+                std::copy(H.lslot.begin(), H.lslot.end(), lslot.begin() + H.cell_off);
+                if (!H.wdist.empty()) std::copy(H.wdist.begin(), H.wdist.end(), wd.begin() + H.cell_off);
+            }
+            if ((rc = h2d(E, E->d_links, all.data(), all.size() * sizeof(LinkRec))) != LBM_OK) return rc;
+            if ((rc = h2d(E, E->d_flags, flags.data(), flags.size())) != LBM_OK) return rc;
+            if ((rc = h2d(E, E->d_lslot, lslot.data(), lslot.size() * 4)) != LBM_OK) return rc;
+            if ((rc = h2d(E, E->d_wdist, wd.data(), wd.size() * 2)) != LBM_OK) return rc;
+            if (slots != E->total_slots) {
+                E->total_slots = slots;
+                if ((rc = dev_alloc(E, &E->d_acc, static_cast<size_t>(slots) * 4)) != LBM_OK) return rc;
+                if ((rc = dev_alloc(E, &E->d_dummy_acc, static_cast<size_t>(slots) * 4)) != LBM_OK) return rc;
+                if ((rc = dev_alloc(E, &E->d_hist, static_cast<size_t>(slots) * E->R * 2)) != LBM_OK) return rc;
+                LBM_HIP(hipMemsetAsync(E->d_acc, 0, static_cast<size_t>(slots) * 4 * 8, E->stream));
+                LBM_HIP(hipMemsetAsync(E->d_hist, 0, static_cast<size_t>(slots) * E->R * 16, E->stream));
+                E->graph_dirty = true;
+            }
+        }
+        E->geom_dirty = false;
+        E->desc_dirty = true;
+    }
+    if (E->desc_dirty) {
+        std::vector<LatticeDev> devs;
+        for (size_t l = 0; l < E->lat.size(); ++l) devs.push_back(make_dev(E->lat[l], static_cast<int>(l)));
+        if ((rc = h2d(E, E->d_lat, devs.data(), devs.size() * sizeof(LatticeDev))) != LBM_OK) return rc;
+        E->desc_dirty = false;
+    }
+    if (E->tiles_dirty && E->lat.size() > 1) {
+        std::vector<Tile> tiles;
+        for (size_t l = 0; l < E->lat.size(); ++l) {
+            const lbm_desc& d = E->lat[l].d;
+            for (int y0 = 0; y0 < d.ny; y0 += E->rows)
+                for (int x0 = 0; x0 < d.nx; x0 += E->bx) tiles.push_back(Tile{static_cast<int32_t>(l), y0, x0, 0});
+        }
+        if (tiles.size() > E->tiles_cap) {
+            if ((rc = dev_alloc(E, &E->d_tiles, tiles.size())) != LBM_OK) return rc;
+            E->tiles_cap = tiles.size();
+        }
+        if ((rc = h2d(E, E->d_tiles, tiles.data(), tiles.size() * sizeof(Tile))) != LBM_OK) return rc;
+        E->n_tiles = static_cast<int>(tiles.size());
+        E->tiles_dirty = false;
+        E->graph_dirty = true;
+    }
+    if (E->counters_stale) {
+        std::vector<int64_t> st;
+        std::vector<int32_t> pr;
+        for (auto& H : E->lat) {
+            st.push_back(H.steps);
+            pr.push_back(H.par);
+        }
+        if ((rc = h2d(E, E->d_steps, st.data(), st.size() * 8)) != LBM_OK) return rc;
+        if ((rc = h2d(E, E->d_par, pr.data(), pr.size() * 4)) != LBM_OK) return rc;
+        E->counters_stale = false;
+    }
+    return LBM_OK;
+}
+
+StepArgs base_args(Ensemble* E)
+{
+    StepArgs a{};
+    a.lat = E->d_lat;
+    a.buf0 = E->buf[0];
+    a.buf1 = E->buf[1];
+    a.flags = E->d_flags;
+    a.link_slot = E->d_lslot;
+    a.links = E->d_links;
+    a.wdist = E->d_wdist;
+    a.steps = E->d_steps;
+    a.par = E->d_par;
+    a.acc = E->d_acc;
+    a.hist = E->d_hist;
+    a.R = E->R;
+    a.total_slots = E->total_slots;
+    a.nan_flags = E->d_nan;
+    a.scale = E->scale;
+    a.inv_scale = E->inv_scale;
+    return a;
+}
+
+// Grid for one launch over every lattice (ensemble) or over one lattice.
+void grid_for(Ensemble* E, int single, StepArgs& a, int& blocks)
+{
+    if (single >= 0 || E->lat.size() == 1) {
+        const int l = single >= 0 ? single : 0;
+        const lbm_desc& d = E->lat[l].d;
+        a.tiles = nullptr;
+        a.single = l;
+        a.tiles_x = (d.nx + E->bx - 1) / E->bx;
+        a.tiles_y = (d.ny + E->rows - 1) / E->rows;
+        blocks = a.tiles_x * a.tiles_y;
+    } else {
+        a.tiles = E->d_tiles;
+        a.n_tiles = E->n_tiles;
+        blocks = E->n_tiles;
+    }
+}
+
+// Copies the flushed history rows [read_pos, steps-1) of lattice l to the host.
+int drain(Ensemble* E, int l)
+{
+    HostLattice& H = E->lat[l];
+    const int64_t last = H.steps - 1;  // newest step still in the accumulator
+    if (last <= H.read_pos) return LBM_OK;
+    LBM_HIP(hipStreamSynchronize(E->stream));
+    std::vector<double> ring(static_cast<size_t>(H.n_slots) * E->R * 2);
+    int rc = d2h(E, ring.data(), E->d_hist + static_cast<int64_t>(H.slot_base) * E->R * 2, ring.size() * 8);
+    if (rc != LBM_OK) return rc;
+    for (int64_t k = H.read_pos; k < last; ++k)
+        for (int s = 0; s < H.n_slots; ++s) {
+            const int64_t o = (static_cast<int64_t>(s) * E->R + k % E->R) * 2;
+            H.pend_fx.push_back(ring[o]);
+            H.pend_fy.push_back(ring[o + 1]);
+        }
+    H.read_pos = last;
+    return LBM_OK;
+}
+
+int ensure_history_room(Ensemble* E, int l, int64_t n)
+{
+    HostLattice& H = E->lat[l];
+    if (H.steps + n - 1 - H.read_pos > E->R - 2) return drain(E, l);
+    return LBM_OK;
+}
+
+int launch_batch(Ensemble* E, int n)
+{
+    StepArgs a = base_args(E);
+    int blocks;
+    grid_for(E, -1, a, blocks);
+    a.flush_begin = 0;
+    a.flush_count = static_cast<int>(E->lat.size());
+    for (int i = 0; i < n; ++i) {
+        a.step_arg = i;
+        LBM_HIP(launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream));
+    }
+    LBM_HIP(launch_add_steps(E->d_steps, E->d_par, 0, static_cast<int>(E->lat.size()), n, E->stream));
+    return LBM_OK;
+}
+
+int build_graph(Ensemble* E)
+{
+    if (E->graph) {
+        (void)hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    hipGraph_t g;
+    LBM_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_batch(E, E->graph_steps);
+    hipError_t e = hipStreamEndCapture(E->stream, &g);
+    if (rc != LBM_OK) return rc;
+    if (e != hipSuccess) return fail(LBM_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(LBM_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    E->graph_dirty = false;
+    return LBM_OK;
+}
+
+int require_uploaded(Ensemble* E)
+{
+    for (size_t l = 0; l < E->lat.size(); ++l)
+        if (!E->lat[l].uploaded)
+            return fail(LBM_ERR_INVALID, "lattice " + std::to_string(l) + " has no state (call lbm_upload)");
+    return LBM_OK;
+}
+
+// Runs the state kernel over lattice l in row chunks; out may be null (mass only).
+int materialize(Ensemble* E, int l, double* host_out, double* mass_out)
+{
+    HostLattice& H = E->lat[l];
+    const lbm_desc& d = H.d;
+    int rc;
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    const double zero = 0.0;
+    if ((rc = h2d(E, E->d_mass, &zero, 8)) != LBM_OK) return rc;
+    const int64_t row_bytes = static_cast<int64_t>(d.nx) * 9 * 8;
+    const int chunk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(d.ny, kStagingBytes / row_bytes)));
+    const bool from_state = H.steps == 0;
+    // state = buf[1-par] after upload; C_{k-1} = buf[1-par] after k >= 1 steps
+    const void* src = E->buf[1 - H.par];
+    for (int y0 = 0; y0 < d.ny; y0 += chunk) {
+        const int y1 = std::min(d.ny, y0 + chunk);
+        StateArgs s{};
+        s.lat = E->d_lat;
+        s.lattice = l;
+        s.y0 = y0;
+        s.y1 = y1;
+        s.from_state = from_state ? 1 : 0;
+        s.src = src;
+        s.flags = E->d_flags;
+        s.link_slot = E->d_lslot;
+        s.links = E->d_links;
+        s.out = E->d_stage;
+        s.mass_dev = E->d_mass;
+        s.scale = E->scale;
+        s.inv_scale = E->inv_scale;
+        LBM_HIP(launch_state(E->storage, s, d.nx, E->stream));
+        if (host_out) {
+            rc = d2h(E, host_out + static_cast<int64_t>(y0) * d.nx * 9, E->d_stage,
+                     static_cast<size_t>(y1 - y0) * row_bytes);
+            if (rc != LBM_OK) return rc;
+        }
+    }
+    if (mass_out) {
+        double m;
+        if ((rc = d2h(E, &m, E->d_mass, 8)) != LBM_OK) return rc;
+        *mass_out = static_cast<double>(H.fluid) + m;
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+}  // namespace lbm
+
+using lbm::Ensemble;
+using lbm::fail;
+
+struct lbm_ensemble : Ensemble {};
+
+extern "C" {
+
+const char* lbm_last_error(void) { return lbm::g_error.c_str(); }
+
+int lbm_desc_for_case(int case_type, int32_t nx, int32_t ny, lbm_desc* out)
+{
+    if (!out) return fail(LBM_ERR_INVALID, "null desc");
+    if (case_type < 0 || case_type > 13) return fail(LBM_ERR_INVALID, "unknown case type");
+    lbm_desc d{};
+    d.nx = nx;
+    d.ny = ny;
+    d.collision = LBM_MRT;
+    d.les = 0;
+    d.cs = 0.12;  // g_cs default (src/lbm_types.hpp:32)
+    d.tau = 1.0;
+    d.u_in = 0.1;
+    enum { CYLINDER = 0, CAVITY = 1, STEP = 2, RIBS = 3, PERIODIC_HILLS = 9 };
+    if (case_type == CAVITY) {
+        // src/lbm.hpp:408-433 stream, :531-532 lid, :543 no forces
+        d.x_edge = d.y_edge = LBM_EDGE_WALL;
+        d.lid = 1;
+    } else if (case_type == RIBS || case_type == PERIODIC_HILLS) {
+        // src/lbm.hpp:434-456 fully periodic, no open boundaries
+        d.x_edge = d.y_edge = LBM_EDGE_PERIODIC;
+        d.forces = 1;
+    } else {
+        // src/lbm.hpp:457-484 periodic y, open x; inlet + outflow (:533-539)
+        d.x_edge = LBM_EDGE_OPEN;
+        d.y_edge = LBM_EDGE_PERIODIC;
+        d.outlet = 1;
+        d.forces = 1;
+        if (case_type == STEP) {
+            d.inlet = LBM_INLET_PARABOLIC;
+            d.inlet_y0 = ny / 3;
+            d.inlet_h = ny - 1 - ny / 3;
+        } else {
+            d.inlet = LBM_INLET_ZOU_HE;
+        }
+    }
+    *out = d;
+    return LBM_OK;
+}
+
+int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_exp, void* stream,
+               lbm_handle* out)
+{
+    using namespace lbm;
+    if (!descs || n <= 0 || !out) return fail(LBM_ERR_INVALID, "lbm_create: bad arguments");
+    if (storage < ST_F64 || storage > ST_BF16S) return fail(LBM_ERR_INVALID, "unknown storage form");
+    if ((storage == ST_F16S || storage == ST_BF16S) && (scale_exp < 0 || scale_exp > 14))
+        return fail(LBM_ERR_INVALID, "scale exponent must be in [0,14]");
+    for (int l = 0; l < n; ++l) {
+        const lbm_desc& d = descs[l];
+        if (d.nx < 3 || d.ny < 3) return fail(LBM_ERR_INVALID, "lattice must be at least 3x3");
+        if (static_cast<int64_t>(d.nx) * d.ny > (1ll << 31)) return fail(LBM_ERR_INVALID, "lattice too large");
+        if (d.collision < 0 || d.collision > 1 || d.x_edge < 0 || d.x_edge > 2 || d.y_edge < 0 || d.y_edge > 2 ||
+            d.inlet < 0 || d.inlet > 2)
+            return fail(LBM_ERR_INVALID, "bad enum in lbm_desc");
+        if (d.tau <= 0.0) return fail(LBM_ERR_INVALID, "tau must be positive");
+        if (d.inlet == LBM_INLET_PARABOLIC && d.inlet_h <= 0)
+            return fail(LBM_ERR_INVALID, "parabolic inlet needs inlet_h > 0");
+    }
+    auto* E = new lbm_ensemble();
+    E->storage = storage;
+    E->bpp = storage == ST_F64 ? 8 : (storage == ST_F32 ? 4 : 2);
+    E->scale_exp = (storage == ST_F16S || storage == ST_BF16S) ? scale_exp : 0;
+    E->scale = std::ldexp(1.0f, E->scale_exp);
+    E->inv_scale = std::ldexp(1.0f, -E->scale_exp);
+    if (stream) {
+        E->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreate(&E->stream) != hipSuccess) {
+            delete E;
+            return fail(LBM_ERR_DEVICE, "hipStreamCreate failed");
+        }
+        E->own_stream = true;
+    }
+    int64_t pool = 0, cells = 0;
+    E->lat.resize(n);
+    for (int l = 0; l < n; ++l) {
+        HostLattice& H = E->lat[l];
+        H.d = descs[l];
+        H.pitch = (descs[l].nx + 63) / 64 * 64;
+        H.f_off = pool;
+        const int64_t plane = static_cast<int64_t>(descs[l].ny) * H.pitch;
+        pool += (9 * plane + 255) / 256 * 256;
+        H.cell_off = cells;
+        cells += (plane + 255) / 256 * 256;
+    }
+    E->pool_elems = pool;
+    E->cell_elems = cells;
+    int rc = LBM_OK;
+    for (int b = 0; b < 2 && rc == LBM_OK; ++b) {
+        hipError_t e = hipMalloc(&E->buf[b], static_cast<size_t>(pool) * E->bpp);
+        if (e != hipSuccess) rc = fail(LBM_ERR_NOMEM, "population buffers: " + std::string(hipGetErrorString(e)));
+        else E->device_bytes += pool * E->bpp;
+    }
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_flags, cells);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_lslot, cells);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_wdist, cells);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_lat, n);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_steps, n);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_par, n);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_nan, n);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_dummy_nan, n);
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_mass, 1);
+    // staging for row chunks of AoS FP64 state: at most kStagingBytes or one row
+    int64_t stage = 0;
+    for (auto& H : E->lat) {
+        const int64_t row = static_cast<int64_t>(H.d.nx) * 72;
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(H.d.ny, kStagingBytes / row));
+        stage = std::max<int64_t>(stage, chunk * row);
+    }
+    if (rc == LBM_OK) rc = dev_alloc(E, &E->d_stage, static_cast<size_t>(stage / 8 + 1));
+    if (rc == LBM_OK) {
+        std::vector<int32_t> z(n, 0);
+        std::vector<int64_t> z64(n, 0);
+        rc = lbm::h2d(E, E->d_nan, z.data(), n * 4);
+        if (rc == LBM_OK) rc = lbm::h2d(E, E->d_dummy_nan, z.data(), n * 4);
+        if (rc == LBM_OK) rc = lbm::h2d(E, E->d_par, z.data(), n * 4);
+        if (rc == LBM_OK) rc = lbm::h2d(E, E->d_steps, z64.data(), n * 8);
+    }
+    if (rc != LBM_OK) {
+        std::string msg = lbm::g_error;
+        lbm_destroy(E);
+        return fail(rc, msg);
+    }
+    *out = E;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_handle h)
+{
+    if (!h) return LBM_OK;
+    Ensemble* E = h;
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    void* ptrs[] = {E->buf[0], E->buf[1], E->d_flags, E->d_lslot, E->d_wdist, E->d_links, E->d_lat,
+                    E->d_tiles, E->d_steps, E->d_par, E->d_acc, E->d_hist, E->d_dummy_acc, E->d_nan,
+                    E->d_dummy_nan, E->d_mass, E->d_stage};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (E->own_stream) (void)hipStreamDestroy(E->stream);
+    delete h;
+    return LBM_OK;
+}
+
+int lbm_set_obstacle(lbm_handle h, int32_t l, const uint8_t* mask)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!mask) return fail(LBM_ERR_INVALID, "null mask");
+    auto& H = h->lat[l];
+    const size_t n = static_cast<size_t>(H.d.nx) * H.d.ny;
+    H.obstacle.resize(n);
+    for (size_t c = 0; c < n; ++c) H.obstacle[c] = mask[c] ? 1 : 0;
+    H.bb_kind = 0;
+    H.moving = false;
+    H.frad = false;
+    H.explicit_links.clear();
+    H.geom_dirty = true;
+    h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_bb_circle(lbm_handle h, int32_t l, double cx, double cy, double r, int32_t scheme)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (scheme < 0 || scheme > 2) return fail(LBM_ERR_INVALID, "bad scheme");
+    auto& H = h->lat[l];
+    H.bb_kind = 1;
+    H.bcx = cx;
+    H.bcy = cy;
+    H.br = r;
+    H.bb_scheme = scheme;
+    H.geom_dirty = h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_bb_polygon(lbm_handle h, int32_t l, const double* xy, int32_t n, int32_t scheme)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!xy || n < 0 || scheme < 0 || scheme > 2) return fail(LBM_ERR_INVALID, "bad polygon arguments");
+    auto& H = h->lat[l];
+    H.bb_kind = 2;
+    H.poly.assign(xy, xy + 2 * n);
+    H.bb_scheme = scheme;
+    H.geom_dirty = h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_moving_wall(lbm_handle h, int32_t l, double omega, double rcx, double rcy)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    auto& H = h->lat[l];
+    H.moving = true;
+    H.omega = omega;
+    H.rcx = rcx;
+    H.rcy = rcy;
+    H.geom_dirty = h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_force_radius(lbm_handle h, int32_t l, double cx, double cy, double r)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    auto& H = h->lat[l];
+    H.frad = true;
+    H.fcx = cx;
+    H.fcy = cy;
+    H.fr = r;
+    H.geom_dirty = h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_links(lbm_handle h, int32_t l, const lbm_link* links, int32_t n)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !links)) return fail(LBM_ERR_INVALID, "bad links");
+    auto& H = h->lat[l];
+    H.explicit_links.assign(links, links + n);
+    H.geom_dirty = h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_force_slots(lbm_handle h, int32_t l, int32_t n_slots)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (n_slots < 1 || n_slots > 32767) return fail(LBM_ERR_INVALID, "n_slots out of range");
+    auto& H = h->lat[l];
+    if (H.steps > 0 || H.uploaded) return fail(LBM_ERR_INVALID, "set force slots before lbm_upload");
+    H.n_slots = n_slots;
+    H.geom_dirty = h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_params(lbm_handle h, int32_t l, double tau, double u_in)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!(tau > 0.0)) return fail(LBM_ERR_INVALID, "tau must be positive");
+    auto& H = h->lat[l];
+    if (H.d.tau != tau || H.d.u_in != u_in) {
+        H.d.tau = tau;
+        H.d.u_in = u_in;
+        h->desc_dirty = true;
+    }
+    return LBM_OK;
+}
+
+int lbm_upload(lbm_handle h, int32_t l, const double* f)
+{
+    using namespace lbm;
+    int rc = lattice_ok(h, l);
+    if (rc) return rc;
+    if (!f) return fail(LBM_ERR_INVALID, "null population array");
+    Ensemble* E = h;
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    HostLattice& H = E->lat[l];
+    const lbm_desc& d = H.d;
+    const int64_t row_bytes = static_cast<int64_t>(d.nx) * 72;
+    const int chunk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(d.ny, kStagingBytes / row_bytes)));
+    for (int y0 = 0; y0 < d.ny; y0 += chunk) {
+        const int y1 = std::min(d.ny, y0 + chunk);
+        if ((rc = h2d(E, E->d_stage, f + static_cast<int64_t>(y0) * d.nx * 9, (y1 - y0) * row_bytes)) != LBM_OK)
+            return rc;
+        PrologueArgs p{};
+        p.lat = E->d_lat;
+        p.lattice = l;
+        p.y0 = y0;
+        p.y1 = y1;
+        p.in = E->d_stage;
+        p.state_dst = E->buf[1 - H.par];
+        p.coll_dst = E->buf[H.par];
+        p.flags = E->d_flags;
+        p.wdist = E->d_wdist;
+        p.scale = E->scale;
+        p.inv_scale = E->inv_scale;
+        LBM_HIP(launch_prologue(E->storage, p, d.nx, E->stream));
+    }
+    LBM_HIP(hipStreamSynchronize(E->stream));
+    H.steps = 0;
+    H.read_pos = 0;
+    H.pend_fx.clear();
+    H.pend_fy.clear();
+    H.uploaded = true;
+    // clear this lattice's accumulators (both parities) and NaN flag
+    std::vector<double> z(static_cast<size_t>(H.n_slots) * 2, 0.0);
+    for (int p = 0; p < 2; ++p)
+        if ((rc = h2d(E, E->d_acc + (static_cast<int64_t>(p) * E->total_slots + H.slot_base) * 2, z.data(),
+                      z.size() * 8)) != LBM_OK)
+            return rc;
+    const int32_t zero = 0;
+    if ((rc = h2d(E, E->d_nan + l, &zero, 4)) != LBM_OK) return rc;
+    E->counters_stale = true;
+    return LBM_OK;
+}
+
+int lbm_download(lbm_handle h, int32_t l, double* f)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!f) return fail(LBM_ERR_INVALID, "null output array");
+    if (!h->lat[l].uploaded) return fail(LBM_ERR_INVALID, "lattice has no state");
+    return lbm::materialize(h, l, f, nullptr);
+}
+
+int lbm_mass(lbm_handle h, int32_t l, double* mass)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!mass) return fail(LBM_ERR_INVALID, "null output");
+    if (!h->lat[l].uploaded) return fail(LBM_ERR_INVALID, "lattice has no state");
+    return lbm::materialize(h, l, nullptr, mass);
+}
+
+int lbm_step(lbm_handle h, int32_t n)
+{
+    using namespace lbm;
+    if (!h) return fail(LBM_ERR_INVALID, "null handle");
+    if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+    Ensemble* E = h;
+    int rc;
+    if ((rc = require_uploaded(E)) != LBM_OK) return rc;
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    while (n > 0) {
+        const bool use_graph = E->graph_steps >= 2 && n >= E->graph_steps;
+        const int batch = use_graph ? E->graph_steps : n;
+        for (size_t l = 0; l < E->lat.size(); ++l)
+            if ((rc = ensure_history_room(E, static_cast<int>(l), batch)) != LBM_OK) return rc;
+        if (use_graph) {
+            if (E->graph_dirty || !E->graph)
+                if ((rc = build_graph(E)) != LBM_OK) return rc;
+            LBM_HIP(hipGraphLaunch(E->graph, E->stream));
+        } else {
+            if ((rc = launch_batch(E, batch)) != LBM_OK) return rc;
+        }
+        for (auto& H : E->lat) {
+            H.steps += batch;
+            H.par ^= batch & 1;
+        }
+        n -= batch;
+    }
+    return LBM_OK;
+}
+
+int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
+{
+    using namespace lbm;
+    int rc = lattice_ok(h, l);
+    if (rc) return rc;
+    if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+    Ensemble* E = h;
+    HostLattice& H = E->lat[l];
+    if (!H.uploaded) return fail(LBM_ERR_INVALID, "lattice has no state");
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    if ((rc = ensure_history_room(E, l, n)) != LBM_OK) return rc;
+    StepArgs a = base_args(E);
+    int blocks;
+    grid_for(E, l, a, blocks);
+    a.abs_mode = 1;
+    a.flush_begin = l;
+    a.flush_count = 1;
+    for (int i = 0; i < n; ++i) {
+        a.k_abs = H.steps + i;
+        a.src_par = H.par ^ (i & 1);
+        LBM_HIP(launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream));
+    }
+    H.steps += n;
+    H.par ^= n & 1;
+    E->counters_stale = true;
+    return LBM_OK;
+}
+
+int lbm_set_launch(lbm_handle h, int32_t bx, int32_t rows, int32_t graph_steps)
+{
+    using namespace lbm;
+    if (!h) return fail(LBM_ERR_INVALID, "null handle");
+    Ensemble* E = h;
+    const int nbx = bx > 0 ? bx : E->bx, nrows = rows > 0 ? rows : E->rows;
+    if (!step_shape_supported(nbx, nrows)) return fail(LBM_ERR_INVALID, "unsupported launch shape");
+    if (graph_steps < 0 || (graph_steps > 0 && (graph_steps % 2 != 0 || graph_steps > 2000)))
+        return fail(LBM_ERR_INVALID, "graph_steps must be even and at most 2000");
+    if (nbx != E->bx || nrows != E->rows) {
+        E->bx = nbx;
+        E->rows = nrows;
+        E->tiles_dirty = true;
+        E->graph_dirty = true;
+    }
+    if (graph_steps > 0 && graph_steps != E->graph_steps) {
+        E->graph_steps = graph_steps;
+        E->graph_dirty = true;
+    }
+    return LBM_OK;
+}
+
+int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, float* best_ms)
+{
+    using namespace lbm;
+    if (!h) return fail(LBM_ERR_INVALID, "null handle");
+    Ensemble* E = h;
+    int rc;
+    if ((rc = require_uploaded(E)) != LBM_OK) return rc;
+    void* scratch = nullptr;
+    if (hipMalloc(&scratch, static_cast<size_t>(E->pool_elems) * E->bpp) != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "autotune scratch buffer");
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int shapes[][2] = {{64, 1},  {64, 2},  {64, 4},  {128, 1}, {128, 2}, {128, 4}, {256, 1},
+                             {256, 2}, {256, 4}, {512, 1}, {512, 2}, {256, 8}, {128, 8}};
+    float best = 1e30f;
+    int bb = E->bx, br = E->rows;
+    const int saved_bx = E->bx, saved_rows = E->rows;
+    rc = LBM_OK;
+    for (auto& s : shapes) {
+        E->bx = s[0];
+        E->rows = s[1];
+        E->tiles_dirty = true;
+        if ((rc = commit(E)) != LBM_OK) break;
+        StepArgs a = base_args(E);
+        int blocks;
+        grid_for(E, -1, a, blocks);
+        a.dst_override = scratch;
+        a.acc = E->d_dummy_acc;
+        a.nan_flags = E->d_dummy_nan;
+        a.flush_count = 0;
+        const int reps = 10;
+        for (int w = 0; w < 2; ++w) (void)launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream);
+        (void)hipEventRecord(e0, E->stream);
+        for (int r = 0; r < reps; ++r) (void)launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream);
+        (void)hipEventRecord(e1, E->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(LBM_ERR_DEVICE, "autotune launch failed");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= reps;
+        if (ms < best) {
+            best = ms;
+            bb = s[0];
+            br = s[1];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (rc != LBM_OK) {
+        E->bx = saved_bx;
+        E->rows = saved_rows;
+        E->tiles_dirty = true;
+        return rc;
+    }
+    E->bx = bb;
+    E->rows = br;
+    E->tiles_dirty = true;
+    E->graph_dirty = true;
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    if (best_bx) *best_bx = bb;
+    if (best_rows) *best_rows = br;
+    if (best_ms) *best_ms = best;
+    return LBM_OK;
+}
+
+int lbm_forces(lbm_handle h, int32_t l, double* fx, double* fy, int32_t max_steps, int32_t* n_steps)
+{
+    using namespace lbm;
+    int rc = lattice_ok(h, l);
+    if (rc) return rc;
+    Ensemble* E = h;
+    HostLattice& H = E->lat[l];
+    if ((rc = drain(E, l)) != LBM_OK) return rc;
+    if (H.steps >= 1 && H.read_pos == H.steps - 1) {
+        LBM_HIP(hipStreamSynchronize(E->stream));
+        std::vector<double> cur(static_cast<size_t>(H.n_slots) * 2);
+        const int64_t off = (((H.steps - 1) & 1) * static_cast<int64_t>(E->total_slots) + H.slot_base) * 2;
+        if ((rc = d2h(E, cur.data(), E->d_acc + off, cur.size() * 8)) != LBM_OK) return rc;
+        for (int s = 0; s < H.n_slots; ++s) {
+            H.pend_fx.push_back(cur[2 * s]);
+            H.pend_fy.push_back(cur[2 * s + 1]);
+        }
+        H.read_pos = H.steps;
+    }
+    const int64_t avail = static_cast<int64_t>(H.pend_fx.size()) / H.n_slots;
+    const int64_t take = std::min<int64_t>(avail, max_steps);
+    if (take > 0 && (!fx || !fy)) return fail(LBM_ERR_INVALID, "null force output");
+    const size_t nv = static_cast<size_t>(take) * H.n_slots;
+    if (nv) {
+        std::memcpy(fx, H.pend_fx.data(), nv * 8);
+        std::memcpy(fy, H.pend_fy.data(), nv * 8);
+        H.pend_fx.erase(H.pend_fx.begin(), H.pend_fx.begin() + nv);
+        H.pend_fy.erase(H.pend_fy.begin(), H.pend_fy.begin() + nv);
+    }
+    if (n_steps) *n_steps = static_cast<int32_t>(take);
+    return LBM_OK;
+}
+
+int lbm_nan_flag(lbm_handle h, int32_t l, int32_t* flag)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!flag) return fail(LBM_ERR_INVALID, "null output");
+    if ((rc = lbm::d2h(h, flag, h->d_nan + l, 4)) != LBM_OK) return rc;
+    const int32_t zero = 0;
+    return lbm::h2d(h, h->d_nan + l, &zero, 4);
+}
+
+int lbm_steps_done(lbm_handle h, int32_t l, int64_t* steps)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!steps) return fail(LBM_ERR_INVALID, "null output");
+    *steps = h->lat[l].steps;
+    return LBM_OK;
+}
+
+int lbm_get_info(lbm_handle h, lbm_info* out)
+{
+    using namespace lbm;
+    if (!h || !out) return fail(LBM_ERR_INVALID, "null argument");
+    int rc;
+    if ((rc = commit(h)) != LBM_OK) return rc;
+    lbm_info r{};
+    for (auto& H : h->lat) {
+        const int64_t n = static_cast<int64_t>(H.d.nx) * H.d.ny;
+        r.cells += n;
+        r.fluid_cells += H.fluid;
+        r.link_cells += static_cast<int64_t>(H.recs.size());
+        if (H.d.les) r.les_cells += n;
+    }
+    r.bytes_per_pop = h->bpp;
+    r.n_lattices = static_cast<int32_t>(h->lat.size());
+    r.device_bytes = h->device_bytes;
+    r.block_x = h->bx;
+    r.rows = h->rows;
+    *out = r;
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_handle h)
+{
+    if (!h) return fail(LBM_ERR_INVALID, "null handle");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(LBM_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_wall_distance(lbm_handle h, int32_t l, double* out)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!out) return fail(LBM_ERR_INVALID, "null output");
+    auto& H = h->lat[l];
+    if (H.obstacle.empty()) H.obstacle.assign(static_cast<size_t>(H.d.nx) * H.d.ny, 0);
+    std::vector<double> d;
+    lbm::wall_distance(H.obstacle, H.d.nx, H.d.ny, d);
+    std::memcpy(out, d.data(), d.size() * 8);
+    return LBM_OK;
+}
+
+}  // extern "C"

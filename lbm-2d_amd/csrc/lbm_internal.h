@@ -1,0 +1,50 @@
+// lbm_internal.h -- structures shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace lbm {
+
+// Per-cell flag byte (pitch layout, one per node).
+enum : uint8_t {
+    CELL_SOLID = 1u << 0,    // obstacle node
+    CELL_SPECIAL = 1u << 1,  // needs the general path (edge, boundary link, solid)
+    CELL_LINKS = 1u << 2,    // fluid node with >= 1 boundary link (link record exists)
+    CELL_WRITE = 1u << 3,    // solid node whose stored value must be kept current
+};
+
+// Population storage forms (match enum lbm_storage).
+enum Storage : int { ST_F64 = 0, ST_F32 = 1, ST_F16S = 2, ST_BF16S = 3 };
+
+// Device descriptor of one lattice (read by every block of that lattice).
+struct alignas(16) LatticeDev {
+    int32_t nx, ny, pitch, lattice;
+    int64_t f_off;        // element offset of population plane 0 in the pool
+    int64_t plane;        // elements per population plane (ny * pitch)
+    int64_t cell_off;     // offset of this lattice in per-cell arrays
+    int32_t link_off;     // first link record of this lattice
+    int32_t slot_base;    // first force slot of this lattice
+    int32_t n_slots;
+    int32_t collision, les, x_edge, y_edge, inlet, inlet_y0, inlet_h, outlet, lid, forces;
+    int32_t pad0;
+    double s_shear, s_bulk, s_normal;  // MRT rates (src/lbm_types.hpp:72-76)
+    double tau, inv_tau, u_in, cs2, nu, fscale;
+};
+
+// Boundary-link record for one fluid node with links (indexed by link_slot[cell]).
+struct alignas(16) LinkRec {
+    uint32_t mask;       // bit i: the neighbour along e_i is solid (link i)
+    uint32_t scheme;     // 2 bits per direction: enum lbm_bb
+    int16_t slot[9];     // force slot per link, -1 = excluded
+    int16_t pad;
+    double q[9];
+    double uwx[9];
+    double uwy[9];
+};
+
+// One block of work of the step kernel: lattice, first row, first column.
+struct Tile {
+    int32_t lattice, y0, x0, pad;
+};
+
+}  // namespace lbm

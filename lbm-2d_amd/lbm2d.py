@@ -1,0 +1,324 @@
+"""ctypes binding of liblbm2d.so (include/lbm_gpu.h) for tests and benchmarks.
+
+The library is the product: every call here goes straight to the C ABI, whose
+kernels run on the GPU.  There is no CPU fallback -- loading fails loudly when
+the library has not been built (``make -C lbm-2d_amd``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm2d.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "lbm_gpu.h")
+
+STORAGE = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
+BYTES_PER_POP = {"f64": 8, "f32": 4, "f16s": 2, "bf16s": 2}
+MRT, BGK = 0, 1
+EDGE_PERIODIC, EDGE_OPEN, EDGE_WALL = 0, 1, 2
+INLET_NONE, INLET_ZOU_HE, INLET_PARABOLIC = 0, 1, 2
+BB_SIMPLE, BB_MEI, BB_BOUZIDI = 0, 1, 2
+
+# CaseType (src/lbm_types.hpp:20-23)
+CYLINDER, CAVITY, STEP, RIBS, URBAN_CANYON, DOWNWASH, SQUARE_CYLINDER, FLAT_PLATE, \
+    SPORTS_BALL, PERIODIC_HILLS, CYLINDER_NEAR_WALL, SIDE_BY_SIDE, ROTATING_CYLINDER, \
+    ORIFICE_PLATE = range(14)
+
+
+class LbmDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32),
+        ("collision", C.c_int32), ("les", C.c_int32), ("cs", C.c_double),
+        ("x_edge", C.c_int32), ("y_edge", C.c_int32),
+        ("inlet", C.c_int32), ("inlet_y0", C.c_int32), ("inlet_h", C.c_int32),
+        ("outlet", C.c_int32), ("lid", C.c_int32), ("forces", C.c_int32),
+        ("body_force_x", C.c_double), ("tau", C.c_double), ("u_in", C.c_double),
+    ]
+
+
+class LbmLink(C.Structure):
+    _fields_ = [
+        ("cell", C.c_int32), ("dir", C.c_int32), ("scheme", C.c_int32), ("slot", C.c_int32),
+        ("q", C.c_double), ("uwx", C.c_double), ("uwy", C.c_double),
+    ]
+
+
+class LbmInfo(C.Structure):
+    _fields_ = [
+        ("cells", C.c_int64), ("fluid_cells", C.c_int64), ("link_cells", C.c_int64),
+        ("les_cells", C.c_int64), ("bytes_per_pop", C.c_int32), ("n_lattices", C.c_int32),
+        ("device_bytes", C.c_int64), ("block_x", C.c_int32), ("rows", C.c_int32),
+    ]
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_h = C.c_void_p
+
+
+def declared_functions() -> list:
+    """Names of the functions declared in include/lbm_gpu.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(lbm_\w+)\s*\(", text, re.M)))
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LbmError(f"{LIB_PATH} is missing: build it with `make -C lbm-2d_amd` "
+                       "(there is no fallback path)")
+    L = C.CDLL(LIB_PATH)
+    i32, i64, dbl = C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "lbm_last_error": (C.c_char_p, []),
+        "lbm_desc_for_case": (C.c_int, [C.c_int, i32, i32, C.POINTER(LbmDesc)]),
+        "lbm_create": (C.c_int, [C.POINTER(LbmDesc), i32, i32, i32, C.c_void_p, C.POINTER(_h)]),
+        "lbm_destroy": (C.c_int, [_h]),
+        "lbm_set_obstacle": (C.c_int, [_h, i32, _u8p]),
+        "lbm_set_bb_circle": (C.c_int, [_h, i32, dbl, dbl, dbl, i32]),
+        "lbm_set_bb_polygon": (C.c_int, [_h, i32, _dp, i32, i32]),
+        "lbm_set_moving_wall": (C.c_int, [_h, i32, dbl, dbl, dbl]),
+        "lbm_set_force_radius": (C.c_int, [_h, i32, dbl, dbl, dbl]),
+        "lbm_set_links": (C.c_int, [_h, i32, C.POINTER(LbmLink), i32]),
+        "lbm_set_force_slots": (C.c_int, [_h, i32, i32]),
+        "lbm_set_params": (C.c_int, [_h, i32, dbl, dbl]),
+        "lbm_upload": (C.c_int, [_h, i32, _dp]),
+        "lbm_download": (C.c_int, [_h, i32, _dp]),
+        "lbm_step": (C.c_int, [_h, i32]),
+        "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
+        "lbm_set_launch": (C.c_int, [_h, i32, i32, i32]),
+        "lbm_autotune": (C.c_int, [_h, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_float)]),
+        "lbm_forces": (C.c_int, [_h, i32, _dp, _dp, i32, C.POINTER(i32)]),
+        "lbm_mass": (C.c_int, [_h, i32, _dp]),
+        "lbm_nan_flag": (C.c_int, [_h, i32, C.POINTER(i32)]),
+        "lbm_steps_done": (C.c_int, [_h, i32, C.POINTER(i64)]),
+        "lbm_get_info": (C.c_int, [_h, C.POINTER(LbmInfo)]),
+        "lbm_sync": (C.c_int, [_h]),
+        "lbm_wall_distance": (C.c_int, [_h, i32, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().lbm_last_error().decode(errors="replace")
+        raise LbmError(f"status {rc}: {msg}")
+
+
+def desc_for_case(case: int, nx: int, ny: int, *, tau: float = 1.0, u_in: float = 0.1,
+                  les: bool = False, cs: float = 0.12, collision: int = MRT,
+                  body_force_x: float = 0.0) -> LbmDesc:
+    d = LbmDesc()
+    _check(lib().lbm_desc_for_case(case, nx, ny, C.byref(d)))
+    d.tau, d.u_in, d.les, d.cs = tau, u_in, int(les), cs
+    d.collision, d.body_force_x = collision, body_force_x
+    return d
+
+
+class Ensemble:
+    """A handle holding one or more lattices that step together on the GPU."""
+
+    def __init__(self, descs: Sequence[LbmDesc], storage: str = "f32", scale_exp: int = 8,
+                 stream: Optional[int] = None):
+        self.storage = storage
+        self.descs = list(descs)
+        arr = (LbmDesc * len(self.descs))(*self.descs)
+        h = _h()
+        _check(lib().lbm_create(arr, len(self.descs), STORAGE[storage], scale_exp,
+                                C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def shape(self, l=0):
+        d = self.descs[l]
+        return d.ny, d.nx
+
+    # ---- geometry
+    def set_obstacle(self, l: int, mask: np.ndarray):
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        assert m.shape == self.shape(l), (m.shape, self.shape(l))
+        _check(lib().lbm_set_obstacle(self._h, l, m.ctypes.data_as(_u8p)))
+
+    def set_bb_circle(self, l, cx, cy, r, scheme=BB_MEI):
+        _check(lib().lbm_set_bb_circle(self._h, l, cx, cy, r, scheme))
+
+    def set_bb_polygon(self, l, xy, scheme=BB_MEI):
+        p = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1)
+        _check(lib().lbm_set_bb_polygon(self._h, l, p.ctypes.data_as(_dp), len(p) // 2, scheme))
+
+    def set_moving_wall(self, l, omega, rcx, rcy):
+        _check(lib().lbm_set_moving_wall(self._h, l, omega, rcx, rcy))
+
+    def set_force_radius(self, l, cx, cy, r):
+        _check(lib().lbm_set_force_radius(self._h, l, cx, cy, r))
+
+    def set_force_slots(self, l, n):
+        _check(lib().lbm_set_force_slots(self._h, l, n))
+
+    def set_links(self, l, links: Sequence[tuple]):
+        """links: iterable of (cell, dir, scheme, slot, q, uwx, uwy)."""
+        arr = (LbmLink * max(1, len(links)))()
+        for k, t in enumerate(links):
+            arr[k] = LbmLink(*t)
+        _check(lib().lbm_set_links(self._h, l, arr, len(links)))
+
+    def set_params(self, l, tau, u_in):
+        _check(lib().lbm_set_params(self._h, l, tau, u_in))
+        self.descs[l].tau, self.descs[l].u_in = tau, u_in
+
+    # ---- state
+    def upload(self, l: int, f: np.ndarray):
+        ny, nx = self.shape(l)
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(ny, nx, 9)
+        _check(lib().lbm_upload(self._h, l, a.ctypes.data_as(_dp)))
+
+    def download(self, l: int = 0) -> np.ndarray:
+        ny, nx = self.shape(l)
+        out = np.empty((ny, nx, 9), dtype=np.float64)
+        _check(lib().lbm_download(self._h, l, out.ctypes.data_as(_dp)))
+        return out
+
+    def step(self, n: int = 1):
+        _check(lib().lbm_step(self._h, n))
+
+    def step_lattice(self, l: int, n: int = 1):
+        _check(lib().lbm_step_lattice(self._h, l, n))
+
+    def set_launch(self, block_x=0, rows=0, graph_steps=0):
+        _check(lib().lbm_set_launch(self._h, block_x, rows, graph_steps))
+
+    def autotune(self):
+        bx, rows, ms = C.c_int32(), C.c_int32(), C.c_float()
+        _check(lib().lbm_autotune(self._h, C.byref(bx), C.byref(rows), C.byref(ms)))
+        return bx.value, rows.value, ms.value
+
+    def forces(self, l: int = 0, n_slots: int = 1):
+        """Per-step FP64 force totals since the last call: arrays (steps, n_slots)."""
+        chunks_x, chunks_y = [], []
+        cap = 8192
+        while True:
+            fx = np.empty(cap * n_slots)
+            fy = np.empty(cap * n_slots)
+            n = C.c_int32()
+            _check(lib().lbm_forces(self._h, l, fx.ctypes.data_as(_dp), fy.ctypes.data_as(_dp),
+                                    cap, C.byref(n)))
+            chunks_x.append(fx[: n.value * n_slots])
+            chunks_y.append(fy[: n.value * n_slots])
+            if n.value < cap:
+                break
+        fx = np.concatenate(chunks_x).reshape(-1, n_slots)
+        fy = np.concatenate(chunks_y).reshape(-1, n_slots)
+        return fx, fy
+
+    def mass(self, l: int = 0) -> float:
+        m = C.c_double()
+        _check(lib().lbm_mass(self._h, l, C.byref(m)))
+        return m.value
+
+    def nan_flag(self, l: int = 0) -> bool:
+        f = C.c_int32()
+        _check(lib().lbm_nan_flag(self._h, l, C.byref(f)))
+        return bool(f.value)
+
+    def steps_done(self, l: int = 0) -> int:
+        s = C.c_int64()
+        _check(lib().lbm_steps_done(self._h, l, C.byref(s)))
+        return s.value
+
+    def info(self) -> LbmInfo:
+        i = LbmInfo()
+        _check(lib().lbm_get_info(self._h, C.byref(i)))
+        return i
+
+    def sync(self):
+        _check(lib().lbm_sync(self._h))
+
+    def wall_distance(self, l: int = 0) -> np.ndarray:
+        ny, nx = self.shape(l)
+        out = np.empty((ny, nx))
+        _check(lib().lbm_wall_distance(self._h, l, out.ctypes.data_as(_dp)))
+        return out
+
+
+# ---- host-side helpers mirroring the reference's geometry and initial conditions ----------
+
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+CXV = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CYV = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+
+
+def equilibrium(rho, u, v) -> np.ndarray:
+    """compute_equilibrium (src/lbm_types.hpp:269-273), broadcast over the last axis."""
+    rho, u, v = np.asarray(rho, float), np.asarray(u, float), np.asarray(v, float)
+    eu = CXV * u[..., None] + CYV * v[..., None]
+    uu = (u * u + v * v)[..., None]
+    return W * rho[..., None] * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uu)
+
+
+def circle_mask(nx, ny, cx, cy, r) -> np.ndarray:
+    """place_cylinder rasterisation: sqrt(dx^2+dy^2) < r (src/lbm.hpp:931-944)."""
+    y, x = np.mgrid[0:ny, 0:nx]
+    dx = (x - cx).astype(float)
+    dy = (y - cy).astype(float)
+    return (np.sqrt(dx * dx + dy * dy) < r).astype(np.uint8)
+
+
+def walls_mask(nx, ny) -> np.ndarray:
+    """place_walls: x=0, x=nx-1 and y=0 are solid (src/lbm.hpp:79-87)."""
+    m = np.zeros((ny, nx), np.uint8)
+    m[0, :] = 1
+    m[:, 0] = 1
+    m[:, nx - 1] = 1
+    return m
+
+
+def polygon_mask(nx, ny, xy) -> np.ndarray:
+    """place_polygon with the even-odd ray-crossing rule (src/geometry.hpp:120-134)."""
+    xy = np.asarray(xy, float).reshape(-1, 2)
+    y, x = np.mgrid[0:ny, 0:nx]
+    px, py = x.astype(float), y.astype(float)
+    inside = np.zeros((ny, nx), bool)
+    n = len(xy)
+    j = n - 1
+    for i in range(n):
+        xi, yi = xy[i]
+        xj, yj = xy[j]
+        cond = (yi > py) != (yj > py)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xint = (xj - xi) * (py - yi) / (yj - yi) + xi
+        inside ^= cond & (px < xint)
+        j = i
+    return inside.astype(np.uint8)

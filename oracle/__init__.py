@@ -53,6 +53,8 @@ def lib() -> C.CDLL:
         L = C.CDLL(ORACLE_SO)
         L.orc_equilibrium.restype = C.c_double
         L.orc_equilibrium.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_weight.restype = C.c_double
+        L.orc_weight.argtypes = [C.c_int]
         L.orc_macros.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_mrt_rates.argtypes = [C.c_double, _dp]
         L.orc_mrt_collide.argtypes = [_dp, C.c_double, C.c_double, C.c_double, _dp,
@@ -370,7 +372,8 @@ class RefLattice:
         return self.L.ref_time_steps(n, tau, u_in)
 
     def total_force(self):
-        return float(self.fx_body.sum(dtype=np.float64)), float(self.fy_body.sum(dtype=np.float64))
+        """Sum over nodes in node order, as the drivers do (src/main.cpp:106-110)."""
+        return float(np.cumsum(self.fx_body.ravel())[-1]), float(np.cumsum(self.fy_body.ravel())[-1])
 
 
 def ref_uniform_draws(n, a, b, fast=False):

@@ -1,0 +1,169 @@
+"""Small lattices exercising every branch of the step, built identically in the
+FP64 oracle and in the GPU library.
+
+Each case mirrors one of the reference's case drivers (src/*.cpp) at a size the
+oracle runs in well under a second per hundred steps.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Callable, Optional
+
+import numpy as np
+
+import oracle as O
+import lbm2d as G
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    nx: int
+    ny: int
+    case: int
+    tau: float
+    u: float
+    steps: int
+    setup: Callable  # (obj) -> None, works for Oracle and RefLattice
+    init: tuple = (1.0, 0.1, 0.0)
+    pert: Optional[tuple] = None
+    les: bool = False
+    cs: float = 0.12
+    collision: int = O.MRT
+    body_force: float = 0.0
+    # GPU-side geometry: list of ("circle", cx, cy, r, scheme) / ("polygon", xy, scheme) /
+    # ("moving", omega, rcx, rcy) / ("force_radius", cx, cy, r)
+    gpu_geom: tuple = ()
+
+
+def _cyl(cx, cy, r):
+    return lambda s: s.place_cylinder(cx, cy, r)
+
+
+def _rot(cx, cy, r, omega, mei=True):
+    def f(s):
+        s.place_cylinder(cx, cy, r)
+        s.set_moving_wall(omega, cx, cy)
+        if not mei:
+            s.set_mei(False)
+    return f
+
+
+def plate_polygon(cx, cy, chord, aoa_deg, thickness=1.0):
+    """Rotated chord x thickness rectangle of the flat-plate driver (src/flat_plate.cpp:90-105)."""
+    rad = -aoa_deg * np.pi / 180.0
+    ca, sa = np.cos(rad), np.sin(rad)
+    hx, hy = chord / 2.0, thickness / 2.0
+    pts = []
+    for px, py in ((-hx, -hy), (hx, -hy), (hx, hy), (-hx, hy)):
+        pts.append((px * ca - py * sa + cx, px * sa + py * ca + cy))
+    return np.array(pts)
+
+
+def _plate(xy):
+    return lambda s: s.place_polygon(xy)
+
+
+def _step_geometry(nx, ny):
+    m = np.zeros((ny, nx), np.uint8)
+    m[0, :] = 1
+    m[ny - 1, :] = 1
+    m[: ny // 3, : nx // 4] = 1
+    return m
+
+
+def _hills(nx, ny):
+    """Periodic-hills obstacle (src/periodic_hills.cpp:95-115) at a small size."""
+    H = ny * 2 // 3
+    h_max = max(1, int(H / 6))
+    m = np.zeros((ny, nx), np.uint8)
+    for x in range(nx):
+        xr = x / nx * 3
+        hill = H - h_max * 0.5 * (1 - np.cos(2 * np.pi * xr)) * (1 - np.cos(2 * np.pi * xr))
+        hy = int(np.ceil(hill))
+        m[ny - 1, x] = 1
+        m[:hy, x] = 1
+    return m
+
+
+def _mask_setup(maskfn):
+    def f(s):
+        s.set_obstacle(maskfn)
+    return f
+
+
+def _near_wall(cx, cy, r):
+    def f(s):
+        s.place_cylinder(cx, cy, r)
+        m = s.obstacle.copy()
+        m[0, :] = 1  # bottom wall (src/cylinder_near_wall.cpp:93-96)
+        s.set_obstacle(m)
+    return f
+
+
+PLATE = plate_polygon(24, 24, 22.0, 10.0)
+HILLS = _hills(96, 36)
+STEPM = _step_geometry(96, 36)
+
+CASES = {
+    "cavity": Case("cavity", 32, 32, O.CAVITY, 0.8, 0.1, 200, lambda s: s.place_walls(),
+                   init=(1.0, 0.0, 0.0)),
+    "cylinder": Case("cylinder", 96, 48, O.CYLINDER, 0.56, 0.1, 100, _cyl(24, 25, 6),
+                     pert=(29, 84), gpu_geom=(("circle", 24, 25, 6, G.BB_MEI),)),
+    "rotating": Case("rotating", 96, 48, O.ROTATING_CYLINDER, 0.6, 0.1, 60,
+                     _rot(24, 24, 6, 0.1 * 0.1 / 6),
+                     gpu_geom=(("circle", 24, 24, 6, G.BB_MEI), ("moving", 0.1 * 0.1 / 6, 24, 24))),
+    "bouzidi": Case("bouzidi", 96, 48, O.ROTATING_CYLINDER, 0.6, 0.1, 60,
+                    _rot(24, 24, 6, 0.01, mei=False),
+                    gpu_geom=(("circle", 24, 24, 6, G.BB_BOUZIDI), ("moving", 0.01, 24, 24))),
+    "les": Case("les", 96, 48, O.CYLINDER, 0.52, 0.1, 60, _cyl(24, 25, 6), pert=(29, 84),
+                les=True, cs=0.1, gpu_geom=(("circle", 24, 25, 6, G.BB_MEI),)),
+    "bgk": Case("bgk", 96, 48, O.CYLINDER, 0.7, 0.1, 60, _cyl(24, 25, 6), pert=(29, 84),
+                collision=O.BGK, gpu_geom=(("circle", 24, 25, 6, G.BB_MEI),)),
+    "plate": Case("plate", 96, 48, O.FLAT_PLATE, 0.6, 0.1, 60, _plate(PLATE),
+                  gpu_geom=(("polygon", PLATE, G.BB_MEI),)),
+    "step": Case("step", 96, 36, O.STEP, 0.7, 0.1, 80, _mask_setup(STEPM), init=(1.0, 0.0, 0.0)),
+    "hills": Case("hills", 96, 36, O.PERIODIC_HILLS, 0.8, 0.1, 60, _mask_setup(HILLS),
+                  init=(1.0, 0.0, 0.0), body_force=2e-5),
+    "near_wall": Case("near_wall", 96, 48, O.CYLINDER_NEAR_WALL, 0.6, 0.1, 60,
+                      _near_wall(24, 14, 6), pert=(29, 84),
+                      gpu_geom=(("circle", 24, 14, 6, G.BB_MEI), ("force_radius", 24, 14, 6))),
+}
+
+
+def build_oracle(c: Case, cls=O.Oracle):
+    o = cls(c.nx, c.ny, c.case, collision=c.collision, les=c.les, cs=c.cs)
+    c.setup(o)
+    if c.body_force:
+        o.set_body_force(c.body_force)
+    o.init_equilibrium(*c.init)
+    if c.pert:
+        o.perturb_seed42(*c.pert)
+    return o
+
+
+def gpu_desc(c: Case):
+    return G.desc_for_case(c.case, c.nx, c.ny, tau=c.tau, u_in=c.u, les=c.les, cs=c.cs,
+                           collision=c.collision, body_force_x=c.body_force)
+
+
+def apply_gpu_geometry(ens: G.Ensemble, l: int, c: Case, obstacle: np.ndarray):
+    ens.set_obstacle(l, obstacle)
+    for g in c.gpu_geom:
+        if g[0] == "circle":
+            ens.set_bb_circle(l, g[1], g[2], g[3], g[4])
+        elif g[0] == "polygon":
+            ens.set_bb_polygon(l, g[1], g[2])
+        elif g[0] == "moving":
+            ens.set_moving_wall(l, g[1], g[2], g[3])
+        elif g[0] == "force_radius":
+            ens.set_force_radius(l, g[1], g[2], g[3])
+
+
+def build_gpu(c: Case, storage: str, o=None, scale_exp: int = 8):
+    """GPU ensemble of one lattice with the oracle's obstacle mask and initial state."""
+    o = o if o is not None else build_oracle(c)
+    ens = G.Ensemble([gpu_desc(c)], storage=storage, scale_exp=scale_exp)
+    apply_gpu_geometry(ens, 0, c, o.obstacle.copy())
+    ens.upload(0, o.f.copy())
+    return ens, o

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP step against the FP64 oracle on identical inputs.
+
+Tolerances (absolute, on populations f_i ~ O(0.5); forces relative to |F|+1e-3):
+  F64 storage  : 1e-12 populations, 1e-10 forces   (same algorithm in FP64)
+  F32 storage  : 2e-6 populations, 1e-4 forces     (FP32 arithmetic on f - w)
+  F16S (2^8)   : 2e-3 populations                  (binary16 deviations)
+  BF16S (2^8)  : 1.5e-2 populations                (bfloat16 deviations)
+"""
+import numpy as np
+import pytest
+
+import lbm2d as G
+from cases import CASES, build_gpu, build_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-12, "f32": 2e-6, "f16s": 2e-3, "bf16s": 1.5e-2}
+FTOL = {"f64": 1e-10, "f32": 1e-4}
+
+
+def _run(name, storage, steps=None, graph_steps=None):
+    c = CASES[name]
+    n = steps or c.steps
+    ens, o = build_gpu(c, storage)
+    if graph_steps:
+        ens.set_launch(graph_steps=graph_steps)
+    fx_o, fy_o = [], []
+    for _ in range(n):
+        o.step(c.tau, c.u)
+        fx, fy = o.total_force()
+        fx_o.append(fx)
+        fy_o.append(fy)
+    ens.step(n)
+    f_gpu = ens.download(0)
+    gx, gy = ens.forces(0)
+    return c, o, ens, f_gpu, np.array(fx_o), np.array(fy_o), gx[:, 0], gy[:, 0]
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_f64_matches_oracle(name):
+    c, o, ens, f, fx_o, fy_o, gx, gy = _run(name, "f64")
+    print(f"[{name} f64] max|df| = {np.abs(f - o.f).max():.3e}")
+    np.testing.assert_allclose(f, o.f, rtol=0, atol=TOL["f64"])
+    if c.case != G.CAVITY:
+        assert len(gx) == c.steps
+        np.testing.assert_allclose(gx, fx_o, rtol=0, atol=FTOL["f64"] * (np.abs(fx_o).max() + 1e-3))
+        np.testing.assert_allclose(gy, fy_o, rtol=0, atol=FTOL["f64"] * (np.abs(fx_o).max() + 1e-3))
+    m = ens.mass(0)
+    assert abs(m - o.mass()) <= 1e-10 * abs(o.mass())
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_f32_matches_oracle(name):
+    c, o, ens, f, fx_o, fy_o, gx, gy = _run(name, "f32")
+    err = np.abs(f - o.f).max()
+    print(f"[{name} f32] max|df| = {err:.3e}")
+    assert err <= TOL["f32"]
+    if c.case != G.CAVITY:
+        scale = np.abs(fx_o).max() + 1e-3
+        assert np.abs(gx - fx_o).max() <= FTOL["f32"] * scale
+        assert np.abs(gy - fy_o).max() <= FTOL["f32"] * scale
+
+
+@pytest.mark.parametrize("storage", ["f16s", "bf16s"])
+@pytest.mark.parametrize("name", ["cylinder", "cavity", "les"])
+def test_16bit_storage_error(name, storage):
+    c, o, ens, f, *_ = _run(name, storage)
+    err = np.abs(f - o.f).max()
+    print(f"[{name} {storage}] max|df| = {err:.3e}")
+    assert err <= TOL[storage]
+
+
+def test_download_before_any_step_returns_upload():
+    c = CASES["cylinder"]
+    ens, o = build_gpu(c, "f64")
+    np.testing.assert_array_equal(ens.download(0), o.f)
+
+
+def test_graph_equals_single_launches():
+    c = CASES["cylinder"]
+    a, _ = build_gpu(c, "f32")
+    b, _ = build_gpu(c, "f32")
+    a.set_launch(graph_steps=20)
+    a.step(40)  # two graph replays
+    for _ in range(40):
+        b.step_lattice(0, 1)
+    np.testing.assert_array_equal(a.download(0), b.download(0))
+    fa, _ = a.forces(0)
+    fb, _ = b.forces(0)
+    np.testing.assert_allclose(fa, fb, rtol=1e-12, atol=1e-15)
+
+
+def test_odd_step_counts_and_interleaving():
+    c = CASES["cylinder"]
+    ens, o = build_gpu(c, "f64")
+    ens.set_launch(graph_steps=4)
+    done = 0
+    for n in (1, 3, 5, 2, 7):
+        ens.step(n)
+        o.step(c.tau, c.u, n)
+        done += n
+        np.testing.assert_allclose(ens.download(0), o.f, rtol=0, atol=TOL["f64"])
+    assert ens.steps_done(0) == done
+    fx, _ = ens.forces(0)
+    assert fx.shape == (done, 1)
+
+
+def test_ensemble_of_uneven_lattices_matches_each_alone():
+    names = ["cylinder", "cavity", "step", "plate"]
+    from cases import apply_gpu_geometry, gpu_desc
+    cs = [CASES[n] for n in names]
+    orcs = [build_oracle(c) for c in cs]
+    ens = G.Ensemble([gpu_desc(c) for c in cs], storage="f64")
+    for l, (c, o) in enumerate(zip(cs, orcs)):
+        apply_gpu_geometry(ens, l, c, o.obstacle.copy())
+        ens.upload(l, o.f.copy())
+    ens.set_launch(graph_steps=10)
+    ens.step(30)
+    for l, (c, o) in enumerate(zip(cs, orcs)):
+        o.step(c.tau, c.u, 30)
+        np.testing.assert_allclose(ens.download(l), o.f, rtol=0, atol=TOL["f64"])
+
+
+def test_autotune_keeps_state():
+    c = CASES["cylinder"]
+    ens, o = build_gpu(c, "f32")
+    ens.step(3)
+    before = ens.download(0)
+    bx, rows, ms = ens.autotune()
+    assert bx in (64, 128, 256, 512) and rows in (1, 2, 4, 8) and ms > 0
+    np.testing.assert_array_equal(ens.download(0), before)
+    ens.step(5)
+    o.step(c.tau, c.u, 8)
+    assert np.abs(ens.download(0) - o.f).max() <= TOL["f32"]
+
+
+def test_nan_flag_trips():
+    c = CASES["cylinder"]
+    o = build_oracle(c)
+    f = o.f.copy()
+    f[20, 60, 3] = np.nan
+    ens, _ = build_gpu(c, "f32", o=o)
+    ens.upload(0, f)
+    assert not ens.nan_flag(0)
+    ens.step(2)
+    assert ens.nan_flag(0)
+    assert not ens.nan_flag(0)  # reading clears it
+
+
+def test_params_change_between_steps():
+    c = CASES["cylinder"]
+    ens, o = build_gpu(c, "f64")
+    ens.step(5)
+    o.step(c.tau, c.u, 5)
+    ens.set_params(0, 0.62, 0.08)
+    ens.step(5)
+    o.step(0.62, 0.08, 5)
+    np.testing.assert_allclose(ens.download(0), o.f, rtol=0, atol=TOL["f64"])
+
+
+def test_wall_distance_matches_oracle():
+    import oracle as O
+    c = CASES["les"]
+    ens, o = build_gpu(c, "f32")
+    np.testing.assert_array_equal(ens.wall_distance(0), O.wall_distance(o.obstacle))
