@@ -110,7 +110,7 @@ def graph_step_ms(ens, torch, replays=20, graph_steps=100):
 def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1):
     ens, mask = cylinder_lattice(nx, ny, nx // 4, ny // 2, r, tau, 0.1, storage, stream,
                                  les=les, cs=cs, perturb=False)
-    bx, rows, _ = ens.autotune()
+    shape, _ = ens.autotune()
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
     kms = kernel_launch_ms(ens, torch, n=50)
@@ -123,7 +123,7 @@ def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False,
         "kernel_ms": kms,
         "bytes_per_step": b,
         "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "launch": [bx, rows],
+        "launch": list(shape),
         "mass": m0,
     }
     ens.close()
@@ -212,7 +212,7 @@ def bench_ensemble(steps, warmup, torch, stream, n=256):
         ens.set_links(l, ensemble_links(mask, circles, d.nx, d.ny))
         f = G.equilibrium(np.ones((d.ny, d.nx)), np.full((d.ny, d.nx), 0.1), np.zeros((d.ny, d.nx)))
         ens.upload(l, f)
-    bx, rows, _ = ens.autotune()
+    shape, _ = ens.autotune()
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
     # the same lattices, one launch per lattice per step, no graph
@@ -238,7 +238,7 @@ def bench_ensemble(steps, warmup, torch, stream, n=256):
         "hbm_frac": b / (t / steps) / 1e9 / HBM_PEAK_GBS,
         "plain_per_lattice_GLUPS": glups_plain,
         "speedup_vs_plain": glups / glups_plain,
-        "launch": [bx, rows],
+        "launch": list(shape),
         "cd_lattice0_last": float(2 * fx[-1, 0] / (2 * max(c[2] for c in lats[0][2]) * 0.01)),
     }
     ens.close()
@@ -305,7 +305,7 @@ def main():
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         ens, mask = cylinder_lattice(800, 300, 200, 150, 30, 0.68, 0.1, "f32", stream.cuda_stream)
-        bx, rows, _ = ens.autotune()
+        shape, _ = ens.autotune()
         ens.set_launch(graph_steps=100)
         info = ens.info()
         t = time_steps(ens, args.steps, args.warmup, torch)
@@ -352,7 +352,7 @@ def main():
                         "r=30 at (200,150), Re=100 tau=0.68 u=0.1, Mei interpolated bounce-back, "
                         "Zou/He inlet, outflow copy, periodic y, seed-42 perturbation, "
                         "hipGraph of 100 steps",
-            "cells": info.cells, "fluid_cells": info.fluid_cells, "launch": [bx, rows],
+            "cells": info.cells, "fluid_cells": info.fluid_cells, "launch": list(shape),
             "graph_step_ms": gms, "cd_last_step": cd_last,
         },
         "roofline": {

@@ -94,6 +94,8 @@ typedef struct lbm_info {
     int32_t n_lattices;
     int64_t device_bytes;     /* device memory held by the handle */
     int32_t block_x, rows;    /* launch shape chosen for the step kernel */
+    int32_t nodes_per_thread;
+    int32_t launch_options;
 } lbm_info;
 
 /* Message of the last failing call on this host thread. */
@@ -111,7 +113,8 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
                void* stream, lbm_handle* out);
 int lbm_destroy(lbm_handle h);
 
-/* Geometry.  lbm_set_obstacle takes nx*ny bytes (nonzero = solid), the
+/* Geometry.  Set it before lbm_upload: a change afterwards invalidates the stored
+ * state until the next upload.  lbm_set_obstacle takes nx*ny bytes (nonzero = solid), the
  * LBMCapabilities::obstacle array; it resets every link to simple bounce-back
  * counted in slot 0, the reference's behaviour without bb_geom. */
 int lbm_set_obstacle(lbm_handle h, int32_t lattice, const uint8_t* mask);
@@ -132,7 +135,8 @@ int lbm_set_links(lbm_handle h, int32_t lattice, const lbm_link* links, int32_t 
 /* Number of force slots of a lattice (default 1). */
 int lbm_set_force_slots(lbm_handle h, int32_t lattice, int32_t n_slots);
 
-/* Relaxation time and inlet/lid velocity for the following steps. */
+/* Relaxation time and inlet/lid velocity for the following steps (the collision of
+ * the state already stored is redone with the new tau). */
 int lbm_set_params(lbm_handle h, int32_t lattice, double tau, double u_in);
 
 /* State transfer in the reference layout (AoS FP64, nx*ny*9 doubles). */
@@ -144,12 +148,20 @@ int lbm_download(lbm_handle h, int32_t lattice, double* f_aos);
 int lbm_step(lbm_handle h, int32_t n);
 /* Advances one lattice by n steps with one launch per step and no graph. */
 int lbm_step_lattice(lbm_handle h, int32_t lattice, int32_t n);
-/* Launch shape / graph length; 0 = keep current.  Valid block_x: 64,128,256,512;
- * rows: 1,2,4,8. */
-int lbm_set_launch(lbm_handle h, int32_t block_x, int32_t rows, int32_t graph_steps);
-/* Times every launch shape on the current state and keeps the fastest.  The
- * lattice state and force history are unchanged. */
-int lbm_autotune(lbm_handle h, int32_t* best_block_x, int32_t* best_rows, float* best_ms);
+/* Launch options of the step kernel (bit field). */
+#define LBM_LAUNCH_XCD_ORDER 1     /* consecutive tiles stay on one XCD (one L2) */
+#define LBM_LAUNCH_STREAM_STORES 2 /* nontemporal stores of the new populations */
+
+/* Launch shape of the step kernel (threads per block, rows per block, consecutive
+ * nodes per thread), option bits and hipGraph length; 0 (-1 for options) keeps the
+ * current value.  Shapes are a compiled set; a group of nodes moves at most 16
+ * bytes per population plane. */
+int lbm_set_launch(lbm_handle h, int32_t block_x, int32_t rows, int32_t nodes_per_thread,
+                   int32_t options, int32_t graph_steps);
+/* Times every compiled launch shape with every option set on the current state and
+ * keeps the fastest.  The lattice state and force history are unchanged. */
+int lbm_autotune(lbm_handle h, int32_t* best_block_x, int32_t* best_rows, int32_t* best_nodes,
+                 int32_t* best_options, float* best_ms);
 
 /* Total force per step, per slot, since the previous call (FP64), in step order.
  * fx/fy receive up to max_steps*n_slots values (step-major); *n_steps the count. */

@@ -42,8 +42,6 @@ int fail(int code, const std::string& msg)
 
 constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-constexpr double W[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
-                         1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 
 constexpr int kHistoryRows = 4096;
 constexpr int64_t kStagingBytes = 256ll << 20;
@@ -152,7 +150,7 @@ struct Ensemble {
     double* d_stage = nullptr;
     int total_slots = 0;
     int R = kHistoryRows;
-    int bx = 256, rows = 1, graph_steps = 100;
+    int bx = 256, rows = 1, vec = 1, options = 0, graph_steps = 100;
     bool geom_dirty = true, desc_dirty = true, tiles_dirty = true, counters_stale = false;
     hipGraphExec_t graph = nullptr;
     bool graph_dirty = true;
@@ -393,13 +391,13 @@ LatticeDev make_dev(const HostLattice& H, int index)
     // MRTParams::from_tau (src/lbm_types.hpp:72-76)
     double s = 1.0 / d.tau;
     L.s_shear = s < 0.5 ? 0.5 : (s > 1.99 ? 1.99 : s);
-    L.s_bulk = 1.2;
-    L.s_normal = 1.0;
     L.tau = d.tau;
     L.inv_tau = 1.0 / d.tau;
     L.u_in = d.u_in;
+    L.cs = d.les ? d.cs : 0.0;
     L.cs2 = d.les ? d.cs * d.cs : 0.0;
     L.nu = (d.tau - 0.5) / 3.0;
+    L.inv_nu = L.nu > 0.0 ? 1.0 / L.nu : 0.0;
     // Fscale (src/lbm.hpp:296-297)
     L.fscale = d.body_force_x != 0.0 ? (1.0 - 1.0 / (2.0 * d.tau)) * 3.0 * d.body_force_x : 0.0;
     return L;
@@ -469,7 +467,8 @@ int commit(Ensemble* E)
         for (size_t l = 0; l < E->lat.size(); ++l) {
             const lbm_desc& d = E->lat[l].d;
             for (int y0 = 0; y0 < d.ny; y0 += E->rows)
-                for (int x0 = 0; x0 < d.nx; x0 += E->bx) tiles.push_back(Tile{static_cast<int32_t>(l), y0, x0, 0});
+                for (int x0 = 0; x0 < d.nx; x0 += E->bx * E->vec)
+                    tiles.push_back(Tile{static_cast<int32_t>(l), y0, x0, 0});
         }
         if (tiles.size() > E->tiles_cap) {
             if ((rc = dev_alloc(E, &E->d_tiles, tiles.size())) != LBM_OK) return rc;
@@ -513,6 +512,7 @@ StepArgs base_args(Ensemble* E)
     a.nan_flags = E->d_nan;
     a.scale = E->scale;
     a.inv_scale = E->inv_scale;
+    a.options = E->options;
     return a;
 }
 
@@ -524,7 +524,7 @@ void grid_for(Ensemble* E, int single, StepArgs& a, int& blocks)
         const lbm_desc& d = E->lat[l].d;
         a.tiles = nullptr;
         a.single = l;
-        a.tiles_x = (d.nx + E->bx - 1) / E->bx;
+        a.tiles_x = (d.nx + E->bx * E->vec - 1) / (E->bx * E->vec);
         a.tiles_y = (d.ny + E->rows - 1) / E->rows;
         blocks = a.tiles_x * a.tiles_y;
     } else {
@@ -570,7 +570,7 @@ int launch_batch(Ensemble* E, int n)
     a.flush_count = static_cast<int>(E->lat.size());
     for (int i = 0; i < n; ++i) {
         a.step_arg = i;
-        LBM_HIP(launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream));
+        LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream));
     }
     LBM_HIP(launch_add_steps(E->d_steps, E->d_par, 0, static_cast<int>(E->lat.size()), n, E->stream));
     return LBM_OK;
@@ -604,7 +604,10 @@ int require_uploaded(Ensemble* E)
 }
 
 // Runs the state kernel over lattice l in row chunks; out may be null (mass only).
-int materialize(Ensemble* E, int l, double* host_out, double* mass_out)
+// Runs the state kernel over lattice l in row chunks: host_out receives the state
+// (AoS FP64) when non-null, mass_out its FP64 mass; recollide rewrites the stored
+// post-collision buffer from the state with the current descriptor.
+int materialize(Ensemble* E, int l, double* host_out, double* mass_out, bool recollide = false)
 {
     HostLattice& H = E->lat[l];
     const lbm_desc& d = H.d;
@@ -629,8 +632,10 @@ int materialize(Ensemble* E, int l, double* host_out, double* mass_out)
         s.flags = E->d_flags;
         s.link_slot = E->d_lslot;
         s.links = E->d_links;
-        s.out = E->d_stage;
-        s.mass_dev = E->d_mass;
+        s.out = host_out ? E->d_stage : nullptr;
+        s.mass_dev = mass_out ? E->d_mass : nullptr;
+        s.coll_dst = recollide ? E->buf[H.par] : nullptr;
+        s.wdist = E->d_wdist;
         s.scale = E->scale;
         s.inv_scale = E->inv_scale;
         LBM_HIP(launch_state(E->storage, s, d.nx, E->stream));
@@ -817,6 +822,7 @@ int lbm_set_obstacle(lbm_handle h, int32_t l, const uint8_t* mask)
     H.frad = false;
     H.explicit_links.clear();
     H.geom_dirty = true;
+    H.uploaded = false;  // the stored state belongs to the old geometry
     h->geom_dirty = true;
     return LBM_OK;
 }
@@ -833,6 +839,7 @@ int lbm_set_bb_circle(lbm_handle h, int32_t l, double cx, double cy, double r, i
     H.br = r;
     H.bb_scheme = scheme;
     H.geom_dirty = h->geom_dirty = true;
+    H.uploaded = false;  // the stored state belongs to the old geometry
     return LBM_OK;
 }
 
@@ -846,6 +853,7 @@ int lbm_set_bb_polygon(lbm_handle h, int32_t l, const double* xy, int32_t n, int
     H.poly.assign(xy, xy + 2 * n);
     H.bb_scheme = scheme;
     H.geom_dirty = h->geom_dirty = true;
+    H.uploaded = false;  // the stored state belongs to the old geometry
     return LBM_OK;
 }
 
@@ -859,6 +867,7 @@ int lbm_set_moving_wall(lbm_handle h, int32_t l, double omega, double rcx, doubl
     H.rcx = rcx;
     H.rcy = rcy;
     H.geom_dirty = h->geom_dirty = true;
+    H.uploaded = false;  // the stored state belongs to the old geometry
     return LBM_OK;
 }
 
@@ -872,6 +881,7 @@ int lbm_set_force_radius(lbm_handle h, int32_t l, double cx, double cy, double r
     H.fcy = cy;
     H.fr = r;
     H.geom_dirty = h->geom_dirty = true;
+    H.uploaded = false;  // the stored state belongs to the old geometry
     return LBM_OK;
 }
 
@@ -883,6 +893,7 @@ int lbm_set_links(lbm_handle h, int32_t l, const lbm_link* links, int32_t n)
     auto& H = h->lat[l];
     H.explicit_links.assign(links, links + n);
     H.geom_dirty = h->geom_dirty = true;
+    H.uploaded = false;  // the stored state belongs to the old geometry
     return LBM_OK;
 }
 
@@ -895,6 +906,7 @@ int lbm_set_force_slots(lbm_handle h, int32_t l, int32_t n_slots)
     if (H.steps > 0 || H.uploaded) return fail(LBM_ERR_INVALID, "set force slots before lbm_upload");
     H.n_slots = n_slots;
     H.geom_dirty = h->geom_dirty = true;
+    H.uploaded = false;  // the stored state belongs to the old geometry
     return LBM_OK;
 }
 
@@ -904,6 +916,14 @@ int lbm_set_params(lbm_handle h, int32_t l, double tau, double u_in)
     if (rc) return rc;
     if (!(tau > 0.0)) return fail(LBM_ERR_INVALID, "tau must be positive");
     auto& H = h->lat[l];
+    if (H.d.tau != tau && H.uploaded) {
+        // The stored buffer already holds the next step's collision (pull form), done
+        // with the old tau: redo it from the current state.  The state itself was
+        // produced with the old inlet velocity, so that one changes afterwards.
+        H.d.tau = tau;
+        h->desc_dirty = true;
+        if ((rc = lbm::materialize(h, l, nullptr, nullptr, true)) != LBM_OK) return rc;
+    }
     if (H.d.tau != tau || H.d.u_in != u_in) {
         H.d.tau = tau;
         H.d.u_in = u_in;
@@ -1028,7 +1048,7 @@ int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
     for (int i = 0; i < n; ++i) {
         a.k_abs = H.steps + i;
         a.src_par = H.par ^ (i & 1);
-        LBM_HIP(launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream));
+        LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream));
     }
     H.steps += n;
     H.par ^= n & 1;
@@ -1036,18 +1056,25 @@ int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
     return LBM_OK;
 }
 
-int lbm_set_launch(lbm_handle h, int32_t bx, int32_t rows, int32_t graph_steps)
+int lbm_set_launch(lbm_handle h, int32_t bx, int32_t rows, int32_t nodes_per_thread, int32_t options,
+                   int32_t graph_steps)
 {
     using namespace lbm;
     if (!h) return fail(LBM_ERR_INVALID, "null handle");
     Ensemble* E = h;
     const int nbx = bx > 0 ? bx : E->bx, nrows = rows > 0 ? rows : E->rows;
-    if (!step_shape_supported(nbx, nrows)) return fail(LBM_ERR_INVALID, "unsupported launch shape");
+    const int nv = nodes_per_thread > 0 ? nodes_per_thread : E->vec;
+    if (!step_shape_supported(E->storage, nbx, nrows, nv))
+        return fail(LBM_ERR_INVALID, "launch shape not compiled for this storage form");
     if (graph_steps < 0 || (graph_steps > 0 && (graph_steps % 2 != 0 || graph_steps > 2000)))
         return fail(LBM_ERR_INVALID, "graph_steps must be even and at most 2000");
-    if (nbx != E->bx || nrows != E->rows) {
+    const int nopt = options >= 0 ? options : E->options;
+    if (nopt & ~(LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES)) return fail(LBM_ERR_INVALID, "unknown launch option bits");
+    if (nbx != E->bx || nrows != E->rows || nv != E->vec || nopt != E->options) {
         E->bx = nbx;
         E->rows = nrows;
+        E->vec = nv;
+        E->options = nopt;
         E->tiles_dirty = true;
         E->graph_dirty = true;
     }
@@ -1058,43 +1085,51 @@ int lbm_set_launch(lbm_handle h, int32_t bx, int32_t rows, int32_t graph_steps)
     return LBM_OK;
 }
 
-int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, float* best_ms)
+int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* best_nodes, int32_t* best_options,
+                 float* best_ms)
 {
     using namespace lbm;
     if (!h) return fail(LBM_ERR_INVALID, "null handle");
     Ensemble* E = h;
     int rc;
     if ((rc = require_uploaded(E)) != LBM_OK) return rc;
+    if ((rc = commit(E)) != LBM_OK) return rc;
     void* scratch = nullptr;
     if (hipMalloc(&scratch, static_cast<size_t>(E->pool_elems) * E->bpp) != hipSuccess)
         return fail(LBM_ERR_NOMEM, "autotune scratch buffer");
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const int shapes[][2] = {{64, 1},  {64, 2},  {64, 4},  {128, 1}, {128, 2}, {128, 4}, {256, 1},
-                             {256, 2}, {256, 4}, {512, 1}, {512, 2}, {256, 8}, {128, 8}};
+    int shapes[64][3];
+    const int n_shapes = std::min(64, step_shapes(E->storage, shapes, 64));
     float best = 1e30f;
-    int bb = E->bx, br = E->rows;
-    const int saved_bx = E->bx, saved_rows = E->rows;
+    const int saved[4] = {E->bx, E->rows, E->vec, E->options};
+    int bs[4] = {E->bx, E->rows, E->vec, E->options};
     rc = LBM_OK;
-    for (auto& s : shapes) {
-        E->bx = s[0];
-        E->rows = s[1];
+    for (int si = 0; si < 4 * n_shapes; ++si) {
+        E->bx = shapes[si % n_shapes][0];
+        E->rows = shapes[si % n_shapes][1];
+        E->vec = shapes[si % n_shapes][2];
+        E->options = si / n_shapes;
         E->tiles_dirty = true;
         if ((rc = commit(E)) != LBM_OK) break;
         StepArgs a = base_args(E);
         int blocks;
         grid_for(E, -1, a, blocks);
+        // a scratch destination and dummy accumulators keep state and history intact
         a.dst_override = scratch;
         a.acc = E->d_dummy_acc;
         a.nan_flags = E->d_dummy_nan;
         a.flush_count = 0;
         const int reps = 10;
-        for (int w = 0; w < 2; ++w) (void)launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream);
+        hipError_t le = hipSuccess;
+        for (int w = 0; w < 2 && le == hipSuccess; ++w)
+            le = launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream);
         (void)hipEventRecord(e0, E->stream);
-        for (int r = 0; r < reps; ++r) (void)launch_step(E->storage, E->bx, E->rows, blocks, a, E->stream);
+        for (int r = 0; r < reps && le == hipSuccess; ++r)
+            le = launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream);
         (void)hipEventRecord(e1, E->stream);
-        if (hipEventSynchronize(e1) != hipSuccess) {
+        if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
             rc = fail(LBM_ERR_DEVICE, "autotune launch failed");
             break;
         }
@@ -1103,26 +1138,28 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, float* best
         ms /= reps;
         if (ms < best) {
             best = ms;
-            bb = s[0];
-            br = s[1];
+            bs[0] = E->bx;
+            bs[1] = E->rows;
+            bs[2] = E->vec;
+            bs[3] = E->options;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(scratch);
-    if (rc != LBM_OK) {
-        E->bx = saved_bx;
-        E->rows = saved_rows;
-        E->tiles_dirty = true;
-        return rc;
-    }
-    E->bx = bb;
-    E->rows = br;
+    const int* pick = rc == LBM_OK ? bs : saved;
+    E->bx = pick[0];
+    E->rows = pick[1];
+    E->vec = pick[2];
+    E->options = pick[3];
     E->tiles_dirty = true;
     E->graph_dirty = true;
+    if (rc != LBM_OK) return rc;
     if ((rc = commit(E)) != LBM_OK) return rc;
-    if (best_bx) *best_bx = bb;
-    if (best_rows) *best_rows = br;
+    if (best_bx) *best_bx = bs[0];
+    if (best_rows) *best_rows = bs[1];
+    if (best_nodes) *best_nodes = bs[2];
+    if (best_options) *best_options = bs[3];
     if (best_ms) *best_ms = best;
     return LBM_OK;
 }
@@ -1198,6 +1235,8 @@ int lbm_get_info(lbm_handle h, lbm_info* out)
     r.device_bytes = h->device_bytes;
     r.block_x = h->bx;
     r.rows = h->rows;
+    r.nodes_per_thread = h->vec;
+    r.launch_options = h->options;
     *out = r;
     return LBM_OK;
 }

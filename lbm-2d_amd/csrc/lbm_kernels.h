@@ -10,6 +10,12 @@
 namespace lbm {
 
 // Arguments of the fused step kernel (passed by value; graph capture keeps a copy).
+// Step-kernel launch options (bit field).
+enum : int32_t {
+    LAUNCH_XCD_ORDER = 1,       // consecutive tiles on one XCD (blocks go round-robin over 8)
+    LAUNCH_STREAM_STORES = 2,   // nontemporal stores of the new populations
+};
+
 struct StepArgs {
     const LatticeDev* lat;
     const Tile* tiles;          // ensemble mode: one entry per block
@@ -36,6 +42,7 @@ struct StepArgs {
     int64_t k_abs;
     int32_t src_par;
     int32_t flush_begin, flush_count;
+    int32_t options;            // LAUNCH_* bits
     float scale, inv_scale;     // 2^k and 2^-k of the 16-bit forms
 };
 
@@ -50,8 +57,10 @@ struct StateArgs {
     const uint8_t* flags;
     const int32_t* link_slot;
     const LinkRec* links;
-    double* out;                // AoS FP64, (y1-y0)*nx*9
-    double* mass_dev;
+    double* out;                // AoS FP64, (y1-y0)*nx*9; null: not written
+    double* mass_dev;           // null: no mass sum
+    void* coll_dst;             // non-null: write the state's collision here (re-collide)
+    const uint16_t* wdist;
     float scale, inv_scale;
 };
 
@@ -68,13 +77,16 @@ struct PrologueArgs {
     float scale, inv_scale;
 };
 
-hipError_t launch_step(int storage, int block_x, int rows, int n_blocks, const StepArgs& a,
+hipError_t launch_step(int storage, int block_x, int rows, int vec, int n_blocks, const StepArgs& a,
                        hipStream_t s);
 hipError_t launch_state(int storage, const StateArgs& a, int nx, hipStream_t s);
 hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream_t s);
 hipError_t launch_add_steps(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
                             hipStream_t s);
 
-bool step_shape_supported(int block_x, int rows);
+bool step_shape_supported(int storage, int block_x, int rows, int vec);
+// Fills up to max compiled shapes {block_x, rows, nodes per thread} for a storage
+// form; returns their count.
+int step_shapes(int storage, int (*out)[3], int max);
 
 }  // namespace lbm

@@ -36,6 +36,9 @@ enum { INLET_NONE = 0, INLET_ZOU_HE = 1, INLET_PARABOLIC = 2 };
 enum { BB_SIMPLE = 0, BB_MEI = 1, BB_BOUZIDI = 2 };
 
 // ------------------------------------------------------------------ storage forms
+//
+// dec/enc convert between a stored value and the deviation f - w; dec_raw/enc_raw
+// keep the stored scale (h = 2^k (f - w)), for arithmetic done in scaled units.
 
 template <int ST> struct Form;
 template <> struct Form<ST_F64> {
@@ -43,41 +46,34 @@ template <> struct Form<ST_F64> {
     using real = double;
     static __device__ __forceinline__ real dec(store v, float) { return v; }
     static __device__ __forceinline__ store enc(real d, float) { return d; }
+    static __device__ __forceinline__ real dec_raw(store v) { return v; }
+    static __device__ __forceinline__ store enc_raw(real d) { return d; }
 };
 template <> struct Form<ST_F32> {
     using store = float;
     using real = float;
     static __device__ __forceinline__ real dec(store v, float) { return v; }
     static __device__ __forceinline__ store enc(real d, float) { return d; }
+    static __device__ __forceinline__ real dec_raw(store v) { return v; }
+    static __device__ __forceinline__ store enc_raw(real d) { return d; }
 };
 template <> struct Form<ST_F16S> {
     using store = uint16_t;
     using real = float;
-    static __device__ __forceinline__ real dec(store v, float inv_scale)
-    {
-        return static_cast<float>(__builtin_bit_cast(_Float16, v)) * inv_scale;
-    }
+    static __device__ __forceinline__ real dec_raw(store v) { return static_cast<float>(__builtin_bit_cast(_Float16, v)); }
     // Conversion to binary16 rounds to nearest even (v_cvt_f16_f32).
-    static __device__ __forceinline__ store enc(real d, float scale)
-    {
-        return __builtin_bit_cast(uint16_t, static_cast<_Float16>(d * scale));
-    }
+    static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(h)); }
+    static __device__ __forceinline__ real dec(store v, float inv_scale) { return dec_raw(v) * inv_scale; }
+    static __device__ __forceinline__ store enc(real d, float scale) { return enc_raw(d * scale); }
 };
 template <> struct Form<ST_BF16S> {
     using store = uint16_t;
     using real = float;
-    static __device__ __forceinline__ real dec(store v, float inv_scale)
-    {
-        return __uint_as_float(static_cast<uint32_t>(v) << 16) * inv_scale;
-    }
-    // Round to nearest even on the 16 dropped mantissa bits; NaN stays NaN.
-    static __device__ __forceinline__ store enc(real d, float scale)
-    {
-        uint32_t u = __float_as_uint(d * scale);
-        if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return static_cast<uint16_t>(u >> 16);
-    }
+    static __device__ __forceinline__ real dec_raw(store v) { return __uint_as_float(static_cast<uint32_t>(v) << 16); }
+    // Round to nearest even (v_cvt_pk_bf16_f32 on gfx950); NaN stays NaN.
+    static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(h)); }
+    static __device__ __forceinline__ real dec(store v, float inv_scale) { return dec_raw(v) * inv_scale; }
+    static __device__ __forceinline__ store enc(real d, float scale) { return enc_raw(d * scale); }
 };
 
 template <typename R> __device__ __forceinline__ R wgt(int i) { return static_cast<R>(WD[i]); }
@@ -89,54 +85,62 @@ template <typename R> __device__ __forceinline__ R clamp_rate(R s)
     return s < R(0.5) ? R(0.5) : (s > R(1.99) ? R(1.99) : s);
 }
 
+// FP32: the hardware square root, reciprocal and exp2 (1 ulp class); FP64: correctly rounded.
 template <typename R> __device__ __forceinline__ R dsqrt(R x);
-template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqrt(x); }
+template <typename R> __device__ __forceinline__ R recip(R x);
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double recip<double>(double x) { return 1.0 / x; }
 template <typename R> __device__ __forceinline__ R dexp(R x);
 template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
 template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
 
-// MRT (src/lbm.hpp:128-218) or BGK (src/lbm.hpp:334-354) on deviations d_i = f_i - w_i,
-// followed by the body force (src/lbm.hpp:327-331).  The relaxation is applied as
-// f <- f - M^-1 S (m - m_eq): rho and j are conserved, so only the six
-// non-conserved moment differences enter.  With w subtracted, the moments of w
-// (e = -2, eps = 1, others 0) cancel against the equilibria analytically.
+// MRT (src/lbm.hpp:128-218) or BGK (src/lbm.hpp:334-354), then the body force
+// (src/lbm.hpp:327-331), on d_i = sc (f_i - w_i): sc = 1 for plain deviations, 2^k
+// when a 16-bit form computes in its stored units.
+//
+// The MRT relaxation f <- f - M^-1 S (m - m_eq) needs only the six non-conserved
+// moment differences (rho, j conserved).  With w subtracted and the rates of
+// MRTParams::from_tau (s_bulk = 1.2, s_normal = 1; src/lbm_types.hpp:72-76) folded in:
+//   e - e_eq     = -2 d0 + S_ax + 4 S_dg - 3 J2          (S_ax = d1+d2+d3+d4,
+//   eps - eps_eq = 3 (d0 - S_ax + J2)                      S_dg = d5+d6+d7+d8,
+//   qx - qx_eq   = 2 jx - 3 (d1 - d3)                      J2 = rho |u|^2)
+//   qy - qy_eq   = 2 jy - 3 (d2 - d4)
+//   pxx - pxx_eq = (d1 + d3) - (d2 + d4) - (jx u - jy v)
+//   pxy - pxy_eq = (d5 + d7) - (d6 + d8) - jx v
+// and the rows of M^-1 regroup into the few sums below.
 template <typename R>
-__device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist)
+__device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist, R sc)
 {
-    const R s0 = d[1] + d[2] + d[3] + d[4];
-    const R s1 = d[5] + d[6] + d[7] + d[8];
-    const R sum = d[0] + s0 + s1;  // rho - 1
-    const R rho = R(1) + sum;
-    const R jx = (d[1] - d[3]) + (d[5] - d[6]) - (d[7] - d[8]);
-    const R jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
-    const R inv_rho = R(1) / rho;
-    const R u = jx * inv_rho, v = jy * inv_rho;
-    const R usq = u * u + v * v;
+    const R a13 = d[1] + d[3], s13 = d[1] - d[3];
+    const R a24 = d[2] + d[4], s24 = d[2] - d[4];
+    const R a57 = d[5] + d[7], s57 = d[5] - d[7];
+    const R a68 = d[6] + d[8], s68 = d[6] - d[8];
+    const R sax = a13 + a24, sdg = a57 + a68;
+    const R sum = d[0] + sax + sdg;   // sc (rho - 1)
+    const R jx = s13 + s57 - s68;     // sc rho u
+    const R jy = s24 + s57 + s68;     // sc rho v
+    const R rho_s = sc + sum;         // sc rho
+    const R ir = recip<R>(rho_s);
+    const R u = jx * ir, v = jy * ir; // velocity (unscaled)
 
     if (L.collision != 0) {
-        // BGK: d_i -= (d_i - (feq_i - w_i)) / tau
+        // BGK: d_i -= (d_i - sc (feq_i - w_i)) / tau
         const R it = static_cast<R>(L.inv_tau);
+        const R usq = u * u + v * v;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             const R eu = R(CX[i]) * u + R(CY[i]) * v;
-            const R feq_dev = wgt<R>(i) * (sum + rho * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
+            const R feq_dev = wgt<R>(i) * (sum + rho_s * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
             d[i] -= it * (d[i] - feq_dev);
         }
     } else {
-        const R e_d = R(-4) * d[0] - s0 + R(2) * s1;
-        const R eps_d = R(4) * d[0] - R(2) * s0 + s1;
-        const R qx_d = R(-2) * d[1] + R(2) * d[3] + d[5] - d[6] - d[7] + d[8];
-        const R qy_d = R(-2) * d[2] + R(2) * d[4] + d[5] + d[6] - d[7] - d[8];
-        const R pxx_d = d[1] - d[2] + d[3] - d[4];
-        const R pxy_d = d[5] - d[6] + d[7] - d[8];
-        const R ru = R(3) * rho * usq;
-        const R ne = e_d + R(2) * sum - ru;      // e - e_eq
-        const R neps = eps_d - sum + ru;         // eps - eps_eq
-        const R nqx = qx_d + jx;                 // qx - qx_eq
-        const R nqy = qy_d + jy;
-        const R npxx = pxx_d - (jx * u - jy * v);  // pxx - rho (u^2 - v^2)
-        const R npxy = pxy_d - jx * v;             // pxy - rho u v
+        const R J2 = jx * u + jy * v;                                   // sc rho |u|^2
+        const R ne = R(-2) * d[0] + sax + R(4) * sdg - R(3) * J2;        // e - e_eq
+        const R e3 = d[0] - sax + J2;                                    // (eps - eps_eq) / 3
+        const R npxx = (a13 - a24) - (jx * u - jy * v);
+        const R npxy = (a57 - a68) - jx * v;
         R ss = static_cast<R>(L.s_shear);
         if (L.les) {
             // Smagorinsky with Van Driest damping (src/lbm.hpp:155-182, y+ :314-325).
@@ -144,41 +148,43 @@ __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist
             R cs2 = static_cast<R>(L.cs2);
             const R nu = static_cast<R>(L.nu);
             if (nu > R(0)) {
-                const R y_plus = wall_dist * dsqrt<R>(usq + R(1e-12)) / nu;
+                const R y_plus = wall_dist * dsqrt<R>(u * u + v * v + R(1e-12)) * static_cast<R>(L.inv_nu);
                 if (y_plus > R(0)) {
-                    const R damp = R(1) - dexp<R>(-y_plus / R(26));
-                    const R csd = dsqrt<R>(cs2) * damp;
+                    const R damp = R(1) - dexp<R>(y_plus * R(-1.0 / 26.0));
+                    const R csd = static_cast<R>(L.cs) * damp;
                     cs2 = csd * csd;
                 }
             }
+            // A = 9 cs^2 Q / (2 rho); Q/rho is scale free
             const R Q = dsqrt<R>(R(2) * (npxx * npxx + npxy * npxy));
-            const R A = R(9) * cs2 * Q / (R(2) * rho);
+            const R A = R(4.5) * cs2 * Q * ir;
             const R tau = static_cast<R>(L.tau);
             const R tau_eff = (tau + dsqrt<R>(tau * tau + R(4) * A)) * R(0.5);
-            ss = clamp_rate<R>(R(1) / tau_eff);
+            ss = clamp_rate<R>(recip<R>(tau_eff));
         }
-        const R sb = static_cast<R>(L.s_bulk), sn = static_cast<R>(L.s_normal);
-        const R De = sb * ne, Deps = sb * neps;
-        const R Dqx = sn * nqx, Dqy = sn * nqy;
-        const R Dpxx = ss * npxx, Dpxy = ss * npxy;
-        // rows of M^-1 applied to (0, De, Deps, 0, Dqx, 0, Dqy, Dpxx, Dpxy)
-        const R a = De * R(-1.0 / 36.0) - Deps * R(1.0 / 18.0);
-        const R b = De * R(1.0 / 18.0) + Deps * R(1.0 / 36.0);
-        const R qx6 = Dqx * R(1.0 / 6.0), qy6 = Dqy * R(1.0 / 6.0);
-        const R qx12 = Dqx * R(1.0 / 12.0), qy12 = Dqy * R(1.0 / 12.0);
-        const R p4 = Dpxx * R(0.25), x4 = Dpxy * R(0.25);
-        d[0] -= (Deps - De) * R(1.0 / 9.0);
-        d[1] -= a - qx6 + p4;
-        d[2] -= a - qy6 - p4;
-        d[3] -= a + qx6 + p4;
-        d[4] -= a + qy6 - p4;
-        d[5] -= b + qx12 + qy12 + x4;
-        d[6] -= b - qx12 + qy12 - x4;
-        d[7] -= b - qx12 - qy12 + x4;
-        d[8] -= b + qx12 - qy12 - x4;
+        // M^-1 S (m - m_eq), s_bulk = 1.2 and s_normal = 1
+        const R a = ne * R(-1.2 / 36.0) - e3 * R(3.6 / 18.0);
+        const R b = ne * R(1.2 / 18.0) + e3 * R(3.6 / 36.0);
+        const R c0 = e3 * R(3.6 / 9.0) - ne * R(1.2 / 9.0);
+        const R qx6 = jx * R(1.0 / 3.0) - s13 * R(0.5);  // (qx - qx_eq) / 6
+        const R qy6 = jy * R(1.0 / 3.0) - s24 * R(0.5);
+        const R ss4 = ss * R(0.25);
+        const R p4 = ss4 * npxx, x4 = ss4 * npxy;
+        const R t = a + p4, t2 = a - p4;
+        const R u1 = b + x4, u2 = b - x4;
+        const R qp = (qx6 + qy6) * R(0.5), qm = (qx6 - qy6) * R(0.5);
+        d[0] -= c0;
+        d[1] = d[1] - t + qx6;
+        d[3] = d[3] - t - qx6;
+        d[2] = d[2] - t2 + qy6;
+        d[4] = d[4] - t2 - qy6;
+        d[5] -= u1 + qp;
+        d[7] -= u1 - qp;
+        d[6] -= u2 - qm;
+        d[8] -= u2 + qm;
     }
     if (L.fscale != 0.0) {
-        const R fa = static_cast<R>(WD[1] * L.fscale), fd = static_cast<R>(WD[5] * L.fscale);
+        const R fa = static_cast<R>(WD[1] * L.fscale) * sc, fd = static_cast<R>(WD[5] * L.fscale) * sc;
         d[1] += fa;
         d[3] -= fa;
         d[5] += fd;
@@ -393,26 +399,196 @@ __device__ __forceinline__ void flush_forces(ForcePart p, double* accp)
 
 // ------------------------------------------------------------------ step kernel
 
-template <int ST, int BX, int ROWS>
-__global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
+// Store of a value or an aligned pack of values, optionally nontemporal.
+template <typename T> __device__ __forceinline__ void store_val(T* p, const T& v, bool nt)
+{
+    if constexpr (sizeof(T) == 16) {
+        using U = uint32_t __attribute__((ext_vector_type(4)));
+        const U u = __builtin_bit_cast(U, v);
+        if (nt) __builtin_nontemporal_store(u, reinterpret_cast<U*>(p));
+        else *reinterpret_cast<U*>(p) = u;
+    } else if constexpr (sizeof(T) == 8) {
+        using U = uint32_t __attribute__((ext_vector_type(2)));
+        const U u = __builtin_bit_cast(U, v);
+        if (nt) __builtin_nontemporal_store(u, reinterpret_cast<U*>(p));
+        else *reinterpret_cast<U*>(p) = u;
+    } else if constexpr (sizeof(T) == 4) {
+        const uint32_t u = __builtin_bit_cast(uint32_t, v);
+        if (nt) __builtin_nontemporal_store(u, reinterpret_cast<uint32_t*>(p));
+        else *reinterpret_cast<uint32_t*>(p) = u;
+    } else {
+        static_assert(sizeof(T) == 2, "store size");
+        const uint16_t u = __builtin_bit_cast(uint16_t, v);
+        if (nt) __builtin_nontemporal_store(u, reinterpret_cast<uint16_t*>(p));
+        else *reinterpret_cast<uint16_t*>(p) = u;
+    }
+}
+
+// One node, one cell per thread: the bulk pull or the general path.
+template <int ST>
+__device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs& a,
+                                            const Ctx<ST>& c, typename Form<ST>::store* dst,
+                                            double* accp, int x, int y, ForcePart& part, bool& bad)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    const uint8_t fl = a.flags[L.cell_off + idx];
+    R d[9];
+    const bool solid = (fl & CELL_SOLID) != 0;
+    if (!(fl & CELL_SPECIAL)) {
+        const S* b = c.base + idx;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(b[j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale);
+    } else {
+        if (solid && !(fl & CELL_WRITE)) return;
+        state_general<ST>(c, x, y, solid, d);
+        if ((fl & CELL_LINKS) && L.forces) {
+            // momentum exchange on the post-boundary state (src/lbm.hpp:542-581)
+            const LinkRec& rec = a.links[L.link_off + a.link_slot[L.cell_off + idx]];
+#pragma unroll
+            for (int i = 1; i < 9; ++i) {
+                if (!(rec.mask & (1u << i)) || rec.slot[i] < 0) continue;
+                const double fb = WD[OPP[i]] + static_cast<double>(d[OPP[i]]);
+                part.add(L.slot_base + rec.slot[i], CX[i] * 2.0 * fb, CY[i] * 2.0 * fb, accp);
+            }
+        }
+    }
+    if (!solid) {
+        R wd = R(0);
+        if (L.les) {
+            const uint16_t w = a.wdist[L.cell_off + idx];
+            wd = w == 0xffffu ? R(1e9) : static_cast<R>(w);
+        }
+        collide<R>(d, L, wd, R(1));
+        if (!(d[0] == d[0])) bad = true;
+    }
+    S* o = dst + L.f_off + idx;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale), a.options & LAUNCH_STREAM_STORES);
+}
 
+// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane i-1 /
+// lane i+1; lanes without a source keep their own value.
+__device__ __forceinline__ uint32_t from_left_lane(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x138, 0xf,
+                                                              0xf, false));
+}
+__device__ __forceinline__ uint32_t from_right_lane(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x130, 0xf,
+                                                              0xf, false));
+}
+template <typename S> __device__ __forceinline__ S shift_in(S v, bool left)
+{
+    if constexpr (sizeof(S) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        uint32_t lo = static_cast<uint32_t>(u), hi = static_cast<uint32_t>(u >> 32);
+        lo = left ? from_left_lane(lo) : from_right_lane(lo);
+        hi = left ? from_left_lane(hi) : from_right_lane(hi);
+        return __builtin_bit_cast(S, (static_cast<uint64_t>(hi) << 32) | lo);
+    } else if constexpr (sizeof(S) == 4) {
+        const uint32_t u = __builtin_bit_cast(uint32_t, v);
+        return __builtin_bit_cast(S, left ? from_left_lane(u) : from_right_lane(u));
+    } else {
+        const uint32_t u = static_cast<uint32_t>(v);
+        return static_cast<S>(left ? from_left_lane(u) : from_right_lane(u));
+    }
+}
+
+template <typename S, int V> struct alignas(sizeof(S) * V) Pack {
+    S v[V];
+};
+
+// V consecutive bulk nodes of one row per thread.  Every plane moves as one aligned
+// vector of V values; a population arriving from x-1 (x+1) for the first (last) node
+// of the group comes from the neighbouring lane through a DPP shift, and the first
+// (last) lane of the wave loads it itself.  Requires an interior row.
+template <int ST, int V>
+__device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
+                                             typename Form<ST>::store* dst, int x, int y, bool bulk,
+                                             bool& bad)
+{
+    using S = typename Form<ST>::store;
+    using R = typename Form<ST>::real;
+    using P = Pack<S, V>;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    P in[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        in[j] = *reinterpret_cast<const P*>(c.base + j * L.plane + idx - CY[j] * L.pitch);
+    S edge[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        if (CX[j] == 1) edge[j] = shift_in<S>(in[j].v[V - 1], true);
+        if (CX[j] == -1) edge[j] = shift_in<S>(in[j].v[0], false);
+    }
+    if (!bulk) return;
+    const int lane = __lane_id();
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+            if (CX[j] == 1) edge[j] = c.base[j * L.plane + idx - CY[j] * L.pitch - 1];
+    }
+    if (lane == 63) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+            if (CX[j] == -1) edge[j] = c.base[j * L.plane + idx - CY[j] * L.pitch + V];
+    }
+    Pack<uint16_t, V> wd4;
+    if (L.les) wd4 = *reinterpret_cast<const Pack<uint16_t, V>*>(a.wdist + L.cell_off + idx);
+    P out[9];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        R d[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            S s;
+            if (CX[j] == 0) s = in[j].v[k];
+            else if (CX[j] == 1) s = k == 0 ? edge[j] : in[j].v[k - 1];
+            else s = k == V - 1 ? edge[j] : in[j].v[k + 1];
+            d[j] = Form<ST>::dec_raw(s);
+        }
+        R wd = R(0);
+        if (L.les) wd = wd4.v[k] == 0xffffu ? R(1e9) : static_cast<R>(wd4.v[k]);
+        collide<R>(d, L, wd, static_cast<R>(a.scale));  // in stored units
+        if (!(d[0] == d[0])) bad = true;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) out[j].v[k] = Form<ST>::enc_raw(d[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        store_val(reinterpret_cast<P*>(dst + L.f_off + j * L.plane + idx), out[j], a.options & LAUNCH_STREAM_STORES);
+}
+
+template <int ST, int BX, int ROWS, int V>
+__global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
+{
+    using S = typename Form<ST>::store;
+
+    // Workgroups are handed to the 8 XCDs round-robin; with LAUNCH_XCD_ORDER XCD x
+    // takes one contiguous range of tiles, so lines shared by neighbouring tiles stay
+    // in one L2.  The map is a bijection on [0, gridDim.x).
+    int b = static_cast<int>(blockIdx.x);
+    if (a.options & LAUNCH_XCD_ORDER) {
+        const int nb = static_cast<int>(gridDim.x), q = nb >> 3, r = nb & 7, xcd = b & 7;
+        b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+    }
     int lat_id, y0, x0;
     if (a.tiles) {
-        const Tile t = a.tiles[blockIdx.x];
+        const Tile t = a.tiles[b];
         lat_id = t.lattice;
         y0 = t.y0;
         x0 = t.x0;
     } else {
         lat_id = a.single;
-        const int tx = blockIdx.x % a.tiles_x;
-        const int ty = blockIdx.x / a.tiles_x;
+        const int tx = b % a.tiles_x;
+        const int ty = b / a.tiles_x;
         y0 = ty * ROWS;
-        x0 = tx * BX;
+        x0 = tx * BX * V;
     }
-    const LatticeDev& L = a.lat[lat_id];
+    const LatticeDev L = a.lat[lat_id];  // by value: scalar registers, loaded before any store
     const int64_t k = a.abs_mode ? a.k_abs : a.steps[lat_id] + a.step_arg;
     const int sp = a.abs_mode ? a.src_par : (a.par[lat_id] ^ (a.step_arg & 1));
     const S* src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
@@ -441,47 +617,36 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
     const Ctx<ST> c{L, src + L.f_off, a.flags, a.link_slot, a.links, a.inv_scale};
     ForcePart part;
     bool bad = false;
-    const int x = x0 + static_cast<int>(threadIdx.x);
+    const int x = x0 + V * static_cast<int>(threadIdx.x);
 
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         const int y = y0 + r;
-        if (x >= L.nx || y >= L.ny) continue;
-        const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
-        const uint8_t fl = a.flags[L.cell_off + idx];
-        R d[9];
-        const bool solid = (fl & CELL_SOLID) != 0;
-        if (!(fl & CELL_SPECIAL)) {
-            const S* b = c.base + idx;
+        if (y >= L.ny) break;  // uniform across the block
+        bool done = false;
+        if constexpr (V > 1) {
+            if (y >= 1 && y <= L.ny - 2) {  // uniform across the block
+                // the group is bulk when it lies inside [1, nx-2] and no node is special
+                bool bulk = false;
+                if (x + V - 1 <= L.nx - 2 && x >= 1) {
+                    const Pack<uint8_t, V> f =
+                        *reinterpret_cast<const Pack<uint8_t, V>*>(a.flags + L.cell_off + static_cast<int64_t>(y) * L.pitch + x);
+                    uint32_t any = 0;
 #pragma unroll
-            for (int j = 0; j < 9; ++j)
-                d[j] = Form<ST>::dec(b[j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale);
-        } else {
-            if (solid && !(fl & CELL_WRITE)) continue;
-            state_general<ST>(c, x, y, solid, d);
-            if ((fl & CELL_LINKS) && L.forces) {
-                // momentum exchange on the post-boundary state (src/lbm.hpp:542-581)
-                const LinkRec& rec = a.links[L.link_off + a.link_slot[L.cell_off + idx]];
-#pragma unroll
-                for (int i = 1; i < 9; ++i) {
-                    if (!(rec.mask & (1u << i)) || rec.slot[i] < 0) continue;
-                    const double fb = WD[OPP[i]] + static_cast<double>(d[OPP[i]]);
-                    part.add(L.slot_base + rec.slot[i], CX[i] * 2.0 * fb, CY[i] * 2.0 * fb, accp);
+                    for (int q = 0; q < V; ++q) any |= f.v[q];
+                    bulk = !(any & CELL_SPECIAL);
                 }
+                // every lane inside the padded row takes part in the lane shifts
+                if (x < L.pitch) group_update<ST, V>(L, a, c, dst, x, y, bulk, bad);
+                done = bulk;
             }
         }
-        if (!solid) {
-            R wd = R(0);
-            if (L.les) {
-                const uint16_t w = a.wdist[L.cell_off + idx];
-                wd = w == 0xffffu ? R(1e9) : static_cast<R>(w);
-            }
-            collide<R>(d, L, wd);
-            if (!(d[0] == d[0])) bad = true;
+        if (!done) {
+            // one copy of the general path, looped (keeps the kernel small)
+#pragma unroll 1
+            for (int q = 0; q < V; ++q)
+                if (x + q < L.nx) cell_update<ST>(L, a, c, dst, accp, x + q, y, part, bad);
         }
-        S* o = dst + L.f_off + idx;
-#pragma unroll
-        for (int j = 0; j < 9; ++j) o[j * L.plane] = Form<ST>::enc(d[j], a.scale);
     }
     if (bad) atomicOr(a.nan_flags + lat_id, 1);
     flush_forces(part, accp);
@@ -494,7 +659,7 @@ __global__ __launch_bounds__(256) void state_kernel(StateArgs a)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
-    const LatticeDev& L = a.lat[a.lattice];
+    const LatticeDev L = a.lat[a.lattice];
     const int x = blockIdx.x * 256 + threadIdx.x;
     const int y = a.y0 + blockIdx.y;
     double mass = 0.0;
@@ -515,15 +680,34 @@ __global__ __launch_bounds__(256) void state_kernel(StateArgs a)
             const Ctx<ST> c{L, base, a.flags, a.link_slot, a.links, a.inv_scale};
             state_general<ST>(c, x, y, solid, d);
         }
-        double* o = a.out + (static_cast<int64_t>(y - a.y0) * L.nx + x) * 9;
+        if (a.out) {
+            double* o = a.out + (static_cast<int64_t>(y - a.y0) * L.nx + x) * 9;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            o[j] = WD[j] + static_cast<double>(d[j]);
-            if (!solid) mass += static_cast<double>(d[j]);
+            for (int j = 0; j < 9; ++j) o[j] = WD[j] + static_cast<double>(d[j]);
+        }
+        if (!solid) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) mass += static_cast<double>(d[j]);
+        }
+        if (a.coll_dst && (!solid || (fl & CELL_WRITE))) {
+            // the stored post-collision state of this step, with the current parameters
+            if (!solid) {
+                R wd = R(0);
+                if (L.les) {
+                    const uint16_t w = a.wdist[L.cell_off + idx];
+                    wd = w == 0xffffu ? R(1e9) : static_cast<R>(w);
+                }
+                collide<R>(d, L, wd, R(1));
+            }
+            S* co = static_cast<S*>(a.coll_dst) + L.f_off + idx;
+#pragma unroll
+            for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale);
         }
     }
-    mass = wave_sum(mass);
-    if (__lane_id() == 0 && mass != 0.0) atomicAdd(a.mass_dev, mass);
+    if (a.mass_dev) {
+        mass = wave_sum(mass);
+        if (__lane_id() == 0 && mass != 0.0) atomicAdd(a.mass_dev, mass);
+    }
 }
 
 template <int ST>
@@ -531,7 +715,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
-    const LatticeDev& L = a.lat[a.lattice];
+    const LatticeDev L = a.lat[a.lattice];
     const int x = blockIdx.x * 256 + threadIdx.x;
     const int y = a.y0 + blockIdx.y;
     if (x >= L.nx || y >= a.y1) return;
@@ -550,7 +734,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
             const uint16_t w = a.wdist[L.cell_off + idx];
             wd = w == 0xffffu ? R(1e9) : static_cast<R>(w);
         }
-        collide<R>(d, L, wd);
+        collide<R>(d, L, wd, R(1));
     }
 #pragma unroll
     for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale);
@@ -567,41 +751,73 @@ __global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, in
     }
 }
 
-template <int ST, int BX, int ROWS>
+// A group of nodes moves at most 16 bytes per plane and thread (one dwordx4).
+template <int ST, int V> constexpr bool group_fits() { return sizeof(typename Form<ST>::store) * V <= 16; }
+
+template <int ST, int BX, int ROWS, int V>
 hipError_t launch_step_t(int n_blocks, const StepArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((step_kernel<ST, BX, ROWS>), dim3(n_blocks), dim3(BX), 0, s, a);
-    return hipGetLastError();
+    if constexpr (!group_fits<ST, V>()) {
+        return hipErrorInvalidValue;
+    } else {
+        hipLaunchKernelGGL((step_kernel<ST, BX, ROWS, V>), dim3(n_blocks), dim3(BX), 0, s, a);
+        return hipGetLastError();
+    }
 }
 
+// Launch shapes (threads per block, rows per block, nodes per thread) compiled in.
+#define LBM_SHAPES(X)                                                                          \
+    X(64, 1, 1) X(128, 1, 1) X(256, 1, 1) X(256, 2, 1) X(64, 1, 2) X(128, 1, 2) X(256, 1, 2) \
+    X(64, 1, 4) X(128, 1, 4) X(256, 1, 4) X(64, 2, 4) X(128, 2, 4) X(64, 1, 8) X(128, 1, 8)
+
 template <int ST>
-hipError_t launch_step_st(int bx, int rows, int n_blocks, const StepArgs& a, hipStream_t s)
+hipError_t launch_step_st(int bx, int rows, int v, int n_blocks, const StepArgs& a, hipStream_t s)
 {
-#define LBM_SHAPE(B, RW) \
-    if (bx == B && rows == RW) return launch_step_t<ST, B, RW>(n_blocks, a, s);
-    LBM_SHAPE(64, 1) LBM_SHAPE(64, 2) LBM_SHAPE(64, 4) LBM_SHAPE(64, 8)
-    LBM_SHAPE(128, 1) LBM_SHAPE(128, 2) LBM_SHAPE(128, 4) LBM_SHAPE(128, 8)
-    LBM_SHAPE(256, 1) LBM_SHAPE(256, 2) LBM_SHAPE(256, 4) LBM_SHAPE(256, 8)
-    LBM_SHAPE(512, 1) LBM_SHAPE(512, 2) LBM_SHAPE(512, 4) LBM_SHAPE(512, 8)
+#define LBM_SHAPE(B, RW, VV) \
+    if (bx == B && rows == RW && v == VV) return launch_step_t<ST, B, RW, VV>(n_blocks, a, s);
+    LBM_SHAPES(LBM_SHAPE)
 #undef LBM_SHAPE
     return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-bool step_shape_supported(int bx, int rows)
+static int store_bytes(int storage) { return storage == ST_F64 ? 8 : (storage == ST_F32 ? 4 : 2); }
+
+bool step_shape_supported(int storage, int bx, int rows, int v)
 {
-    return (bx == 64 || bx == 128 || bx == 256 || bx == 512) &&
-           (rows == 1 || rows == 2 || rows == 4 || rows == 8);
+    if (store_bytes(storage) * v > 16) return false;
+#define LBM_SHAPE(B, RW, VV) \
+    if (bx == B && rows == RW && v == VV) return true;
+    LBM_SHAPES(LBM_SHAPE)
+#undef LBM_SHAPE
+    return false;
 }
 
-hipError_t launch_step(int storage, int bx, int rows, int n_blocks, const StepArgs& a, hipStream_t s)
+int step_shapes(int storage, int (*out)[3], int max)
+{
+    int n = 0;
+#define LBM_SHAPE(B, RW, VV)                              \
+    if (store_bytes(storage) * VV <= 16) {                \
+        if (n < max) {                                    \
+            out[n][0] = B;                                \
+            out[n][1] = RW;                               \
+            out[n][2] = VV;                               \
+        }                                                 \
+        ++n;                                              \
+    }
+    LBM_SHAPES(LBM_SHAPE)
+#undef LBM_SHAPE
+    return n;
+}
+
+hipError_t launch_step(int storage, int bx, int rows, int v, int n_blocks, const StepArgs& a, hipStream_t s)
 {
     switch (storage) {
-    case ST_F64: return launch_step_st<ST_F64>(bx, rows, n_blocks, a, s);
-    case ST_F32: return launch_step_st<ST_F32>(bx, rows, n_blocks, a, s);
-    case ST_F16S: return launch_step_st<ST_F16S>(bx, rows, n_blocks, a, s);
-    case ST_BF16S: return launch_step_st<ST_BF16S>(bx, rows, n_blocks, a, s);
+    case ST_F64: return launch_step_st<ST_F64>(bx, rows, v, n_blocks, a, s);
+    case ST_F32: return launch_step_st<ST_F32>(bx, rows, v, n_blocks, a, s);
+    case ST_F16S: return launch_step_st<ST_F16S>(bx, rows, v, n_blocks, a, s);
+    case ST_BF16S: return launch_step_st<ST_BF16S>(bx, rows, v, n_blocks, a, s);
     }
     return hipErrorInvalidValue;
 }

@@ -13,7 +13,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblbm2d.so")
+# LBM2D_LIB selects another build of the same library (tuning experiments).
+LIB_PATH = os.environ.get("LBM2D_LIB") or os.path.join(HERE, "liblbm2d.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "lbm_gpu.h")
 
 STORAGE = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
@@ -52,6 +53,7 @@ class LbmInfo(C.Structure):
         ("cells", C.c_int64), ("fluid_cells", C.c_int64), ("link_cells", C.c_int64),
         ("les_cells", C.c_int64), ("bytes_per_pop", C.c_int32), ("n_lattices", C.c_int32),
         ("device_bytes", C.c_int64), ("block_x", C.c_int32), ("rows", C.c_int32),
+        ("nodes_per_thread", C.c_int32), ("launch_options", C.c_int32),
     ]
 
 
@@ -98,8 +100,9 @@ def lib() -> C.CDLL:
         "lbm_download": (C.c_int, [_h, i32, _dp]),
         "lbm_step": (C.c_int, [_h, i32]),
         "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
-        "lbm_set_launch": (C.c_int, [_h, i32, i32, i32]),
-        "lbm_autotune": (C.c_int, [_h, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_float)]),
+        "lbm_set_launch": (C.c_int, [_h, i32, i32, i32, i32, i32]),
+        "lbm_autotune": (C.c_int, [_h, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                   C.POINTER(i32), C.POINTER(C.c_float)]),
         "lbm_forces": (C.c_int, [_h, i32, _dp, _dp, i32, C.POINTER(i32)]),
         "lbm_mass": (C.c_int, [_h, i32, _dp]),
         "lbm_nan_flag": (C.c_int, [_h, i32, C.POINTER(i32)]),
@@ -217,13 +220,15 @@ class Ensemble:
     def step_lattice(self, l: int, n: int = 1):
         _check(lib().lbm_step_lattice(self._h, l, n))
 
-    def set_launch(self, block_x=0, rows=0, graph_steps=0):
-        _check(lib().lbm_set_launch(self._h, block_x, rows, graph_steps))
+    def set_launch(self, block_x=0, rows=0, nodes_per_thread=0, options=-1, graph_steps=0):
+        _check(lib().lbm_set_launch(self._h, block_x, rows, nodes_per_thread, options, graph_steps))
 
     def autotune(self):
-        bx, rows, ms = C.c_int32(), C.c_int32(), C.c_float()
-        _check(lib().lbm_autotune(self._h, C.byref(bx), C.byref(rows), C.byref(ms)))
-        return bx.value, rows.value, ms.value
+        """Returns ((block_x, rows, nodes_per_thread, options), ms per step) of the fastest."""
+        bx, rows, v, opt, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
+        _check(lib().lbm_autotune(self._h, C.byref(bx), C.byref(rows), C.byref(v), C.byref(opt),
+                                  C.byref(ms)))
+        return (bx.value, rows.value, v.value, opt.value), ms.value
 
     def forces(self, l: int = 0, n_slots: int = 1):
         """Per-step FP64 force totals since the last call: arrays (steps, n_slots)."""

@@ -126,8 +126,9 @@ def test_autotune_keeps_state():
     ens, o = build_gpu(c, "f32")
     ens.step(3)
     before = ens.download(0)
-    bx, rows, ms = ens.autotune()
-    assert bx in (64, 128, 256, 512) and rows in (1, 2, 4, 8) and ms > 0
+    (bx, rows, nodes, options), ms = ens.autotune()
+    assert bx in (64, 128, 256, 512) and rows in (1, 2, 4, 8) and nodes in (1, 2, 4, 8) and ms > 0
+    assert options in (0, 1, 2, 3)
     np.testing.assert_array_equal(ens.download(0), before)
     ens.step(5)
     o.step(c.tau, c.u, 8)
@@ -163,3 +164,27 @@ def test_wall_distance_matches_oracle():
     c = CASES["les"]
     ens, o = build_gpu(c, "f32")
     np.testing.assert_array_equal(ens.wall_distance(0), O.wall_distance(o.obstacle))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_every_launch_shape_and_option_is_bitwise_identical(storage):
+    """Each compiled (block, rows, nodes per thread) shape, with and without the XCD
+    tile order and streaming stores, computes the same populations bit for bit."""
+    c = CASES["cylinder"]
+    shapes = [(256, 1, 1), (64, 1, 1), (256, 2, 1), (128, 1, 2), (64, 1, 4), (128, 2, 4),
+              (64, 1, 8)]
+    ref = None
+    for bx, rows, v in shapes:
+        for opt in (0, 1, 2, 3):
+            ens, _ = build_gpu(c, storage)
+            try:
+                ens.set_launch(bx, rows, v, opt)
+            except G.LbmError:
+                assert G.BYTES_PER_POP[storage] * v > 16
+                continue
+            ens.step(7)
+            f = ens.download(0)
+            if ref is None:
+                ref = f
+            else:
+                np.testing.assert_array_equal(f, ref, err_msg=f"shape {(bx, rows, v)} options {opt}")
