@@ -107,10 +107,14 @@ def graph_step_ms(ens, torch, replays=20, graph_steps=100):
     return float(np.mean([evs[i].elapsed_time(evs[i + 1]) for i in range(replays)])) / graph_steps
 
 
-def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1):
+def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1, launch=""):
     ens, mask = cylinder_lattice(nx, ny, nx // 4, ny // 2, r, tau, 0.1, storage, stream,
                                  les=les, cs=cs, perturb=False)
-    shape, _ = ens.autotune()
+    if launch:
+        shape = tuple(int(v) for v in launch.split(","))
+        ens.set_launch(*shape)
+    else:
+        shape, _ = ens.autotune()
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
     kms = kernel_launch_ms(ens, torch, n=50)
@@ -283,15 +287,55 @@ print(json.dumps({{'kind': kind, 'steps': n, 'seconds': total, 'threads': thread
                       f"FP64 AoS, OpenMP {r['threads']} threads"}
 
 
+def profiled_traffic(path=os.path.join(ROOT, "profiles", "step_kernel_cylinder800x300_f32.md")):
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration)."""
+    try:
+        for line in open(path):
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50000)
     ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--launch", default="", help="fix the step-kernel launch: bx,rows,nodes,options "
+                                                   "(default: autotune)")
     ap.add_argument("--large", action="store_true", help="also time the 32768x16384 domain")
     ap.add_argument("--no-forms", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", default="", help="run one extra form only and print it (profiling): "
+                                                "f32_6400x2400_les, f16s_6400x2400_les, bf16s_6400x2400_les, "
+                                                "batched_256")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
+    if args.form:
+        import torch
+        torch.cuda.init()
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            if args.form == "batched_256":
+                out = bench_ensemble(200, 20, torch, stream.cuda_stream)
+            else:
+                storage = args.form.split("_")[0]
+                out = bench_form(6400, 2400, 240, 0.644, storage, 100, 10, torch, stream.cuda_stream, les=True,
+                                 launch=args.launch)
+        print(json.dumps({args.form: out}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(the lattice update is measured on a single MI355X).", file=sys.stderr)
@@ -305,9 +349,14 @@ def main():
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         ens, mask = cylinder_lattice(800, 300, 200, 150, 30, 0.68, 0.1, "f32", stream.cuda_stream)
-        shape, _ = ens.autotune()
+        if args.launch:
+            shape = tuple(int(v) for v in args.launch.split(","))
+            ens.set_launch(*shape)
+        else:
+            shape, _ = ens.autotune()
         ens.set_launch(graph_steps=100)
         info = ens.info()
+        print(f"# headline lattice ready, launch {shape}", file=sys.stderr, flush=True)
         t = time_steps(ens, args.steps, args.warmup, torch)
         glups = info.cells * args.steps / t / 1e9
         kms = kernel_launch_ms(ens, torch)
@@ -316,21 +365,23 @@ def main():
         fx, fy = ens.forces(0)
         cd_last = 2.0 * fx[-1, 0] / (60 * 0.01)
         ens.close()
+        print(f"# headline done: {glups:.1f} GLUPS", file=sys.stderr, flush=True)
 
         forms = {}
         if not args.no_forms:
             big_steps = 200
-            forms["fp32_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "f32", big_steps, 20,
-                                                     torch, stream.cuda_stream, les=True)
-            forms["fp16s_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "f16s", big_steps, 20,
-                                                      torch, stream.cuda_stream, les=True)
-            forms["bf16s_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "bf16s", big_steps, 20,
-                                                      torch, stream.cuda_stream, les=True)
-            forms["fp16s_over_fp32"] = forms["fp16s_6400x2400_les"]["GLUPS"] / forms["fp32_6400x2400_les"]["GLUPS"]
+            for storage in ("f32", "f16s", "bf16s"):
+                forms[f"{storage}_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, storage, big_steps, 20,
+                                                               torch, stream.cuda_stream, les=True)
+                print(f"# {storage} 6400x2400 done", file=sys.stderr, flush=True)
+            forms["fp16s_over_fp32"] = forms["f16s_6400x2400_les"]["GLUPS"] / forms["f32_6400x2400_les"]["GLUPS"]
             forms["batched_256"] = bench_ensemble(1000, 100, torch, stream.cuda_stream)
+            print("# batched ensemble done", file=sys.stderr, flush=True)
             if args.large:
-                forms["fp32_32768x16384"] = bench_form(32768, 16384, 60, 0.505, "f32", 20, 2, torch,
-                                                       stream.cuda_stream, les=True, cs=0.12)
+                for storage in ("f32", "f16s", "bf16s"):
+                    forms[f"{storage}_32768x16384_les"] = bench_form(32768, 16384, 60, 0.505, storage, 20, 2,
+                                                                     torch, stream.cuda_stream, les=True, cs=0.12)
+                    print(f"# {storage} 32768x16384 done", file=sys.stderr, flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline()
     achieved = bytes_launch / (kms * 1e-3) / 1e9
@@ -361,7 +412,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(),
             "kernel_ms": kms,
             "bytes_per_launch": bytes_launch,
         },

@@ -445,10 +445,11 @@ int commit(Ensemble* E)
             if ((rc = h2d(E, E->d_wdist, wd.data(), wd.size() * 2)) != LBM_OK) return rc;
             if (slots != E->total_slots) {
                 E->total_slots = slots;
-                if ((rc = dev_alloc(E, &E->d_acc, static_cast<size_t>(slots) * 4)) != LBM_OK) return rc;
-                if ((rc = dev_alloc(E, &E->d_dummy_acc, static_cast<size_t>(slots) * 4)) != LBM_OK) return rc;
+                const size_t acc_n = static_cast<size_t>(slots) * 2 * kAccCopies * 2;
+                if ((rc = dev_alloc(E, &E->d_acc, acc_n)) != LBM_OK) return rc;
+                if ((rc = dev_alloc(E, &E->d_dummy_acc, acc_n)) != LBM_OK) return rc;
                 if ((rc = dev_alloc(E, &E->d_hist, static_cast<size_t>(slots) * E->R * 2)) != LBM_OK) return rc;
-                LBM_HIP(hipMemsetAsync(E->d_acc, 0, static_cast<size_t>(slots) * 4 * 8, E->stream));
+                LBM_HIP(hipMemsetAsync(E->d_acc, 0, acc_n * 8, E->stream));
                 LBM_HIP(hipMemsetAsync(E->d_hist, 0, static_cast<size_t>(slots) * E->R * 16, E->stream));
                 E->graph_dirty = true;
             }
@@ -970,7 +971,7 @@ int lbm_upload(lbm_handle h, int32_t l, const double* f)
     H.uploaded = true;
     // clear this lattice's accumulators (both parities) and NaN flag
     std::vector<double> z(static_cast<size_t>(H.n_slots) * 2, 0.0);
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < 2 * kAccCopies; ++p)
         if ((rc = h2d(E, E->d_acc + (static_cast<int64_t>(p) * E->total_slots + H.slot_base) * 2, z.data(),
                       z.size() * 8)) != LBM_OK)
             return rc;
@@ -1174,12 +1175,18 @@ int lbm_forces(lbm_handle h, int32_t l, double* fx, double* fy, int32_t max_step
     if ((rc = drain(E, l)) != LBM_OK) return rc;
     if (H.steps >= 1 && H.read_pos == H.steps - 1) {
         LBM_HIP(hipStreamSynchronize(E->stream));
-        std::vector<double> cur(static_cast<size_t>(H.n_slots) * 2);
-        const int64_t off = (((H.steps - 1) & 1) * static_cast<int64_t>(E->total_slots) + H.slot_base) * 2;
-        if ((rc = d2h(E, cur.data(), E->d_acc + off, cur.size() * 8)) != LBM_OK) return rc;
+        // the newest step is still in the accumulator copies: sum them in the kernel's order
+        std::vector<double> all(static_cast<size_t>(kAccCopies) * E->total_slots * 2);
+        const int64_t off = ((H.steps - 1) & 1) * static_cast<int64_t>(kAccCopies) * E->total_slots * 2;
+        if ((rc = d2h(E, all.data(), E->d_acc + off, all.size() * 8)) != LBM_OK) return rc;
         for (int s = 0; s < H.n_slots; ++s) {
-            H.pend_fx.push_back(cur[2 * s]);
-            H.pend_fy.push_back(cur[2 * s + 1]);
+            double fx = 0.0, fy = 0.0;
+            for (int cp = 0; cp < kAccCopies; ++cp) {
+                fx += all[(static_cast<size_t>(cp) * E->total_slots + H.slot_base + s) * 2];
+                fy += all[(static_cast<size_t>(cp) * E->total_slots + H.slot_base + s) * 2 + 1];
+            }
+            H.pend_fx.push_back(fx);
+            H.pend_fy.push_back(fy);
         }
         H.read_pos = H.steps;
     }

@@ -5,6 +5,11 @@
 
 namespace lbm {
 
+// Force accumulators are kept in this many copies (chosen by workgroup) so that the
+// FP64 atomics of different workgroups rarely meet on one address; the copies are
+// summed in fixed order when a step's totals are moved to the history.
+constexpr int kAccCopies = 16;
+
 // Per-cell flag byte (pitch layout, one per node).
 enum : uint8_t {
     CELL_SOLID = 1u << 0,    // obstacle node

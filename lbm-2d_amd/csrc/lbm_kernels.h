@@ -32,7 +32,7 @@ struct StepArgs {
     const uint16_t* wdist;
     const int64_t* steps;       // per-lattice step count at the start of the batch
     const int32_t* par;         // per-lattice index of the buffer holding the state
-    double* acc;                // [2][total_slots][2] force accumulators
+    double* acc;                // [2][kAccCopies][total_slots][2] force accumulators
     double* hist;               // [total_slots][R][2] force history ring
     int32_t R;
     int32_t total_slots;

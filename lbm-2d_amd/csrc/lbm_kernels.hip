@@ -593,7 +593,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
     const int sp = a.abs_mode ? a.src_par : (a.par[lat_id] ^ (a.step_arg & 1));
     const S* src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
     S* dst = static_cast<S*>(a.dst_override ? a.dst_override : (sp ? a.buf0 : a.buf1));
-    double* accp = a.acc + static_cast<int64_t>(k & 1) * a.total_slots * 2;
+    double* accp = a.acc + (static_cast<int64_t>(k & 1) * kAccCopies + blockIdx.x % kAccCopies) * a.total_slots * 2;
 
     // Step k-1 is complete: move its force totals to the history, clear its accumulator.
     if (blockIdx.x == 0) {
@@ -602,14 +602,20 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
             const int64_t kl = a.abs_mode ? a.k_abs : a.steps[l] + a.step_arg;
             if (kl < 1) continue;
             const LatticeDev& Ll = a.lat[l];
-            double* prev = a.acc + static_cast<int64_t>((kl - 1) & 1) * a.total_slots * 2;
+            double* prev = a.acc + static_cast<int64_t>((kl - 1) & 1) * kAccCopies * a.total_slots * 2;
             const int64_t row = (kl - 1) % a.R;
             for (int s = 0; s < Ll.n_slots; ++s) {
                 const int gs = Ll.slot_base + s;
-                a.hist[(static_cast<int64_t>(gs) * a.R + row) * 2] = prev[2 * gs];
-                a.hist[(static_cast<int64_t>(gs) * a.R + row) * 2 + 1] = prev[2 * gs + 1];
-                prev[2 * gs] = 0.0;
-                prev[2 * gs + 1] = 0.0;
+                double fx = 0.0, fy = 0.0;
+                for (int cp = 0; cp < kAccCopies; ++cp) {
+                    double* q = prev + (static_cast<int64_t>(cp) * a.total_slots + gs) * 2;
+                    fx += q[0];
+                    fy += q[1];
+                    q[0] = 0.0;
+                    q[1] = 0.0;
+                }
+                a.hist[(static_cast<int64_t>(gs) * a.R + row) * 2] = fx;
+                a.hist[(static_cast<int64_t>(gs) * a.R + row) * 2 + 1] = fy;
             }
         }
     }
