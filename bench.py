@@ -13,8 +13,10 @@ lattices stepped one launch each, and the 32768x16384 domain when
 
 ``roofline`` is for the dominant kernel (the fused step kernel of the headline
 workload): algorithmic bytes per launch = N_fluid*18*s + N_cells*1 (s = bytes per
-stored population), divided by the average launch duration measured here with
-HIP events on the library's stream, against 8 TB/s.
+stored population; + 2 B per node of the u16 wall distance with LES), divided by the
+average launch duration measured here with HIP events on the library's stream
+around hipGraph replays, against 8 TB/s.  ``traffic`` is the HBM bytes per launch
+read from the committed rocprofv3 PMC summary of the same workload.
 
 ``cpu_baseline`` times the reference itself (oracle/_ref/libref_fast.so, built
 from the reference sources) or, when that is absent, the FP64 oracle, on this
@@ -80,22 +82,10 @@ def time_steps(ens, steps, warmup, torch):
     return time.perf_counter() - t0
 
 
-def kernel_launch_ms(ens, torch, n=200):
-    """Average duration of one step-kernel launch, HIP events on the library's stream."""
-    stream = torch.cuda.current_stream()
-    ens.step_lattice(0, 10)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
-    for i in range(n):
-        starts[i].record(stream)
-        ens.step_lattice(0, 1)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-
-
 def graph_step_ms(ens, torch, replays=20, graph_steps=100):
-    """Average time per step inside hipGraph replays (events around each replay)."""
+    """Average duration of one step-kernel launch: HIP events on the library's stream
+    around each replay of a hipGraph of graph_steps launches, whose kernels run back to
+    back (the per-step counter update is one 1-thread kernel per replay)."""
     stream = torch.cuda.current_stream()
     ens.step(graph_steps)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(replays + 1)]
@@ -117,7 +107,7 @@ def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False,
         shape, _ = ens.autotune()
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
-    kms = kernel_launch_ms(ens, torch, n=50)
+    kms = graph_step_ms(ens, torch, replays=5)
     b = algorithmic_bytes(info, 2)
     m0 = ens.mass(0)
     out = {
@@ -359,8 +349,7 @@ def main():
         print(f"# headline lattice ready, launch {shape}", file=sys.stderr, flush=True)
         t = time_steps(ens, args.steps, args.warmup, torch)
         glups = info.cells * args.steps / t / 1e9
-        kms = kernel_launch_ms(ens, torch)
-        gms = graph_step_ms(ens, torch)
+        kms = graph_step_ms(ens, torch)
         bytes_launch = algorithmic_bytes(info, 0)
         fx, fy = ens.forces(0)
         cd_last = 2.0 * fx[-1, 0] / (60 * 0.01)
@@ -404,7 +393,7 @@ def main():
                         "Zou/He inlet, outflow copy, periodic y, seed-42 perturbation, "
                         "hipGraph of 100 steps",
             "cells": info.cells, "fluid_cells": info.fluid_cells, "launch": list(shape),
-            "graph_step_ms": gms, "cd_last_step": cd_last,
+            "cd_last_step": cd_last,
         },
         "roofline": {
             "bound": "hbm",
