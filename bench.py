@@ -50,18 +50,7 @@ def cylinder_lattice(nx, ny, cx, cy, r, tau, u, storage, stream, les=False, cs=0
     f = G.equilibrium(np.ones((ny, nx)), np.full((ny, nx), u), np.zeros((ny, nx)))
     if perturb:
         # seed-42 transverse perturbation of the cylinder driver (src/main.cpp:68-83)
-        import oracle as O
-        g = O.MT19937(42)
-        for x in range(cx + 5, min(nx, cx + 60)):
-            for y in range(ny):
-                if mask[y, x]:
-                    continue
-                dv = g.uniform(-1e-4, 1e-4)
-                fr = f[y, x]
-                rho = fr.sum()
-                uu = (fr * G.CXV).sum() / rho
-                vv = (fr * G.CYV).sum() / rho
-                f[y, x] = G.equilibrium(np.array(rho), np.array(uu), np.array(vv + dv))
+        G.perturb_seed42(f, mask, cx + 5, cx + 60)
     ens.upload(0, f)
     return ens, mask
 

@@ -142,6 +142,10 @@ int lbm_set_params(lbm_handle h, int32_t lattice, double tau, double u_in);
 /* State transfer in the reference layout (AoS FP64, nx*ny*9 doubles). */
 int lbm_upload(lbm_handle h, int32_t lattice, const double* f_aos);
 int lbm_download(lbm_handle h, int32_t lattice, double* f_aos);
+/* Initial state feq(rho, u, v) on every node, built on the device: the drivers'
+ * initialisation loop (src/main.cpp:60-66, src/cavity.cpp:78-84) without a host array,
+ * for lattices too large to stage in AoS FP64.  Resets the step count like an upload. */
+int lbm_init_equilibrium(lbm_handle h, int32_t lattice, double rho, double u, double v);
 
 /* Advances every lattice by n steps: one launch per step covering all lattices,
  * replayed from a hipGraph of graph_steps launches when n allows. */

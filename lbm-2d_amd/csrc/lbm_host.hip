@@ -933,28 +933,32 @@ int lbm_set_params(lbm_handle h, int32_t l, double tau, double u_in)
     return LBM_OK;
 }
 
-int lbm_upload(lbm_handle h, int32_t l, const double* f)
+namespace lbm {
+namespace {
+// Prologue over all rows of lattice l (f null: uniform equilibrium), then resets the
+// lattice's step count, force history and NaN flag.
+int load_state(Ensemble* E, int l, const double* f, double rho, double u, double v)
 {
-    using namespace lbm;
-    int rc = lattice_ok(h, l);
-    if (rc) return rc;
-    if (!f) return fail(LBM_ERR_INVALID, "null population array");
-    Ensemble* E = h;
+    int rc;
     if ((rc = commit(E)) != LBM_OK) return rc;
     HostLattice& H = E->lat[l];
     const lbm_desc& d = H.d;
     const int64_t row_bytes = static_cast<int64_t>(d.nx) * 72;
-    const int chunk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(d.ny, kStagingBytes / row_bytes)));
+    const int chunk = f ? static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(d.ny, kStagingBytes / row_bytes)))
+                        : d.ny;
     for (int y0 = 0; y0 < d.ny; y0 += chunk) {
         const int y1 = std::min(d.ny, y0 + chunk);
-        if ((rc = h2d(E, E->d_stage, f + static_cast<int64_t>(y0) * d.nx * 9, (y1 - y0) * row_bytes)) != LBM_OK)
+        if (f && (rc = h2d(E, E->d_stage, f + static_cast<int64_t>(y0) * d.nx * 9, (y1 - y0) * row_bytes)) != LBM_OK)
             return rc;
         PrologueArgs p{};
         p.lat = E->d_lat;
         p.lattice = l;
         p.y0 = y0;
         p.y1 = y1;
-        p.in = E->d_stage;
+        p.in = f ? E->d_stage : nullptr;
+        p.eq_rho = rho;
+        p.eq_u = u;
+        p.eq_v = v;
         p.state_dst = E->buf[1 - H.par];
         p.coll_dst = E->buf[H.par];
         p.flags = E->d_flags;
@@ -969,7 +973,7 @@ int lbm_upload(lbm_handle h, int32_t l, const double* f)
     H.pend_fx.clear();
     H.pend_fy.clear();
     H.uploaded = true;
-    // clear this lattice's accumulators (both parities) and NaN flag
+    // clear this lattice's accumulators (both parities, every copy) and NaN flag
     std::vector<double> z(static_cast<size_t>(H.n_slots) * 2, 0.0);
     for (int p = 0; p < 2 * kAccCopies; ++p)
         if ((rc = h2d(E, E->d_acc + (static_cast<int64_t>(p) * E->total_slots + H.slot_base) * 2, z.data(),
@@ -979,6 +983,24 @@ int lbm_upload(lbm_handle h, int32_t l, const double* f)
     if ((rc = h2d(E, E->d_nan + l, &zero, 4)) != LBM_OK) return rc;
     E->counters_stale = true;
     return LBM_OK;
+}
+}  // namespace
+}  // namespace lbm
+
+int lbm_upload(lbm_handle h, int32_t l, const double* f)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!f) return fail(LBM_ERR_INVALID, "null population array");
+    return lbm::load_state(h, l, f, 0.0, 0.0, 0.0);
+}
+
+int lbm_init_equilibrium(lbm_handle h, int32_t l, double rho, double u, double v)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!(rho > 0.0)) return fail(LBM_ERR_INVALID, "density must be positive");
+    return lbm::load_state(h, l, nullptr, rho, u, v);
 }
 
 int lbm_download(lbm_handle h, int32_t l, double* f)

@@ -69,7 +69,8 @@ struct PrologueArgs {
     const LatticeDev* lat;
     int32_t lattice;
     int32_t y0, y1;
-    const double* in;
+    const double* in;           // null: uniform equilibrium feq(eq_rho, eq_u, eq_v)
+    double eq_rho, eq_u, eq_v;
     void* state_dst;
     void* coll_dst;
     const uint8_t* flags;

@@ -726,10 +726,22 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
     const int y = a.y0 + blockIdx.y;
     if (x >= L.nx || y >= a.y1) return;
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
-    const double* in = a.in + (static_cast<int64_t>(y - a.y0) * L.nx + x) * 9;
     R d[9];
+    if (a.in) {
+        const double* in = a.in + (static_cast<int64_t>(y - a.y0) * L.nx + x) * 9;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) d[j] = static_cast<R>(in[j] - WD[j]);
+        for (int j = 0; j < 9; ++j) d[j] = static_cast<R>(in[j] - WD[j]);
+    } else {
+        // uniform equilibrium feq(rho, u, v) - w, as the case drivers' initial loop,
+        // unfused so that it rounds as compute_equilibrium (src/lbm_types.hpp:269-273)
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const double eu = CX[j] * a.eq_u + CY[j] * a.eq_v;
+            const double uu = a.eq_u * a.eq_u + a.eq_v * a.eq_v;
+            d[j] = static_cast<R>(WD[j] * a.eq_rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uu) - WD[j]);
+        }
+    }
     S* st = static_cast<S*>(a.state_dst) + L.f_off + idx;
     S* co = static_cast<S*>(a.coll_dst) + L.f_off + idx;
 #pragma unroll

@@ -98,6 +98,7 @@ def lib() -> C.CDLL:
         "lbm_set_params": (C.c_int, [_h, i32, dbl, dbl]),
         "lbm_upload": (C.c_int, [_h, i32, _dp]),
         "lbm_download": (C.c_int, [_h, i32, _dp]),
+        "lbm_init_equilibrium": (C.c_int, [_h, i32, dbl, dbl, dbl]),
         "lbm_step": (C.c_int, [_h, i32]),
         "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
         "lbm_set_launch": (C.c_int, [_h, i32, i32, i32, i32, i32]),
@@ -208,6 +209,9 @@ class Ensemble:
         a = np.ascontiguousarray(f, dtype=np.float64).reshape(ny, nx, 9)
         _check(lib().lbm_upload(self._h, l, a.ctypes.data_as(_dp)))
 
+    def init_equilibrium(self, l: int, rho: float = 1.0, u: float = 0.0, v: float = 0.0):
+        _check(lib().lbm_init_equilibrium(self._h, l, rho, u, v))
+
     def download(self, l: int = 0) -> np.ndarray:
         ny, nx = self.shape(l)
         out = np.empty((ny, nx, 9), dtype=np.float64)
@@ -283,6 +287,8 @@ class Ensemble:
 W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
 CXV = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
 CYV = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+_CX = tuple(int(c) for c in CXV)
+_CY = tuple(int(c) for c in CYV)
 
 
 def equilibrium(rho, u, v) -> np.ndarray:
@@ -291,6 +297,135 @@ def equilibrium(rho, u, v) -> np.ndarray:
     eu = CXV * u[..., None] + CYV * v[..., None]
     uu = (u * u + v * v)[..., None]
     return W * rho[..., None] * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uu)
+
+
+class Mt19937:
+    """std::mt19937 (Matsumoto & Nishimura 1998) with libstdc++'s
+    uniform_real_distribution<double>: generate_canonical over two 32-bit draws,
+    (g1 + g2 * 2^32) / 2^64, kept below 1.  The generator of the drivers' seed-42
+    perturbation (src/main.cpp:68-69)."""
+
+    def __init__(self, seed: int = 5489):
+        mt = [seed & 0xFFFFFFFF]
+        for i in range(1, 624):
+            mt.append((1812433253 * (mt[-1] ^ (mt[-1] >> 30)) + i) & 0xFFFFFFFF)
+        self._mt = mt
+        self._i = 624
+
+    def _twist(self):
+        mt = self._mt
+        for k in range(624):
+            y = (mt[k] & 0x80000000) | (mt[(k + 1) % 624] & 0x7FFFFFFF)
+            mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+        self._i = 0
+
+    def next(self) -> int:
+        if self._i >= 624:
+            self._twist()
+        y = self._mt[self._i]
+        self._i += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y & 0xFFFFFFFF
+
+    def uniform(self, a: float, b: float) -> float:
+        lo = float(self.next())
+        r = (lo + float(self.next()) * 4294967296.0) / 18446744073709551616.0
+        if r >= 1.0:
+            r = 1.0 - 2.0 ** -53
+        return r * (b - a) + a
+
+
+def perturb_seed42(f: np.ndarray, obstacle: np.ndarray, x0: int, x1: int, amp: float = 1e-4) -> None:
+    """The cylinder drivers' shedding trigger (src/main.cpp:67-83), in place on AoS
+    populations f (ny, nx, 9): columns x0 <= x < x1, rows in order, fluid nodes get
+    feq(rho, u, v + U(-amp, amp)) with the node's own moments."""
+    g = Mt19937(42)
+    ny, nx = obstacle.shape
+    for x in range(x0, min(nx, x1)):
+        for y in range(ny):
+            if obstacle[y, x]:
+                continue
+            dv = g.uniform(-amp, amp)
+            fr = f[y, x].tolist()
+            rho = 0.0
+            jx = 0.0
+            jy = 0.0
+            for i in range(9):
+                rho += fr[i]
+                jx += fr[i] * _CX[i]
+                jy += fr[i] * _CY[i]
+            u, v = jx / rho, jy / rho
+            f[y, x] = equilibrium(np.array(rho), np.array(u), np.array(v + dv))
+
+
+# ---- force analysis ----------------------------------------------------------------
+
+def coefficients(fx, fy, length: float, u: float):
+    """Cd, Cl = 2F / (L u^2) per step (src/main.cpp:112-115)."""
+    k = 2.0 / (length * u * u)
+    return np.asarray(fx, float) * k, np.asarray(fy, float) * k
+
+
+def late_stats(cd, cl):
+    """Mean Cd and Cl amplitude (max - min) / 2 over the latter half of the history,
+    with Cl's extremes started at 0 (src/main.cpp:185-195)."""
+    cd, cl = np.asarray(cd, float), np.asarray(cl, float)
+    n_late = int(len(cd) * 0.5)
+    cd_mean = float(np.cumsum(cd[n_late:])[-1] / (len(cd) - n_late))
+    tail = cl[n_late:]
+    return cd_mean, float((max(0.0, tail.max()) - min(0.0, tail.min())) / 2.0)
+
+
+def welch_psd(x, nperseg: int, noverlap: int):
+    """One-sided Welch power spectral density at fs = 1: periodic Hann window,
+    per-segment mean removed, density scaling (the estimator scripts/postprocess.py:423-427
+    takes from scipy.signal.welch)."""
+    x = np.asarray(x, float)
+    step = nperseg - noverlap
+    n_seg = (len(x) - noverlap) // step
+    win = 0.5 - 0.5 * np.cos(2.0 * np.pi * np.arange(nperseg) / nperseg)
+    acc = np.zeros(nperseg // 2 + 1)
+    for k in range(n_seg):
+        seg = x[k * step:k * step + nperseg]
+        acc += np.abs(np.fft.rfft((seg - seg.mean()) * win)) ** 2
+    psd = acc / n_seg / (win * win).sum()
+    psd[1:-1 if nperseg % 2 == 0 else None] *= 2.0
+    return np.fft.rfftfreq(nperseg, 1.0), psd
+
+
+def strouhal_welch(cl, length: float, u: float):
+    """Strouhal number as scripts/postprocess.py:386-441 computes it: drop the first fifth,
+    Welch with nperseg = min(n/4, 16384) and 50% overlap, peak in [0.01, 0.5] U/L,
+    St = f L / U.  None when the history is too short or the band empty."""
+    cl = np.asarray(cl, float)
+    t = cl[len(cl) // 5:]
+    if len(t) < 64:
+        return None
+    nperseg = min(len(t) // 4, 16384)
+    if nperseg < 4:
+        return None
+    f, psd = welch_psd(t - t.mean(), nperseg, nperseg // 2)
+    band = (f >= 0.01 * u / length) & (f <= 0.5 * u / length)
+    if not band.any():
+        return None
+    return float(f[band][np.argmax(psd[band])] * length / u)
+
+
+def strouhal_crossings(cl, length: float, u: float, tail: float = 0.5):
+    """Strouhal number from the mean period between upward crossings of Cl's mean over the
+    last `tail` of the history, crossings placed by linear interpolation.  Resolves St
+    far below the Welch bin width 1/nperseg when the history holds few periods."""
+    c = np.asarray(cl, float)
+    c = c[int(len(c) * (1.0 - tail)):]
+    c = c - c.mean()
+    i = np.nonzero((c[:-1] < 0.0) & (c[1:] >= 0.0))[0]
+    if len(i) < 2:
+        return None
+    t = i + (-c[i]) / (c[i + 1] - c[i])
+    return float(length / ((t[-1] - t[0]) / (len(t) - 1)) / u)
 
 
 def circle_mask(nx, ny, cx, cy, r) -> np.ndarray:

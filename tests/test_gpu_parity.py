@@ -188,3 +188,28 @@ def test_every_launch_shape_and_option_is_bitwise_identical(storage):
                 ref = f
             else:
                 np.testing.assert_array_equal(f, ref, err_msg=f"shape {(bx, rows, v)} options {opt}")
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s"])
+@pytest.mark.parametrize("name", ["cylinder", "les"])
+def test_device_equilibrium_init_equals_upload(name, storage):
+    """lbm_init_equilibrium builds the drivers' uniform initial state on the device:
+    bit for bit what uploading compute_equilibrium's array gives, before and after steps."""
+    from cases import apply_gpu_geometry, gpu_desc
+    c = CASES[name]
+    o = build_oracle(c)
+    nx, ny = o.nx, o.ny
+    one = np.ones((ny, nx))
+    feq = G.equilibrium(one, 0.1 * one, 0.0 * one)
+    a = G.Ensemble([gpu_desc(c)], storage=storage)
+    b = G.Ensemble([gpu_desc(c)], storage=storage)
+    for e in (a, b):
+        apply_gpu_geometry(e, 0, c, o.obstacle.copy())
+    a.init_equilibrium(0, 1.0, 0.1, 0.0)
+    b.upload(0, feq)
+    np.testing.assert_array_equal(a.download(0), b.download(0))
+    a.step(7)
+    b.step(7)
+    np.testing.assert_array_equal(a.download(0), b.download(0))
+    np.testing.assert_array_equal(a.forces(0)[0], b.forces(0)[0])
+    assert a.steps_done(0) == 7
