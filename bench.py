@@ -47,11 +47,13 @@ def cylinder_lattice(nx, ny, cx, cy, r, tau, u, storage, stream, les=False, cs=0
     mask = G.circle_mask(nx, ny, cx, cy, r)
     ens.set_obstacle(0, mask)
     ens.set_bb_circle(0, cx, cy, r, G.BB_MEI)
-    f = G.equilibrium(np.ones((ny, nx)), np.full((ny, nx), u), np.zeros((ny, nx)))
     if perturb:
+        f = G.equilibrium(np.ones((ny, nx)), np.full((ny, nx), u), np.zeros((ny, nx)))
         # seed-42 transverse perturbation of the cylinder driver (src/main.cpp:68-83)
         G.perturb_seed42(f, mask, cx + 5, cx + 60)
-    ens.upload(0, f)
+        ens.upload(0, f)
+    else:
+        ens.init_equilibrium(0, 1.0, u, 0.0)
     return ens, mask
 
 
@@ -228,6 +230,121 @@ def bench_ensemble(steps, warmup, torch, stream, n=256):
     return out
 
 
+def large_domain(seed=42, nx=32768, ny=16384, n_circles=64, n_plates=64):
+    """Obstacles of the 32768x16384 domain: n_circles circles r~U[20,60] (exact-circle
+    Mei links, force slots 1..n_circles) and n_plates flat plates 155x2 at angle of attack
+    ~U[-15,15] deg (polygon staircase, simple bounce-back, force slot 0).  Centres uniform
+    in x in [nx/16, 15nx/16) and in y clear of the periodic seam, bounding circles at least
+    8 cells apart; numpy Generator(PCG64(seed)), circles first.  Returns the mask and the
+    explicit link list."""
+    import lbm2d as G
+    rng = np.random.default_rng(seed)
+    mask = np.zeros((ny, nx), np.uint8)
+    placed, links = [], []
+    half_diag = float(np.hypot(77.5, 1.0))
+    for k in range(n_circles + n_plates):
+        circle = k < n_circles
+        while True:
+            if circle:
+                r = int(rng.integers(20, 61))
+                br = float(r)
+            else:
+                aoa = float(rng.uniform(-15.0, 15.0))
+                br = half_diag
+            cx = int(rng.integers(nx // 16, nx - nx // 16))
+            cy = int(rng.integers(int(br) + 4, ny - int(br) - 4))
+            if all((cx - a) ** 2 + (cy - b) ** 2 > (br + c + 8) ** 2 for a, b, c in placed):
+                break
+        placed.append((cx, cy, br))
+        x0, y0 = cx - int(br) - 2, cy - int(br) - 2
+        w = h = 2 * int(br) + 5
+        if circle:
+            yy, xx = np.mgrid[y0:y0 + h, x0:x0 + w]
+            dx, dy = (xx - cx).astype(float), (yy - cy).astype(float)
+            mask[y0:y0 + h, x0:x0 + w] |= (np.sqrt(dx * dx + dy * dy) < r).astype(np.uint8)
+            sub = mask[y0:y0 + h, x0:x0 + w]
+            ys, xs = np.nonzero(sub[1:-1, 1:-1] == 0)
+            ys, xs = ys + 1, xs + 1
+            for i in range(1, 9):
+                sx, sy = (xs + G.CXV[i] + x0 - cx).astype(float), (ys + G.CYV[i] + y0 - cy).astype(float)
+                hit = (sub[ys + G.CYV[i], xs + G.CXV[i]] != 0) & (np.sqrt(sx * sx + sy * sy) < r)
+                for yl, xl in zip(ys[hit], xs[hit]):
+                    x, y = x0 + int(xl), y0 + int(yl)
+                    links.append((y * nx + x, i, G.BB_MEI, 1 + k, _q_circle(cx, cy, r, x, y, i), 0.0, 0.0))
+        else:
+            xy = G.plate_polygon(float(cx), float(cy), 155.0, aoa, 2.0)
+            mask[y0:y0 + h, x0:x0 + w] |= G.polygon_mask(w, h, xy - np.array([x0, y0], float))
+    return mask, links
+
+
+def _velocity_error(a, b, fluid, y0, y1):
+    """Sum of squared and max velocity-difference magnitudes over fluid nodes of rows [y0,y1)."""
+    _, ua, va = a.macros(0, y0, y1)
+    _, ub, vb = b.macros(0, y0, y1)
+    fl = fluid[y0:y1]
+    d2 = ((ua - ub) ** 2 + (va - vb) ** 2)[fl]
+    return float(d2.sum()), float(np.sqrt(d2.max())) if d2.size else 0.0, int(fl.sum())
+
+
+def bench_large(steps, warmup, torch, stream, horizon=100, seed=42):
+    """The 32768x16384 LES domain (tau=0.505, u=0.1, cs=0.12, periodic y): FP64 / FP32 /
+    FP16S / BF16S lattices from the same geometry and a uniform equilibrium start; after
+    `horizon` steps the velocity field of each reduced form against FP32 and against the
+    FP64 form (RMS and max of |u - u_ref| over fluid nodes, in units of u_in); then
+    autotuned throughput of each reduced form."""
+    import lbm2d as G
+    nx, ny, tau, u, cs = 32768, 16384, 0.505, 0.1, 0.12
+    t0 = time.perf_counter()
+    mask, links = large_domain(seed, nx, ny)
+    t_geom = time.perf_counter() - t0
+    n_slots = 1 + max(k[3] for k in links)
+    lat = {}
+    for storage in ("f64", "f32", "f16s", "bf16s"):
+        t1 = time.perf_counter()
+        d = G.desc_for_case(G.CYLINDER, nx, ny, tau=tau, u_in=u, les=True, cs=cs)
+        e = G.Ensemble([d], storage=storage, stream=stream)
+        e.set_obstacle(0, mask)
+        e.set_force_slots(0, n_slots)
+        e.set_links(0, links)
+        e.init_equilibrium(0, 1.0, u, 0.0)
+        if storage == "f64":
+            m0 = e.mass(0)
+        e.step(horizon)
+        e.sync()
+        lat[storage] = e
+        print(f"# 32768x16384 {storage}: built and stepped {horizon} in {time.perf_counter() - t1:.1f} s",
+              file=sys.stderr, flush=True)
+    out = {"cells": nx * ny, "obstacles": "64 circles r 20-60 (Mei links) + 64 plates 155x2 (staircase)",
+           "seed": seed, "links": len(links), "host_geometry_s": t_geom, "horizon": horizon}
+    strip = 512
+    fluid = mask == 0
+    out["fp64_mass_drift"] = lat["f64"].mass(0) / m0 - 1.0
+    for ref_name in ("f64", "f32"):
+        for s in ("f32", "f16s", "bf16s"):
+            if s == ref_name:
+                continue
+            sq, mx, n = 0.0, 0.0, 0
+            for y0 in range(0, ny, strip):
+                a, b, c = _velocity_error(lat[s], lat[ref_name], fluid, y0, min(ny, y0 + strip))
+                sq, mx, n = sq + a, max(mx, b), n + c
+            out[f"vel_err_{s}_vs_{ref_name}"] = {"rms": float(np.sqrt(sq / n)) / u, "max": mx / u}
+        print(f"# 32768x16384 velocity errors vs {ref_name} done", file=sys.stderr, flush=True)
+    lat.pop("f64").close()
+    for s in ("f32", "f16s", "bf16s"):
+        e = lat.pop(s)
+        shape, _ = e.autotune()
+        info = e.info()
+        t = time_steps(e, steps, warmup, torch)
+        kms = graph_step_ms(e, torch, replays=2, graph_steps=20)
+        b = algorithmic_bytes(info, 2)
+        out[s] = {"GLUPS": info.cells * steps / t / 1e9, "ms_per_step": 1e3 * t / steps, "kernel_ms": kms,
+                  "bytes_per_step": b, "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "launch": list(shape), "fluid_cells": info.fluid_cells, "nan": e.nan_flag(0)}
+        e.close()
+        print(f"# 32768x16384 {s}: {out[s]['GLUPS']:.1f} GLUPS", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(seconds=12.0):
     """Child-process timing of the reference (or the oracle) on the cylinder workload."""
     code = f"""
@@ -290,7 +407,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="", help="run one extra form only and print it (profiling): "
                                                 "f32_6400x2400_les, f16s_6400x2400_les, bf16s_6400x2400_les, "
-                                                "batched_256")
+                                                "batched_256, large_32768x16384")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -309,6 +426,8 @@ def main():
         with torch.cuda.stream(stream):
             if args.form == "batched_256":
                 out = bench_ensemble(200, 20, torch, stream.cuda_stream)
+            elif args.form == "large_32768x16384":
+                out = bench_large(100, 10, torch, stream.cuda_stream)
             else:
                 storage = args.form.split("_")[0]
                 out = bench_form(6400, 2400, 240, 0.644, storage, 100, 10, torch, stream.cuda_stream, les=True,
@@ -356,10 +475,7 @@ def main():
             forms["batched_256"] = bench_ensemble(1000, 100, torch, stream.cuda_stream)
             print("# batched ensemble done", file=sys.stderr, flush=True)
             if args.large:
-                for storage in ("f32", "f16s", "bf16s"):
-                    forms[f"{storage}_32768x16384_les"] = bench_form(32768, 16384, 60, 0.505, storage, 20, 2,
-                                                                     torch, stream.cuda_stream, les=True, cs=0.12)
-                    print(f"# {storage} 32768x16384 done", file=sys.stderr, flush=True)
+                forms["large_32768x16384_les"] = bench_large(100, 10, torch, stream.cuda_stream)
 
     cpu = None if args.no_cpu else cpu_baseline()
     achieved = bytes_launch / (kms * 1e-3) / 1e9

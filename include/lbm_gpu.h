@@ -146,6 +146,11 @@ int lbm_download(lbm_handle h, int32_t lattice, double* f_aos);
  * initialisation loop (src/main.cpp:60-66, src/cavity.cpp:78-84) without a host array,
  * for lattices too large to stage in AoS FP64.  Resets the step count like an upload. */
 int lbm_init_equilibrium(lbm_handle h, int32_t lattice, double rho, double u, double v);
+/* Density and velocity of rows [y0, y1) of the current state, row-major, nx*(y1-y0)
+ * FP64 each (any output may be null): compute_macros (src/lbm_types.hpp:278-288) on
+ * what lbm_download would return, computed on the device -- the fields the frame
+ * writers take (src/lbm.hpp:588-700) at a third of the transfer. */
+int lbm_macros(lbm_handle h, int32_t lattice, int32_t y0, int32_t y1, double* rho, double* u, double* v);
 
 /* Advances every lattice by n steps: one launch per step covering all lattices,
  * replayed from a hipGraph of graph_steps launches when n allows. */

@@ -148,6 +148,7 @@ struct Ensemble {
     int32_t* d_dummy_nan = nullptr;
     double* d_mass = nullptr;
     double* d_stage = nullptr;
+    int64_t stage_bytes = 0;
     int total_slots = 0;
     int R = kHistoryRows;
     int bx = 256, rows = 1, vec = 1, options = 0, graph_steps = 100;
@@ -654,6 +655,43 @@ int materialize(Ensemble* E, int l, double* host_out, double* mass_out, bool rec
     return LBM_OK;
 }
 
+// Density and velocity of rows [y0, y1) of the current state, each as nx*(y1-y0) FP64.
+int fields(Ensemble* E, int l, int y0, int y1, double* rho, double* u, double* v)
+{
+    HostLattice& H = E->lat[l];
+    const lbm_desc& d = H.d;
+    int rc;
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    const int64_t row_bytes = static_cast<int64_t>(d.nx) * 3 * 8;
+    const int chunk = static_cast<int>(std::min<int64_t>(y1 - y0, E->stage_bytes / row_bytes));
+    if (chunk < 1) return fail(LBM_ERR_INVALID, "staging buffer smaller than one row of fields");
+    const void* src = E->buf[1 - H.par];
+    for (int c0 = y0; c0 < y1; c0 += chunk) {
+        const int c1 = std::min(y1, c0 + chunk);
+        StateArgs s{};
+        s.lat = E->d_lat;
+        s.lattice = l;
+        s.y0 = c0;
+        s.y1 = c1;
+        s.from_state = H.steps == 0 ? 1 : 0;
+        s.src = src;
+        s.flags = E->d_flags;
+        s.link_slot = E->d_lslot;
+        s.links = E->d_links;
+        s.macros = E->d_stage;
+        s.wdist = E->d_wdist;
+        s.scale = E->scale;
+        s.inv_scale = E->inv_scale;
+        LBM_HIP(launch_state(E->storage, s, d.nx, E->stream));
+        const int64_t n = static_cast<int64_t>(c1 - c0) * d.nx;
+        const int64_t off = static_cast<int64_t>(c0 - y0) * d.nx;
+        double* dst[3] = {rho, u, v};
+        for (int k = 0; k < 3; ++k)
+            if (dst[k] && (rc = d2h(E, dst[k] + off, E->d_stage + k * n, n * 8)) != LBM_OK) return rc;
+    }
+    return LBM_OK;
+}
+
 }  // namespace
 }  // namespace lbm
 
@@ -776,6 +814,7 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
         stage = std::max<int64_t>(stage, chunk * row);
     }
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_stage, static_cast<size_t>(stage / 8 + 1));
+    E->stage_bytes = rc == LBM_OK ? stage : 0;
     if (rc == LBM_OK) {
         std::vector<int32_t> z(n, 0);
         std::vector<int64_t> z64(n, 0);
@@ -1001,6 +1040,15 @@ int lbm_init_equilibrium(lbm_handle h, int32_t l, double rho, double u, double v
     if (rc) return rc;
     if (!(rho > 0.0)) return fail(LBM_ERR_INVALID, "density must be positive");
     return lbm::load_state(h, l, nullptr, rho, u, v);
+}
+
+int lbm_macros(lbm_handle h, int32_t l, int32_t y0, int32_t y1, double* rho, double* u, double* v)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!h->lat[l].uploaded) return fail(LBM_ERR_INVALID, "no state uploaded");
+    if (y0 < 0 || y1 > h->lat[l].d.ny || y0 >= y1) return fail(LBM_ERR_INVALID, "row range out of bounds");
+    return lbm::fields(h, l, y0, y1, rho, u, v);
 }
 
 int lbm_download(lbm_handle h, int32_t l, double* f)

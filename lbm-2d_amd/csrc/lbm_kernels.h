@@ -58,6 +58,7 @@ struct StateArgs {
     const int32_t* link_slot;
     const LinkRec* links;
     double* out;                // AoS FP64, (y1-y0)*nx*9; null: not written
+    double* macros;             // planes rho, u, v, each (y1-y0)*nx FP64; null: not written
     double* mass_dev;           // null: no mass sum
     void* coll_dst;             // non-null: write the state's collision here (re-collide)
     const uint16_t* wdist;

@@ -691,6 +691,23 @@ __global__ __launch_bounds__(256) void state_kernel(StateArgs a)
 #pragma unroll
             for (int j = 0; j < 9; ++j) o[j] = WD[j] + static_cast<double>(d[j]);
         }
+        if (a.macros) {
+            // compute_macros (src/lbm_types.hpp:278-288) on the populations w + d, in order
+#pragma clang fp contract(off)
+            double rho = 0.0, mx = 0.0, my = 0.0;
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const double f = WD[j] + static_cast<double>(d[j]);
+                rho += f;
+                mx += f * CX[j];
+                my += f * CY[j];
+            }
+            const int64_t n = static_cast<int64_t>(a.y1 - a.y0) * L.nx;
+            const int64_t o = static_cast<int64_t>(y - a.y0) * L.nx + x;
+            a.macros[o] = rho;
+            a.macros[n + o] = mx / rho;
+            a.macros[2 * n + o] = my / rho;
+        }
         if (!solid) {
 #pragma unroll
             for (int j = 0; j < 9; ++j) mass += static_cast<double>(d[j]);

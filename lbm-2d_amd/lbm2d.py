@@ -99,6 +99,7 @@ def lib() -> C.CDLL:
         "lbm_upload": (C.c_int, [_h, i32, _dp]),
         "lbm_download": (C.c_int, [_h, i32, _dp]),
         "lbm_init_equilibrium": (C.c_int, [_h, i32, dbl, dbl, dbl]),
+        "lbm_macros": (C.c_int, [_h, i32, i32, i32, _dp, _dp, _dp]),
         "lbm_step": (C.c_int, [_h, i32]),
         "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
         "lbm_set_launch": (C.c_int, [_h, i32, i32, i32, i32, i32]),
@@ -211,6 +212,14 @@ class Ensemble:
 
     def init_equilibrium(self, l: int, rho: float = 1.0, u: float = 0.0, v: float = 0.0):
         _check(lib().lbm_init_equilibrium(self._h, l, rho, u, v))
+
+    def macros(self, l: int = 0, y0: int = 0, y1: Optional[int] = None):
+        """(rho, u, v), each (y1-y0, nx) FP64, of rows [y0, y1) of the current state."""
+        ny, nx = self.shape(l)
+        y1 = ny if y1 is None else y1
+        out = [np.empty((max(0, y1 - y0), nx)) for _ in range(3)]
+        _check(lib().lbm_macros(self._h, l, y0, y1, *(o.ctypes.data_as(_dp) for o in out)))
+        return tuple(out)
 
     def download(self, l: int = 0) -> np.ndarray:
         ny, nx = self.shape(l)
@@ -443,6 +452,16 @@ def walls_mask(nx, ny) -> np.ndarray:
     m[:, 0] = 1
     m[:, nx - 1] = 1
     return m
+
+
+def plate_polygon(cx, cy, chord, aoa_deg, thickness=1.0) -> np.ndarray:
+    """Corners of the flat-plate driver's rotated chord x thickness rectangle
+    (src/flat_plate.cpp:90-105), shape (4, 2)."""
+    rad = -aoa_deg * np.pi / 180.0
+    ca, sa = np.cos(rad), np.sin(rad)
+    hx, hy = chord / 2.0, thickness / 2.0
+    return np.array([(px * ca - py * sa + cx, px * sa + py * ca + cy)
+                     for px, py in ((-hx, -hy), (hx, -hy), (hx, hy), (-hx, hy))])
 
 
 def polygon_mask(nx, ny, xy) -> np.ndarray:

@@ -213,3 +213,30 @@ def test_device_equilibrium_init_equals_upload(name, storage):
     np.testing.assert_array_equal(a.download(0), b.download(0))
     np.testing.assert_array_equal(a.forces(0)[0], b.forces(0)[0])
     assert a.steps_done(0) == 7
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "bf16s"])
+def test_macros_equal_compute_macros_of_download(storage):
+    """lbm_macros = compute_macros (src/lbm_types.hpp:278-288) of the downloaded state, bit
+    for bit, on a whole lattice and on a row range; before any step and after steps."""
+    c = CASES["cylinder"]
+    ens, o = build_gpu(c, storage)
+    for steps in (0, 5):
+        ens.step(steps)
+        f = ens.download(0)
+        rho = np.zeros(f.shape[:2])
+        mx = np.zeros(f.shape[:2])
+        my = np.zeros(f.shape[:2])
+        for i in range(9):
+            rho = rho + f[..., i]
+            mx = mx + f[..., i] * G.CXV[i]
+            my = my + f[..., i] * G.CYV[i]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            u, v = mx / rho, my / rho
+        r2, u2, v2 = ens.macros(0)
+        np.testing.assert_array_equal(r2, rho)
+        np.testing.assert_array_equal(u2, u)
+        np.testing.assert_array_equal(v2, v)
+        r3, u3, v3 = ens.macros(0, 7, 19)
+        np.testing.assert_array_equal(u3, u[7:19])
+        np.testing.assert_array_equal(r3, rho[7:19])
