@@ -146,6 +146,12 @@ int lbm_download(lbm_handle h, int32_t lattice, double* f_aos);
  * initialisation loop (src/main.cpp:60-66, src/cavity.cpp:78-84) without a host array,
  * for lattices too large to stage in AoS FP64.  Resets the step count like an upload. */
 int lbm_init_equilibrium(lbm_handle h, int32_t lattice, double rho, double u, double v);
+/* 16-bit storage forms only: store f_i - feq_i(1, u, v) instead of f_i - w_i (the
+ * default, u = v = 0).  With (u, v) the free-stream velocity the stored deviations of
+ * the undisturbed flow are near zero, so binary16 / bfloat16 rounding stays far below
+ * the flow's own variations there.  No counterpart in the reference, whose state is
+ * FP64.  Invalidates the stored state: upload or initialise afterwards. */
+int lbm_set_storage_shift(lbm_handle h, int32_t lattice, double u, double v);
 /* Density and velocity of rows [y0, y1) of the current state, row-major, nx*(y1-y0)
  * FP64 each (any output may be null): compute_macros (src/lbm_types.hpp:278-288) on
  * what lbm_download would return, computed on the device -- the fields the frame

@@ -101,6 +101,7 @@ struct HostLattice {
     bool frad = false;
     double fcx = 0, fcy = 0, fr = 0;
     std::vector<lbm_link> explicit_links;
+    double shift_u = 0.0, shift_v = 0.0;  // 16-bit storage shift: feq(1, shift_u, shift_v) - w
     bool geom_dirty = true;
     // built geometry
     std::vector<LinkRec> recs;
@@ -365,7 +366,7 @@ int build_geometry(HostLattice& H)
     return LBM_OK;
 }
 
-LatticeDev make_dev(const HostLattice& H, int index)
+LatticeDev make_dev(const HostLattice& H, int index, float scale)
 {
     LatticeDev L{};
     const lbm_desc& d = H.d;
@@ -401,6 +402,14 @@ LatticeDev make_dev(const HostLattice& H, int index)
     L.inv_nu = L.nu > 0.0 ? 1.0 / L.nu : 0.0;
     // Fscale (src/lbm.hpp:296-297)
     L.fscale = d.body_force_x != 0.0 ? (1.0 - 1.0 / (2.0 * d.tau)) * 3.0 * d.body_force_x : 0.0;
+    // storage shift feq_i(1, u_s, v_s) - w_i (compute_equilibrium, src/lbm_types.hpp:269-273)
+    for (int i = 0; i < 9; ++i) {
+        const double w = i == 0 ? 4.0 / 9.0 : (i < 5 ? 1.0 / 9.0 : 1.0 / 36.0);
+        const double eu = CX[i] * H.shift_u + CY[i] * H.shift_v;
+        const double uu = H.shift_u * H.shift_u + H.shift_v * H.shift_v;
+        L.shift[i] = static_cast<float>(w * (3.0 * eu + 4.5 * eu * eu - 1.5 * uu));
+        L.shift_s[i] = L.shift[i] * scale;  // exact: scale is a power of two
+    }
     return L;
 }
 
@@ -460,7 +469,7 @@ int commit(Ensemble* E)
     }
     if (E->desc_dirty) {
         std::vector<LatticeDev> devs;
-        for (size_t l = 0; l < E->lat.size(); ++l) devs.push_back(make_dev(E->lat[l], static_cast<int>(l)));
+        for (size_t l = 0; l < E->lat.size(); ++l) devs.push_back(make_dev(E->lat[l], static_cast<int>(l), E->scale));
         if ((rc = h2d(E, E->d_lat, devs.data(), devs.size() * sizeof(LatticeDev))) != LBM_OK) return rc;
         E->desc_dirty = false;
     }
@@ -864,6 +873,21 @@ int lbm_set_obstacle(lbm_handle h, int32_t l, const uint8_t* mask)
     H.geom_dirty = true;
     H.uploaded = false;  // the stored state belongs to the old geometry
     h->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_set_storage_shift(lbm_handle h, int32_t l, double u, double v)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (h->storage != lbm::ST_F16S && h->storage != lbm::ST_BF16S)
+        return fail(LBM_ERR_INVALID, "the storage shift applies to the 16-bit storage forms");
+    if (!(std::fabs(u) < 0.5 && std::fabs(v) < 0.5)) return fail(LBM_ERR_INVALID, "shift velocity out of range");
+    auto& H = h->lat[l];
+    H.shift_u = u;
+    H.shift_v = v;
+    h->desc_dirty = true;
+    H.uploaded = false;  // the stored state is encoded against the old shift
     return LBM_OK;
 }
 

@@ -35,6 +35,9 @@ struct alignas(16) LatticeDev {
     double s_shear;                    // MRT shear rate; s_bulk = 1.2, s_normal = 1 are fixed
                                        // (MRTParams::from_tau, src/lbm_types.hpp:72-76)
     double tau, inv_tau, u_in, cs, cs2, nu, inv_nu, fscale;
+    // 16-bit storage shift: stored h_i = 2^k (f_i - w_i - shift[i]); shift_s = 2^k shift
+    float shift[9], shift_s[9];
+    float pad1[2];
 };
 
 // Boundary-link record for one fluid node with links (indexed by link_slot[cell]).

@@ -38,42 +38,48 @@ enum { BB_SIMPLE = 0, BB_MEI = 1, BB_BOUZIDI = 2 };
 // ------------------------------------------------------------------ storage forms
 //
 // dec/enc convert between a stored value and the deviation f - w; dec_raw/enc_raw
-// keep the stored scale (h = 2^k (f - w)), for arithmetic done in scaled units.
+// keep the stored scale (h = 2^k (f - w - s_i)), for arithmetic done in scaled units.
+// The 16-bit forms store the deviation less a per-direction constant s_i (the lattice's
+// storage shift, zero unless set), passed as `sh`; the FP64 and FP32 forms ignore it.
 
 template <int ST> struct Form;
 template <> struct Form<ST_F64> {
     using store = double;
     using real = double;
-    static __device__ __forceinline__ real dec(store v, float) { return v; }
-    static __device__ __forceinline__ store enc(real d, float) { return d; }
+    static constexpr bool shifted = false;
+    static __device__ __forceinline__ real dec(store v, float, float) { return v; }
+    static __device__ __forceinline__ store enc(real d, float, float) { return d; }
     static __device__ __forceinline__ real dec_raw(store v) { return v; }
     static __device__ __forceinline__ store enc_raw(real d) { return d; }
 };
 template <> struct Form<ST_F32> {
     using store = float;
     using real = float;
-    static __device__ __forceinline__ real dec(store v, float) { return v; }
-    static __device__ __forceinline__ store enc(real d, float) { return d; }
+    static constexpr bool shifted = false;
+    static __device__ __forceinline__ real dec(store v, float, float) { return v; }
+    static __device__ __forceinline__ store enc(real d, float, float) { return d; }
     static __device__ __forceinline__ real dec_raw(store v) { return v; }
     static __device__ __forceinline__ store enc_raw(real d) { return d; }
 };
 template <> struct Form<ST_F16S> {
     using store = uint16_t;
     using real = float;
+    static constexpr bool shifted = true;
     static __device__ __forceinline__ real dec_raw(store v) { return static_cast<float>(__builtin_bit_cast(_Float16, v)); }
     // Conversion to binary16 rounds to nearest even (v_cvt_f16_f32).
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(h)); }
-    static __device__ __forceinline__ real dec(store v, float inv_scale) { return dec_raw(v) * inv_scale; }
-    static __device__ __forceinline__ store enc(real d, float scale) { return enc_raw(d * scale); }
+    static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
+    static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
 };
 template <> struct Form<ST_BF16S> {
     using store = uint16_t;
     using real = float;
+    static constexpr bool shifted = true;
     static __device__ __forceinline__ real dec_raw(store v) { return __uint_as_float(static_cast<uint32_t>(v) << 16); }
     // Round to nearest even (v_cvt_pk_bf16_f32 on gfx950); NaN stays NaN.
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(h)); }
-    static __device__ __forceinline__ real dec(store v, float inv_scale) { return dec_raw(v) * inv_scale; }
-    static __device__ __forceinline__ store enc(real d, float scale) { return enc_raw(d * scale); }
+    static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
+    static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
 };
 
 template <typename R> __device__ __forceinline__ R wgt(int i) { return static_cast<R>(WD[i]); }
@@ -224,7 +230,7 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
     }
     R own[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) own[i] = Form<ST>::dec(c.base[i * L.plane + idx], c.inv_scale);
+    for (int i = 0; i < 9; ++i) own[i] = Form<ST>::dec(c.base[i * L.plane + idx], c.inv_scale, L.shift[i]);
 
     const LinkRec* rec = nullptr;
     uint32_t mask = 0;
@@ -262,7 +268,7 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
                     const int sx = px - CX[i], sy = py - CY[i];
                     if (sx >= 0 && sx < L.nx && sy >= 0 && sy < L.ny) {
                         const R up = Form<ST>::dec(
-                            c.base[i * L.plane + static_cast<int64_t>(sy) * L.pitch + sx], c.inv_scale);
+                            c.base[i * L.plane + static_cast<int64_t>(sy) * L.pitch + sx], c.inv_scale, L.shift[i]);
                         // the reference reads f (not f - w): w_i cancels since weights sum to 1
                         val = R(2) * q * own[i] + (R(1) - R(2) * q) * up;
                     } else {
@@ -297,7 +303,7 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
             d[j] = -wgt<R>(j);  // dropped: f_next keeps its zero (src/lbm.hpp:470)
         } else {
             d[j] = Form<ST>::dec(c.base[j * L.plane + static_cast<int64_t>(sy) * L.pitch + sx],
-                                 c.inv_scale);
+                                 c.inv_scale, L.shift[j]);
         }
     }
 }
@@ -439,7 +445,8 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     if (!(fl & CELL_SPECIAL)) {
         const S* b = c.base + idx;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(b[j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale);
+        for (int j = 0; j < 9; ++j)
+            d[j] = Form<ST>::dec(b[j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale, L.shift[j]);
     } else {
         if (solid && !(fl & CELL_WRITE)) return;
         state_general<ST>(c, x, y, solid, d);
@@ -465,7 +472,8 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     }
     S* o = dst + L.f_off + idx;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale), a.options & LAUNCH_STREAM_STORES);
+    for (int j = 0; j < 9; ++j)
+        store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale, L.shift[j]), a.options & LAUNCH_STREAM_STORES);
 }
 
 // Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane i-1 /
@@ -549,13 +557,17 @@ __device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs
             else if (CX[j] == 1) s = k == 0 ? edge[j] : in[j].v[k - 1];
             else s = k == V - 1 ? edge[j] : in[j].v[k + 1];
             d[j] = Form<ST>::dec_raw(s);
+            if constexpr (Form<ST>::shifted) d[j] += L.shift_s[j];
         }
         R wd = R(0);
         if (L.les) wd = wd4.v[k] == 0xffffu ? R(1e9) : static_cast<R>(wd4.v[k]);
         collide<R>(d, L, wd, static_cast<R>(a.scale));  // in stored units
         if (!(d[0] == d[0])) bad = true;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) out[j].v[k] = Form<ST>::enc_raw(d[j]);
+        for (int j = 0; j < 9; ++j) {
+            if constexpr (Form<ST>::shifted) d[j] -= L.shift_s[j];
+            out[j].v[k] = Form<ST>::enc_raw(d[j]);
+        }
     }
 #pragma unroll
     for (int j = 0; j < 9; ++j)
@@ -677,11 +689,11 @@ __global__ __launch_bounds__(256) void state_kernel(StateArgs a)
         const bool solid = (fl & CELL_SOLID) != 0;
         if (a.from_state) {
 #pragma unroll
-            for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(base[j * L.plane + idx], a.inv_scale);
+            for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(base[j * L.plane + idx], a.inv_scale, L.shift[j]);
         } else if (!(fl & CELL_SPECIAL)) {
 #pragma unroll
             for (int j = 0; j < 9; ++j)
-                d[j] = Form<ST>::dec(base[idx + j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale);
+                d[j] = Form<ST>::dec(base[idx + j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale, L.shift[j]);
         } else {
             const Ctx<ST> c{L, base, a.flags, a.link_slot, a.links, a.inv_scale};
             state_general<ST>(c, x, y, solid, d);
@@ -724,7 +736,7 @@ __global__ __launch_bounds__(256) void state_kernel(StateArgs a)
             }
             S* co = static_cast<S*>(a.coll_dst) + L.f_off + idx;
 #pragma unroll
-            for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale);
+            for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
         }
     }
     if (a.mass_dev) {
@@ -762,7 +774,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
     S* st = static_cast<S*>(a.state_dst) + L.f_off + idx;
     S* co = static_cast<S*>(a.coll_dst) + L.f_off + idx;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) st[j * L.plane] = Form<ST>::enc(d[j], a.scale);
+    for (int j = 0; j < 9; ++j) st[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
     if (!(a.flags[L.cell_off + idx] & CELL_SOLID)) {
         R wd = R(0);
         if (L.les) {
@@ -772,7 +784,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
         collide<R>(d, L, wd, R(1));
     }
 #pragma unroll
-    for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale);
+    for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
 }
 
 // Advances the per-lattice step counters after a batch of launches; an odd batch

@@ -34,6 +34,7 @@ struct Config {
     int storage = LBM_STORE_F64;  // enum lbm_storage
     int scale_exp = 8;            // 16-bit forms
     int graph_steps = 100;        // run_steps: hipGraph length
+    double shift_u = 0.0;         // 16-bit forms: store f - feq(1, shift_u, 0), e.g. the inflow
 };
 
 inline Config& config()
@@ -52,7 +53,7 @@ struct Context {
     lbm_handle h = nullptr;
     // what the handle was built for
     int nx = 0, ny = 0, case_type = -1, collision = -1, les = -1, storage = -1;
-    double cs = 0.0, body_force = 0.0;
+    double cs = 0.0, body_force = 0.0, shift_u = 0.0;
     // geometry last given to the handle
     std::vector<uint8_t> obstacle;
     BounceBackGeometry geom;
@@ -115,7 +116,7 @@ inline void prepare(LBMCapabilities& sys, double tau, double u_inflow)
     const int collision = g_collision == CollisionType::MRT ? LBM_MRT : LBM_BGK;
     const int les = g_use_les ? 1 : 0;
     if (!C.h || C.nx != NX || C.ny != NY || C.case_type != case_type || C.collision != collision || C.les != les ||
-        C.cs != g_cs || C.body_force != sys.body_force_x || C.storage != cfg.storage) {
+        C.cs != g_cs || C.body_force != sys.body_force_x || C.storage != cfg.storage || C.shift_u != cfg.shift_u) {
         if (C.h) {
             lbm_destroy(C.h);
             C.h = nullptr;
@@ -129,6 +130,8 @@ inline void prepare(LBMCapabilities& sys, double tau, double u_inflow)
         d.tau = tau;
         d.u_in = u_inflow;
         check(lbm_create(&d, 1, cfg.storage, cfg.scale_exp, nullptr, &C.h), "lbm_create");
+        if (cfg.shift_u != 0.0) check(lbm_set_storage_shift(C.h, 0, cfg.shift_u, 0.0), "lbm_set_storage_shift");
+        C.shift_u = cfg.shift_u;
         C.nx = NX;
         C.ny = NY;
         C.case_type = case_type;

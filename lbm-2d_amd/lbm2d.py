@@ -99,6 +99,7 @@ def lib() -> C.CDLL:
         "lbm_upload": (C.c_int, [_h, i32, _dp]),
         "lbm_download": (C.c_int, [_h, i32, _dp]),
         "lbm_init_equilibrium": (C.c_int, [_h, i32, dbl, dbl, dbl]),
+        "lbm_set_storage_shift": (C.c_int, [_h, i32, dbl, dbl]),
         "lbm_macros": (C.c_int, [_h, i32, i32, i32, _dp, _dp, _dp]),
         "lbm_step": (C.c_int, [_h, i32]),
         "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
@@ -209,6 +210,10 @@ class Ensemble:
         ny, nx = self.shape(l)
         a = np.ascontiguousarray(f, dtype=np.float64).reshape(ny, nx, 9)
         _check(lib().lbm_upload(self._h, l, a.ctypes.data_as(_dp)))
+
+    def set_storage_shift(self, l: int, u: float, v: float = 0.0):
+        """16-bit forms: store f - feq(1, u, v) rather than f - w (call before upload)."""
+        _check(lib().lbm_set_storage_shift(self._h, l, u, v))
 
     def init_equilibrium(self, l: int, rho: float = 1.0, u: float = 0.0, v: float = 0.0):
         _check(lib().lbm_init_equilibrium(self._h, l, rho, u, v))
@@ -423,18 +428,28 @@ def strouhal_welch(cl, length: float, u: float):
     return float(f[band][np.argmax(psd[band])] * length / u)
 
 
-def strouhal_crossings(cl, length: float, u: float, tail: float = 0.5):
+def strouhal_crossings(cl, length: float, u: float, tail: float = 0.5, hysteresis: float = 0.1):
     """Strouhal number from the mean period between upward crossings of Cl's mean over the
-    last `tail` of the history, crossings placed by linear interpolation.  Resolves St
-    far below the Welch bin width 1/nperseg when the history holds few periods."""
+    last `tail` of the history.  A crossing counts once the signal has been below
+    -hysteresis * rms since the previous one, so step-to-step noise around the mean adds
+    no crossings; each is placed by linear interpolation.  Resolves St far below the
+    Welch bin width 1/nperseg when the history holds few periods."""
     c = np.asarray(cl, float)
     c = c[int(len(c) * (1.0 - tail)):]
     c = c - c.mean()
-    i = np.nonzero((c[:-1] < 0.0) & (c[1:] >= 0.0))[0]
-    if len(i) < 2:
+    h = hysteresis * float(np.sqrt(np.mean(c * c)))
+    below = c < -h
+    up = np.nonzero((c[:-1] < 0.0) & (c[1:] >= 0.0))[0]
+    # keep an upward crossing only if the signal went below -h since the last kept one
+    last_low = np.maximum.accumulate(np.where(below, np.arange(len(c)), -1))
+    times, prev = [], -1
+    for i in up:
+        if last_low[i] > prev:
+            times.append(i + (-c[i]) / (c[i + 1] - c[i]))
+            prev = i
+    if len(times) < 2:
         return None
-    t = i + (-c[i]) / (c[i + 1] - c[i])
-    return float(length / ((t[-1] - t[0]) / (len(t) - 1)) / u)
+    return float(length / ((times[-1] - times[0]) / (len(times) - 1)) / u)
 
 
 def circle_mask(nx, ny, cx, cy, r) -> np.ndarray:

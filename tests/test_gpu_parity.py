@@ -240,3 +240,30 @@ def test_macros_equal_compute_macros_of_download(storage):
         r3, u3, v3 = ens.macros(0, 7, 19)
         np.testing.assert_array_equal(u3, u[7:19])
         np.testing.assert_array_equal(r3, rho[7:19])
+
+
+def test_storage_shift_round_trip_and_errors():
+    """The storage shift changes the encoding, not the state: download after upload is the
+    upload to 16-bit rounding, steps stay within the 16-bit bound of the oracle, and the
+    FP32 / FP64 forms refuse it."""
+    c = CASES["cylinder"]
+    o = build_oracle(c)
+    from cases import apply_gpu_geometry, gpu_desc
+    for storage, sexp in (("f16s", 8), ("f16s", 14), ("bf16s", 8)):
+        e = G.Ensemble([gpu_desc(c)], storage=storage, scale_exp=sexp)
+        apply_gpu_geometry(e, 0, c, o.obstacle.copy())
+        e.set_storage_shift(0, c.u, 0.0)
+        e.upload(0, o.f.copy())
+        err0 = np.abs(e.download(0) - o.f).max()
+        assert err0 <= (1e-5 if storage == "f16s" else 2e-4), (storage, sexp, err0)
+    with pytest.raises(G.LbmError):
+        G.Ensemble([gpu_desc(c)], storage="f32").set_storage_shift(0, 0.1, 0.0)
+    # stepping: same bound as the unshifted form against the oracle
+    ens = G.Ensemble([gpu_desc(c)], storage="f16s", scale_exp=14)
+    apply_gpu_geometry(ens, 0, c, o.obstacle.copy())
+    ens.set_storage_shift(0, c.u, 0.0)
+    ens.upload(0, o.f.copy())
+    for _ in range(c.steps):
+        o.step(c.tau, c.u)
+    ens.step(c.steps)
+    assert np.abs(ens.download(0) - o.f).max() <= TOL["f16s"]

@@ -72,3 +72,12 @@ def test_coefficients_and_late_stats():
     assert m == pytest.approx(1.0)
     # latter half: cl[5:] = [0.5,-0.25,0.1,0.2,-0.1] * 2/0.6 * 0.01 -> (max - min) / 2
     assert a == pytest.approx((0.5 + 0.25) / 2 * 0.01 * 2 / 0.6)
+
+
+def test_strouhal_crossings_ignore_noise_at_the_mean():
+    # a tone with step-to-step noise: plain mean crossings would multiply near zero
+    L, U, st = 60.0, 0.1, 0.2
+    rng = np.random.default_rng(0)
+    t = np.arange(80000)
+    cl = 0.3 * np.sin(2 * np.pi * st * U / L * t) + 2e-3 * rng.standard_normal(len(t))
+    assert abs(G.strouhal_crossings(cl, L, U) - st) / st < 2e-3
