@@ -44,6 +44,7 @@ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 
 constexpr int kHistoryRows = 4096;
+constexpr int kMinBatch = 32;  // accumulator blocks when graphs are shorter or off
 constexpr int64_t kStagingBytes = 256ll << 20;
 
 // q along link i from fluid node (xf,yf) to a circle: first root t in (0,1] of the
@@ -109,6 +110,8 @@ struct HostLattice {
     std::vector<int32_t> lslot;      // pitch layout
     std::vector<uint16_t> wdist;     // pitch layout (LES only)
     std::vector<double> wall_dist;   // nx*ny (LES only)
+    std::vector<int32_t> special;    // y*pitch+x of the general-path nodes
+    int64_t special_off = 0;
     int64_t fluid = 0;
     bool uses_bouzidi = false;
     // run state
@@ -142,7 +145,12 @@ struct Ensemble {
     int n_tiles = 0;
     int64_t* d_steps = nullptr;
     int32_t* d_par = nullptr;
-    double* d_acc = nullptr;
+    double* d_acc = nullptr;       // [acc_blocks][kAccCopies][total_slots][2]
+    int acc_blocks = 0;            // launches a batch may hold
+    int32_t* d_slot_lat = nullptr; // lattice of each force slot
+    int32_t* d_special = nullptr;  // every lattice's special-node list
+    size_t special_cap = 0;
+    std::vector<LatticeDev> host_lat;
     double* d_hist = nullptr;
     double* d_dummy_acc = nullptr;
     int32_t* d_nan = nullptr;
@@ -154,7 +162,7 @@ struct Ensemble {
     int R = kHistoryRows;
     int bx = 256, rows = 1, vec = 1, options = 0, graph_steps = 100;
     bool geom_dirty = true, desc_dirty = true, tiles_dirty = true, counters_stale = false;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // by the starting buffer parity (one lattice)
     bool graph_dirty = true;
     int64_t device_bytes = 0;
 };
@@ -284,6 +292,7 @@ int build_geometry(HostLattice& H)
                         r.q[i] = 0.5;
                         r.uwx[i] = r.uwy[i] = 0.0;
                     }
+                    r.cell = static_cast<int32_t>(p);
                     rec_of[c] = static_cast<int32_t>(H.recs.size());
                     H.lslot[p] = rec_of[c];
                     H.recs.push_back(r);
@@ -340,6 +349,9 @@ int build_geometry(HostLattice& H)
         r.slot[k.dir] = static_cast<int16_t>(k.slot);
         r.scheme = (r.scheme & ~(3u << (2 * k.dir))) | (static_cast<uint32_t>(k.scheme) << (2 * k.dir));
     }
+    for (LinkRec& r : H.recs)
+        for (int i = 1; i < 9; ++i)
+            if ((r.mask & (1u << i)) && (r.uwx[i] != 0.0 || r.uwy[i] != 0.0)) r.scheme |= LINK_MOVING;
     H.uses_bouzidi = false;
     for (const LinkRec& r : H.recs)
         for (int i = 1; i < 9; ++i)
@@ -349,6 +361,15 @@ int build_geometry(HostLattice& H)
         for (size_t p = 0; p < H.flags.size(); ++p)
             if (H.flags[p] & CELL_SOLID) H.flags[p] |= CELL_WRITE;
     }
+    // written solids off the edges, for the one-lattice kernel's list blocks (its link
+    // and edge blocks cover the other general-path nodes)
+    H.special.clear();
+    for (int y = 1; y + 1 < ny; ++y)
+        for (int x = 1; x + 1 < nx; ++x) {
+            const uint8_t f = H.flags[static_cast<int64_t>(y) * H.pitch + x];
+            if ((f & CELL_SOLID) && (f & CELL_WRITE))
+                H.special.push_back(static_cast<int32_t>(static_cast<int64_t>(y) * H.pitch + x));
+        }
     if (d.les) {
         wall_distance(H.obstacle, nx, ny, H.wall_dist);
         H.wdist.assign(static_cast<size_t>(ny) * H.pitch, 0xffff);
@@ -450,28 +471,56 @@ int commit(Ensemble* E)
                 if (!H.wdist.empty()) std::copy(H.wdist.begin(), H.wdist.end(), wd.begin() + H.cell_off);
             }
             if ((rc = h2d(E, E->d_links, all.data(), all.size() * sizeof(LinkRec))) != LBM_OK) return rc;
+            std::vector<int32_t> special;
+            for (auto& H : E->lat) {
+                H.special_off = static_cast<int64_t>(special.size());
+                special.insert(special.end(), H.special.begin(), H.special.end());
+            }
+            if (special.size() > E->special_cap || !E->d_special) {
+                if ((rc = dev_alloc(E, &E->d_special, special.size())) != LBM_OK) return rc;
+                E->special_cap = special.size();
+            }
+            if ((rc = h2d(E, E->d_special, special.data(), special.size() * 4)) != LBM_OK) return rc;
+            E->graph_dirty = true;
             if ((rc = h2d(E, E->d_flags, flags.data(), flags.size())) != LBM_OK) return rc;
             if ((rc = h2d(E, E->d_lslot, lslot.data(), lslot.size() * 4)) != LBM_OK) return rc;
             if ((rc = h2d(E, E->d_wdist, wd.data(), wd.size() * 2)) != LBM_OK) return rc;
             if (slots != E->total_slots) {
                 E->total_slots = slots;
-                const size_t acc_n = static_cast<size_t>(slots) * 2 * kAccCopies * 2;
-                if ((rc = dev_alloc(E, &E->d_acc, acc_n)) != LBM_OK) return rc;
-                if ((rc = dev_alloc(E, &E->d_dummy_acc, acc_n)) != LBM_OK) return rc;
+                const size_t dummy_n = static_cast<size_t>(slots) * kAccCopies * 2;
+                if ((rc = dev_alloc(E, &E->d_dummy_acc, dummy_n)) != LBM_OK) return rc;
                 if ((rc = dev_alloc(E, &E->d_hist, static_cast<size_t>(slots) * E->R * 2)) != LBM_OK) return rc;
-                LBM_HIP(hipMemsetAsync(E->d_acc, 0, acc_n * 8, E->stream));
+                if ((rc = dev_alloc(E, &E->d_slot_lat, slots)) != LBM_OK) return rc;
                 LBM_HIP(hipMemsetAsync(E->d_hist, 0, static_cast<size_t>(slots) * E->R * 16, E->stream));
+                E->acc_blocks = 0;  // reallocated below
                 E->graph_dirty = true;
             }
+            std::vector<int32_t> slot_lat;
+            for (size_t l = 0; l < E->lat.size(); ++l)
+                for (int k = 0; k < E->lat[l].n_slots; ++k) slot_lat.push_back(static_cast<int32_t>(l));
+            if ((rc = h2d(E, E->d_slot_lat, slot_lat.data(), slot_lat.size() * 4)) != LBM_OK) return rc;
         }
         E->geom_dirty = false;
         E->desc_dirty = true;
     }
+    // accumulator blocks: one per launch of the longest batch (a graph), all zero
+    // between batches (flush_kernel clears what it reads)
+    const int want_blocks = std::max(E->graph_steps, kMinBatch);
+    if (E->acc_blocks != want_blocks) {
+        const size_t acc_n = static_cast<size_t>(want_blocks) * kAccCopies * E->total_slots * 2;
+        if ((rc = dev_alloc(E, &E->d_acc, acc_n)) != LBM_OK) return rc;
+        LBM_HIP(hipMemsetAsync(E->d_acc, 0, acc_n * 8, E->stream));
+        E->acc_blocks = want_blocks;
+        E->graph_dirty = true;
+    }
     if (E->desc_dirty) {
-        std::vector<LatticeDev> devs;
-        for (size_t l = 0; l < E->lat.size(); ++l) devs.push_back(make_dev(E->lat[l], static_cast<int>(l), E->scale));
-        if ((rc = h2d(E, E->d_lat, devs.data(), devs.size() * sizeof(LatticeDev))) != LBM_OK) return rc;
+        E->host_lat.clear();
+        for (size_t l = 0; l < E->lat.size(); ++l)
+            E->host_lat.push_back(make_dev(E->lat[l], static_cast<int>(l), E->scale));
+        if ((rc = h2d(E, E->d_lat, E->host_lat.data(), E->host_lat.size() * sizeof(LatticeDev))) != LBM_OK)
+            return rc;
         E->desc_dirty = false;
+        E->graph_dirty = true;  // one-lattice graphs hold the descriptor in their arguments
     }
     if (E->tiles_dirty && E->lat.size() > 1) {
         std::vector<Tile> tiles;
@@ -514,11 +563,8 @@ StepArgs base_args(Ensemble* E)
     a.link_slot = E->d_lslot;
     a.links = E->d_links;
     a.wdist = E->d_wdist;
-    a.steps = E->d_steps;
     a.par = E->d_par;
     a.acc = E->d_acc;
-    a.hist = E->d_hist;
-    a.R = E->R;
     a.total_slots = E->total_slots;
     a.nan_flags = E->d_nan;
     a.scale = E->scale;
@@ -527,82 +573,130 @@ StepArgs base_args(Ensemble* E)
     return a;
 }
 
-// Grid for one launch over every lattice (ensemble) or over one lattice.
-void grid_for(Ensemble* E, int single, StepArgs& a, int& blocks)
+// Nodes on the lattice's edges: rows 0 and ny-1, columns 0 and nx-1 between them.
+int edge_nodes(int nx, int ny)
+{
+    return (ny == 1 ? nx : 2 * nx) + (nx == 1 ? 1 : 2) * std::max(ny - 2, 0);
+}
+
+// Grid for one launch over one lattice (single >= 0, or an ensemble of one) or over
+// every lattice; returns whether the one-lattice kernel applies.
+bool grid_for(Ensemble* E, int single, StepArgs& a, int& blocks)
 {
     if (single >= 0 || E->lat.size() == 1) {
         const int l = single >= 0 ? single : 0;
         const lbm_desc& d = E->lat[l].d;
+        const HostLattice& H = E->lat[l];
+        a.L0 = E->host_lat[l];
+        a.lattice0 = l;
         a.tiles = nullptr;
-        a.single = l;
         a.tiles_x = (d.nx + E->bx * E->vec - 1) / (E->bx * E->vec);
         a.tiles_y = (d.ny + E->rows - 1) / E->rows;
-        blocks = a.tiles_x * a.tiles_y;
-    } else {
-        a.tiles = E->d_tiles;
-        a.n_tiles = E->n_tiles;
-        blocks = E->n_tiles;
+        a.n_links = static_cast<int32_t>(H.recs.size());
+        a.n_edges = edge_nodes(d.nx, d.ny);
+        a.special = E->d_special + H.special_off;
+        a.n_special = static_cast<int32_t>(H.special.size());
+        a.special_blocks = list_blocks(E->bx, a.n_links + a.n_edges) + (a.n_special + E->bx - 1) / E->bx;
+        blocks = a.special_blocks + a.tiles_x * a.tiles_y;
+        return true;
     }
+    a.tiles = E->d_tiles;
+    a.n_tiles = E->n_tiles;
+    blocks = E->n_tiles;
+    return false;
 }
 
-// Copies the flushed history rows [read_pos, steps-1) of lattice l to the host.
+// Copies the flushed history rows [read_pos, steps) of lattice l to the host.
 int drain(Ensemble* E, int l)
 {
     HostLattice& H = E->lat[l];
-    const int64_t last = H.steps - 1;  // newest step still in the accumulator
-    if (last <= H.read_pos) return LBM_OK;
+    if (H.steps <= H.read_pos) return LBM_OK;
     LBM_HIP(hipStreamSynchronize(E->stream));
     std::vector<double> ring(static_cast<size_t>(H.n_slots) * E->R * 2);
     int rc = d2h(E, ring.data(), E->d_hist + static_cast<int64_t>(H.slot_base) * E->R * 2, ring.size() * 8);
     if (rc != LBM_OK) return rc;
-    for (int64_t k = H.read_pos; k < last; ++k)
+    for (int64_t k = H.read_pos; k < H.steps; ++k)
         for (int s = 0; s < H.n_slots; ++s) {
             const int64_t o = (static_cast<int64_t>(s) * E->R + k % E->R) * 2;
             H.pend_fx.push_back(ring[o]);
             H.pend_fy.push_back(ring[o + 1]);
         }
-    H.read_pos = last;
+    H.read_pos = H.steps;
     return LBM_OK;
 }
 
+// A batch of n steps writes history rows steps .. steps+n-1 (mod R): drain first when
+// it would overwrite rows not yet read.
 int ensure_history_room(Ensemble* E, int l, int64_t n)
 {
     HostLattice& H = E->lat[l];
-    if (H.steps + n - 1 - H.read_pos > E->R - 2) return drain(E, l);
+    if (H.steps + n - H.read_pos > E->R) return drain(E, l);
     return LBM_OK;
 }
 
-int launch_batch(Ensemble* E, int n)
+// n <= acc_blocks launches over lattice l (l < 0: every lattice), then the force flush
+// and the counter update.  One-lattice launches take their buffers from the host's
+// parity; graph capture records them for that starting parity.
+int launch_batch(Ensemble* E, int n, int l)
 {
     StepArgs a = base_args(E);
     int blocks;
-    grid_for(E, -1, a, blocks);
-    a.flush_begin = 0;
-    a.flush_count = static_cast<int>(E->lat.size());
+    const bool single = grid_for(E, l, a, blocks);
+    const int li = l >= 0 ? l : 0;
+    const int32_t par = E->lat[li].par;
     for (int i = 0; i < n; ++i) {
-        a.step_arg = i;
-        LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream));
+        a.acc = E->d_acc + static_cast<int64_t>(i) * kAccCopies * E->total_slots * 2;
+        if (single) {
+            const int sp = par ^ (i & 1);
+            a.src = E->buf[sp];
+            a.dst = E->buf[1 - sp];
+        } else {
+            a.step_arg = i;
+        }
+        LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream));
     }
-    LBM_HIP(launch_add_steps(E->d_steps, E->d_par, 0, static_cast<int>(E->lat.size()), n, E->stream));
+    FlushArgs f{};
+    f.acc = E->d_acc;
+    f.hist = E->d_hist;
+    f.steps = E->d_steps;
+    f.slot_lattice = E->d_slot_lat;
+    f.slot_begin = l >= 0 ? E->lat[l].slot_base : 0;
+    f.slot_count = l >= 0 ? E->lat[l].n_slots : E->total_slots;
+    f.n = n;
+    f.total_slots = E->total_slots;
+    f.R = E->R;
+    LBM_HIP(launch_flush(f, E->stream));
+    const int begin = l >= 0 ? l : 0, count = l >= 0 ? 1 : static_cast<int>(E->lat.size());
+    LBM_HIP(launch_add_steps(E->d_steps, E->d_par, begin, count, n, E->stream));
     return LBM_OK;
 }
 
-int build_graph(Ensemble* E)
+void drop_graphs(Ensemble* E)
 {
-    if (E->graph) {
-        (void)hipGraphExecDestroy(E->graph);
-        E->graph = nullptr;
-    }
+    for (auto& g : E->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    E->graph_dirty = false;
+}
+
+// Captures a batch of graph_steps launches over every lattice into graph[p] (p: the
+// starting parity of a one-lattice handle; 0 for ensembles, whose kernel reads it).
+int build_graph(Ensemble* E, int p)
+{
     hipGraph_t g;
     LBM_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_batch(E, E->graph_steps);
+    int rc = launch_batch(E, E->graph_steps, -1);
     hipError_t e = hipStreamEndCapture(E->stream, &g);
     if (rc != LBM_OK) return rc;
     if (e != hipSuccess) return fail(LBM_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&E->graph[p], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(LBM_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    E->graph_dirty = false;
+    if (e != hipSuccess) {
+        E->graph[p] = nullptr;
+        return fail(LBM_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
     return LBM_OK;
 }
 
@@ -800,11 +894,16 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
     }
     E->pool_elems = pool;
     E->cell_elems = cells;
+    // the step kernel issues a node's pull loads before it knows whether the node is on
+    // an edge: the last plane of the last lattice may be read up to one row past its
+    // end, so the buffers carry a guard of more than one row
+    int64_t guard = 256;
+    for (auto& H : E->lat) guard = std::max<int64_t>(guard, H.pitch + 256);
     int rc = LBM_OK;
     for (int b = 0; b < 2 && rc == LBM_OK; ++b) {
-        hipError_t e = hipMalloc(&E->buf[b], static_cast<size_t>(pool) * E->bpp);
+        hipError_t e = hipMalloc(&E->buf[b], static_cast<size_t>(pool + guard) * E->bpp);
         if (e != hipSuccess) rc = fail(LBM_ERR_NOMEM, "population buffers: " + std::string(hipGetErrorString(e)));
-        else E->device_bytes += pool * E->bpp;
+        else E->device_bytes += (pool + guard) * E->bpp;
     }
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_flags, cells);
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_lslot, cells);
@@ -846,10 +945,10 @@ int lbm_destroy(lbm_handle h)
     if (!h) return LBM_OK;
     Ensemble* E = h;
     if (E->stream) (void)hipStreamSynchronize(E->stream);
-    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    lbm::drop_graphs(E);
     void* ptrs[] = {E->buf[0], E->buf[1], E->d_flags, E->d_lslot, E->d_wdist, E->d_links, E->d_lat,
                     E->d_tiles, E->d_steps, E->d_par, E->d_acc, E->d_hist, E->d_dummy_acc, E->d_nan,
-                    E->d_dummy_nan, E->d_mass, E->d_stage};
+                    E->d_dummy_nan, E->d_mass, E->d_stage, E->d_slot_lat, E->d_special};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
@@ -1036,12 +1135,7 @@ int load_state(Ensemble* E, int l, const double* f, double rho, double u, double
     H.pend_fx.clear();
     H.pend_fy.clear();
     H.uploaded = true;
-    // clear this lattice's accumulators (both parities, every copy) and NaN flag
-    std::vector<double> z(static_cast<size_t>(H.n_slots) * 2, 0.0);
-    for (int p = 0; p < 2 * kAccCopies; ++p)
-        if ((rc = h2d(E, E->d_acc + (static_cast<int64_t>(p) * E->total_slots + H.slot_base) * 2, z.data(),
-                      z.size() * 8)) != LBM_OK)
-            return rc;
+    // the accumulators are clear between batches; reset the NaN flag
     const int32_t zero = 0;
     if ((rc = h2d(E, E->d_nan + l, &zero, 4)) != LBM_OK) return rc;
     E->counters_stale = true;
@@ -1102,17 +1196,18 @@ int lbm_step(lbm_handle h, int32_t n)
     int rc;
     if ((rc = require_uploaded(E)) != LBM_OK) return rc;
     if ((rc = commit(E)) != LBM_OK) return rc;
+    if (E->graph_dirty) drop_graphs(E);
     while (n > 0) {
         const bool use_graph = E->graph_steps >= 2 && n >= E->graph_steps;
-        const int batch = use_graph ? E->graph_steps : n;
+        const int batch = use_graph ? E->graph_steps : std::min(n, E->acc_blocks);
         for (size_t l = 0; l < E->lat.size(); ++l)
             if ((rc = ensure_history_room(E, static_cast<int>(l), batch)) != LBM_OK) return rc;
         if (use_graph) {
-            if (E->graph_dirty || !E->graph)
-                if ((rc = build_graph(E)) != LBM_OK) return rc;
-            LBM_HIP(hipGraphLaunch(E->graph, E->stream));
+            const int p = E->lat.size() == 1 ? E->lat[0].par : 0;
+            if (!E->graph[p] && (rc = build_graph(E, p)) != LBM_OK) return rc;
+            LBM_HIP(hipGraphLaunch(E->graph[p], E->stream));
         } else {
-            if ((rc = launch_batch(E, batch)) != LBM_OK) return rc;
+            if ((rc = launch_batch(E, batch, -1)) != LBM_OK) return rc;
         }
         for (auto& H : E->lat) {
             H.steps += batch;
@@ -1133,21 +1228,14 @@ int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
     HostLattice& H = E->lat[l];
     if (!H.uploaded) return fail(LBM_ERR_INVALID, "lattice has no state");
     if ((rc = commit(E)) != LBM_OK) return rc;
-    if ((rc = ensure_history_room(E, l, n)) != LBM_OK) return rc;
-    StepArgs a = base_args(E);
-    int blocks;
-    grid_for(E, l, a, blocks);
-    a.abs_mode = 1;
-    a.flush_begin = l;
-    a.flush_count = 1;
-    for (int i = 0; i < n; ++i) {
-        a.k_abs = H.steps + i;
-        a.src_par = H.par ^ (i & 1);
-        LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream));
+    while (n > 0) {
+        const int batch = std::min(n, E->acc_blocks);
+        if ((rc = ensure_history_room(E, l, batch)) != LBM_OK) return rc;
+        if ((rc = launch_batch(E, batch, l)) != LBM_OK) return rc;
+        H.steps += batch;
+        H.par ^= batch & 1;
+        n -= batch;
     }
-    H.steps += n;
-    H.par ^= n & 1;
-    E->counters_stale = true;
     return LBM_OK;
 }
 
@@ -1210,19 +1298,19 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         if ((rc = commit(E)) != LBM_OK) break;
         StepArgs a = base_args(E);
         int blocks;
-        grid_for(E, -1, a, blocks);
+        const bool single = grid_for(E, -1, a, blocks);
         // a scratch destination and dummy accumulators keep state and history intact
-        a.dst_override = scratch;
+        a.src = E->buf[E->lat[0].par];
+        a.dst = scratch;
         a.acc = E->d_dummy_acc;
         a.nan_flags = E->d_dummy_nan;
-        a.flush_count = 0;
         const int reps = 10;
         hipError_t le = hipSuccess;
         for (int w = 0; w < 2 && le == hipSuccess; ++w)
-            le = launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream);
+            le = launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream);
         (void)hipEventRecord(e0, E->stream);
         for (int r = 0; r < reps && le == hipSuccess; ++r)
-            le = launch_step(E->storage, E->bx, E->rows, E->vec, blocks, a, E->stream);
+            le = launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream);
         (void)hipEventRecord(e1, E->stream);
         if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
             rc = fail(LBM_ERR_DEVICE, "autotune launch failed");
@@ -1267,23 +1355,6 @@ int lbm_forces(lbm_handle h, int32_t l, double* fx, double* fy, int32_t max_step
     Ensemble* E = h;
     HostLattice& H = E->lat[l];
     if ((rc = drain(E, l)) != LBM_OK) return rc;
-    if (H.steps >= 1 && H.read_pos == H.steps - 1) {
-        LBM_HIP(hipStreamSynchronize(E->stream));
-        // the newest step is still in the accumulator copies: sum them in the kernel's order
-        std::vector<double> all(static_cast<size_t>(kAccCopies) * E->total_slots * 2);
-        const int64_t off = ((H.steps - 1) & 1) * static_cast<int64_t>(kAccCopies) * E->total_slots * 2;
-        if ((rc = d2h(E, all.data(), E->d_acc + off, all.size() * 8)) != LBM_OK) return rc;
-        for (int s = 0; s < H.n_slots; ++s) {
-            double fx = 0.0, fy = 0.0;
-            for (int cp = 0; cp < kAccCopies; ++cp) {
-                fx += all[(static_cast<size_t>(cp) * E->total_slots + H.slot_base + s) * 2];
-                fy += all[(static_cast<size_t>(cp) * E->total_slots + H.slot_base + s) * 2 + 1];
-            }
-            H.pend_fx.push_back(fx);
-            H.pend_fy.push_back(fy);
-        }
-        H.read_pos = H.steps;
-    }
     const int64_t avail = static_cast<int64_t>(H.pend_fx.size()) / H.n_slots;
     const int64_t take = std::min<int64_t>(avail, max_steps);
     if (take > 0 && (!fx || !fy)) return fail(LBM_ERR_INVALID, "null force output");

@@ -1,6 +1,7 @@
 // lbm_internal.h -- structures shared by the host runtime and the gfx950 kernels.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 namespace lbm {
@@ -40,16 +41,21 @@ struct alignas(16) LatticeDev {
     float pad1[2];
 };
 
+constexpr uint32_t LINK_MOVING = 1u << 31;
+
 // Boundary-link record for one fluid node with links (indexed by link_slot[cell]).
 struct alignas(16) LinkRec {
     uint32_t mask;       // bit i: the neighbour along e_i is solid (link i)
-    uint32_t scheme;     // 2 bits per direction: enum lbm_bb
+    uint32_t scheme;     // 2 bits per direction: enum lbm_bb; LINK_MOVING: some wall velocity != 0
     int16_t slot[9];     // force slot per link, -1 = excluded
-    int16_t pad;
+    int16_t pad;         // (slot and pad are read as 5 dwords)
+    int32_t cell;        // the node, y * pitch + x in its lattice
     double q[9];
     double uwx[9];
     double uwy[9];
 };
+
+static_assert(offsetof(LinkRec, slot) == 8 && offsetof(LinkRec, cell) == 28, "LinkRec layout");
 
 // One block of work of the step kernel: lattice, first row, first column.
 struct Tile {

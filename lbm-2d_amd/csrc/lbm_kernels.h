@@ -17,33 +17,52 @@ enum : int32_t {
 };
 
 struct StepArgs {
+    // single-lattice launches: everything the kernel needs before its first population
+    // load is here, by value
+    LatticeDev L0;
+    int32_t lattice0;
+    int32_t tiles_x, tiles_y;   // tiles per row band, row bands
+    int32_t options;            // LAUNCH_* bits
+    // general-path nodes, taken by the first special_blocks blocks: the lattice's
+    // boundary-link records, its edge nodes, then its written solids (list of
+    // in-lattice indices y*pitch+x)
+    int32_t n_links;
+    int32_t n_edges;
+    const int32_t* special;
+    int32_t n_special;
+    int32_t special_blocks;
+    const void* src;            // post-collision populations of the previous step
+    void* dst;                  // new post-collision populations (ensemble mode: non-null
+                                // overrides the per-lattice buffer, for the autotuner)
+    // ensemble launches
     const LatticeDev* lat;
-    const Tile* tiles;          // ensemble mode: one entry per block
+    const Tile* tiles;          // one entry per block
     int32_t n_tiles;
-    int32_t single;             // single-lattice mode: lattice index (tiles == nullptr)
-    int32_t tiles_x;            // single-lattice mode: tiles per row band
-    int32_t tiles_y;
+    int32_t step_arg;           // launch index inside the batch (buffer parity)
+    const int32_t* par;         // per-lattice index of the buffer holding the state
     void* buf0;
     void* buf1;
-    void* dst_override;         // non-null: write here instead of the other buffer
+    // both
     const uint8_t* flags;
     const int32_t* link_slot;
     const LinkRec* links;
     const uint16_t* wdist;
-    const int64_t* steps;       // per-lattice step count at the start of the batch
-    const int32_t* par;         // per-lattice index of the buffer holding the state
-    double* acc;                // [2][kAccCopies][total_slots][2] force accumulators
-    double* hist;               // [total_slots][R][2] force history ring
-    int32_t R;
+    double* acc;                // this launch's accumulator block [kAccCopies][total_slots][2]
     int32_t total_slots;
     int32_t* nan_flags;         // per lattice
-    int32_t step_arg;           // launch index inside the batch
-    int32_t abs_mode;           // 1: k = k_abs and src = buf[src_par] (one lattice)
-    int64_t k_abs;
-    int32_t src_par;
-    int32_t flush_begin, flush_count;
-    int32_t options;            // LAUNCH_* bits
     float scale, inv_scale;     // 2^k and 2^-k of the 16-bit forms
+};
+
+// Force totals of a batch of n launches: accumulator blocks -> history ring.
+struct FlushArgs {
+    double* acc;                // [n][kAccCopies][total_slots][2], cleared on the way
+    double* hist;               // [total_slots][R][2]
+    const int64_t* steps;       // per-lattice step count at the start of the batch
+    const int32_t* slot_lattice;  // lattice of each force slot
+    int32_t slot_begin, slot_count;
+    int32_t n;
+    int32_t total_slots;
+    int32_t R;
 };
 
 // Materialise the reference-visible state of rows [y0,y1) of one lattice as AoS FP64
@@ -79,8 +98,12 @@ struct PrologueArgs {
     float scale, inv_scale;
 };
 
-hipError_t launch_step(int storage, int block_x, int rows, int vec, int n_blocks, const StepArgs& a,
+// single: the one-lattice kernel (StepArgs.L0, src, dst); else the ensemble kernel.
+hipError_t launch_step(int storage, int block_x, int rows, int vec, bool single, int n_blocks, const StepArgs& a,
                        hipStream_t s);
+hipError_t launch_flush(const FlushArgs& f, hipStream_t s);
+// Blocks of block_x threads that take n general-path list nodes.
+int list_blocks(int block_x, int n);
 hipError_t launch_state(int storage, const StateArgs& a, int nx, hipStream_t s);
 hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream_t s);
 hipError_t launch_add_steps(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,

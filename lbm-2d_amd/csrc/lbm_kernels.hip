@@ -212,17 +212,73 @@ template <int ST> struct Ctx {
     float inv_scale;
 };
 
-// Post-stream, pre-boundary deviations at node (px,py), read from the
-// post-collision buffer (pull form of the streaming loops src/lbm.hpp:408-484).
+// What the general path reads at node (px,py), fetched in one round of independent
+// loads: its flags and link slot, its own stored values and, per direction j, the
+// stored value at the streaming source (px,py) - e_j, wrapped on periodic edges.  Off
+// the lattice on other edges the address lies outside it but inside the population
+// pool (whose guard covers one row past its end); those values go unused.
+template <int ST> struct Pre {
+    using S = typename Form<ST>::store;
+    uint8_t fl;
+    int32_t ls;
+    S own[9];
+    S pull[9];
+};
+
 template <int ST>
-__device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
-                                            typename Form<ST>::real d[9])
+__device__ __forceinline__ void preload(const Ctx<ST>& c, int px, int py, Pre<ST>& p)
+{
+    const LatticeDev& L = c.L;
+    const int64_t idx = static_cast<int64_t>(py) * L.pitch + px;
+    p.fl = c.flags[L.cell_off + idx];
+    p.ls = c.link_slot[L.cell_off + idx];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        int sx = px - CX[j], sy = py - CY[j];
+        if (L.x_edge == EDGE_PERIODIC) sx = sx < 0 ? sx + L.nx : (sx >= L.nx ? sx - L.nx : sx);
+        if (L.y_edge == EDGE_PERIODIC) sy = sy < 0 ? sy + L.ny : (sy >= L.ny ? sy - L.ny : sy);
+        p.own[j] = c.base[j * L.plane + idx];
+        p.pull[j] = c.base[j * L.plane + static_cast<int64_t>(sy) * L.pitch + sx];
+    }
+}
+
+// Boundary-link data of one node: mask, schemes, force slots (int16 pairs) and q in
+// registers, loaded with the record unconditionally; wall velocities (moving walls
+// only) are read from the record where they are used.
+template <typename R> struct LinkVals {
+    uint32_t mask = 0, scheme = 0;
+    uint32_t slots[5];
+    R q[9];
+    const LinkRec* rec = nullptr;
+    __device__ __forceinline__ int slot(int i) const
+    {
+        return static_cast<int16_t>(static_cast<uint16_t>(slots[i >> 1] >> (16 * (i & 1))));
+    }
+};
+
+template <typename R> __device__ __forceinline__ void load_link(const LinkRec& r, LinkVals<R>& v)
+{
+    v.rec = &r;
+    v.mask = r.mask;
+    v.scheme = r.scheme;
+    const uint32_t* sl = reinterpret_cast<const uint32_t*>(r.slot);  // slot[9] and pad: 5 dwords
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v.slots[k] = sl[k];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v.q[i] = static_cast<R>(r.q[i]);
+}
+
+// Post-stream, pre-boundary deviations at node (px,py), from the post-collision
+// buffer (pull form of the streaming loops src/lbm.hpp:408-484).  lk: the node's
+// boundary links (read when p.fl has CELL_LINKS).
+template <int ST>
+__device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py, const Pre<ST>& p,
+                                               const LinkVals<typename Form<ST>::real>& lk,
+                                               typename Form<ST>::real d[9])
 {
     using R = typename Form<ST>::real;
     const LatticeDev& L = c.L;
-    const int64_t idx = static_cast<int64_t>(py) * L.pitch + px;
-    const uint8_t fl = c.flags[L.cell_off + idx];
-    if (fl & CELL_SOLID) {
+    if (p.fl & CELL_SOLID) {
         // solid nodes are never streamed into: f_next stays zero (src/lbm.hpp:400)
 #pragma unroll
         for (int j = 0; j < 9; ++j) d[j] = -wgt<R>(j);
@@ -230,14 +286,9 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
     }
     R own[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) own[i] = Form<ST>::dec(c.base[i * L.plane + idx], c.inv_scale, L.shift[i]);
+    for (int i = 0; i < 9; ++i) own[i] = Form<ST>::dec(p.own[i], c.inv_scale, L.shift[i]);
 
-    const LinkRec* rec = nullptr;
-    uint32_t mask = 0;
-    if (fl & CELL_LINKS) {
-        rec = &c.links[L.link_off + c.link_slot[L.cell_off + idx]];
-        mask = rec->mask;
-    }
+    const uint32_t mask = (p.fl & CELL_LINKS) ? lk.mask : 0u;
     R rho_own = R(1);
     if (mask) {
         R s = R(0);
@@ -251,9 +302,11 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
         const int i = OPP[j];  // push direction whose target is px - e_j
         if (mask & (1u << i)) {
             // boundary link (px,py) -> (px,py)+e_i solid: value for slot j = opp(i)
-            const int scheme = (rec->scheme >> (2 * i)) & 3;
-            const R q = static_cast<R>(rec->q[i]);
-            const R uwx = static_cast<R>(rec->uwx[i]), uwy = static_cast<R>(rec->uwy[i]);
+            const int scheme = (lk.scheme >> (2 * i)) & 3;
+            const R q = lk.q[i];
+            const bool moving = (lk.scheme & LINK_MOVING) != 0;
+            const R uwx = moving ? static_cast<R>(lk.rec->uwx[i]) : R(0);
+            const R uwy = moving ? static_cast<R>(lk.rec->uwy[i]) : R(0);
             const R eu = R(CX[i]) * uwx + R(CY[i]) * uwy;
             R val;
             if (scheme == BB_MEI) {
@@ -267,8 +320,8 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
                 if (q < R(0.5)) {
                     const int sx = px - CX[i], sy = py - CY[i];
                     if (sx >= 0 && sx < L.nx && sy >= 0 && sy < L.ny) {
-                        const R up = Form<ST>::dec(
-                            c.base[i * L.plane + static_cast<int64_t>(sy) * L.pitch + sx], c.inv_scale, L.shift[i]);
+                        // the upstream node's stored value: the (unwrapped) source of direction i
+                        const R up = Form<ST>::dec(p.pull[i], c.inv_scale, L.shift[i]);
                         // the reference reads f (not f - w): w_i cancels since weights sum to 1
                         val = R(2) * q * own[i] + (R(1) - R(2) * q) * up;
                     } else {
@@ -285,25 +338,16 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
             d[j] = val;
             continue;
         }
-        int sx = px - CX[j], sy = py - CY[j];
+        const int sx = px - CX[j], sy = py - CY[j];
         bool wall = false, open = false;
-        if (sx < 0 || sx >= L.nx) {
-            if (L.x_edge == EDGE_PERIODIC) sx = sx < 0 ? sx + L.nx : sx - L.nx;
-            else if (L.x_edge == EDGE_WALL) wall = true;
-            else open = true;
-        }
-        if (sy < 0 || sy >= L.ny) {
-            if (L.y_edge == EDGE_PERIODIC) sy = sy < 0 ? sy + L.ny : sy - L.ny;
-            else if (L.y_edge == EDGE_WALL) wall = true;
-            else open = true;
-        }
+        if ((sx < 0 || sx >= L.nx) && L.x_edge != EDGE_PERIODIC) (L.x_edge == EDGE_WALL ? wall : open) = true;
+        if ((sy < 0 || sy >= L.ny) && L.y_edge != EDGE_PERIODIC) (L.y_edge == EDGE_WALL ? wall : open) = true;
         if (wall) {
             d[j] = own[i];  // bounced into the same node (src/lbm.hpp:418-419)
         } else if (open) {
             d[j] = -wgt<R>(j);  // dropped: f_next keeps its zero (src/lbm.hpp:470)
         } else {
-            d[j] = Form<ST>::dec(c.base[j * L.plane + static_cast<int64_t>(sy) * L.pitch + sx],
-                                 c.inv_scale, L.shift[j]);
+            d[j] = Form<ST>::dec(p.pull[j], c.inv_scale, L.shift[j]);
         }
     }
 }
@@ -321,17 +365,24 @@ template <typename R> __device__ __forceinline__ void zou_he(R d[9], R u)
 // Node state after streaming and the open boundaries (src/lbm.hpp:530-539).
 // Boundary order: inlet, outlet copy, lid.  Solid nodes skip the inlet and the lid,
 // which leave their zero populations unchanged.
+// p holds the preloaded values of the gathered node: (x,y), or (nx-2,y) on the outlet
+// column (gathered_x).
+template <int ST> __device__ __forceinline__ int gathered_x(const LatticeDev& L, int x)
+{
+    return L.outlet && x == L.nx - 1 ? L.nx - 2 : x;  // enforce_outflow copies x = nx-2
+}
+
 template <int ST>
-__device__ __forceinline__ void state_general(const Ctx<ST>& c, int x, int y, bool solid,
-                              typename Form<ST>::real d[9])
+__device__ __forceinline__ void state_general(const Ctx<ST>& c, int x, int y, bool solid, const Pre<ST>& p,
+                                              const LinkVals<typename Form<ST>::real>& lk,
+                                              typename Form<ST>::real d[9])
 {
     using R = typename Form<ST>::real;
     const LatticeDev& L = c.L;
-    const bool outlet_col = L.outlet && x == L.nx - 1;
-    const int gx = outlet_col ? L.nx - 2 : x;  // enforce_outflow copies x = nx-2
-    gather_general<ST>(c, gx, y, d);
+    const int gx = gathered_x<ST>(L, x);
+    gather_general<ST>(c, gx, y, p, lk, d);
     if (gx == 0 && L.inlet != INLET_NONE) {
-        const bool gsolid = (c.flags[L.cell_off + static_cast<int64_t>(y) * L.pitch + gx] & CELL_SOLID) != 0;
+        const bool gsolid = (p.fl & CELL_SOLID) != 0;
         if (!gsolid) {
             if (L.inlet == INLET_ZOU_HE) {
                 zou_he<R>(d, static_cast<R>(L.u_in));
@@ -431,7 +482,107 @@ template <typename T> __device__ __forceinline__ void store_val(T* p, const T& v
 }
 
 // One node, one cell per thread: the bulk pull or the general path.
+// Links of the node a Pre was taken at (none unless it has CELL_LINKS).
 template <int ST>
+__device__ __forceinline__ void links_of(const Ctx<ST>& c, const Pre<ST>& p, LinkVals<typename Form<ST>::real>& lk)
+{
+    if (p.fl & CELL_LINKS) load_link(c.links[c.L.link_off + p.ls], lk);
+}
+
+// The general path of one node: boundary state from the preloaded gathered node (p,
+// lg), momentum exchange on it with the node's own links (ls_, src/lbm.hpp:542-581),
+// collision, store.
+template <int ST>
+__device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
+                                             typename Form<ST>::store* dst, double* accp, int x, int y,
+                                             uint8_t fl, const Pre<ST>& p,
+                                             const LinkVals<typename Form<ST>::real>& lg,
+                                             const LinkVals<typename Form<ST>::real>& ls_, uint16_t wraw,
+                                             ForcePart& part, bool& bad)
+{
+    using R = typename Form<ST>::real;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    const bool solid = (fl & CELL_SOLID) != 0;
+    R d[9];
+    state_general<ST>(c, x, y, solid, p, lg, d);
+    if ((fl & CELL_LINKS) && L.forces) {
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            if (!(ls_.mask & (1u << i)) || ls_.slot(i) < 0) continue;
+            const double fb = WD[OPP[i]] + static_cast<double>(d[OPP[i]]);
+            part.add(L.slot_base + ls_.slot(i), CX[i] * 2.0 * fb, CY[i] * 2.0 * fb, accp);
+        }
+    }
+    if (!solid) {
+        R wd = R(0);
+        if (L.les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
+        collide<R>(d, L, wd, R(1));
+        if (!(d[0] == d[0])) bad = true;
+    }
+    auto* o = dst + L.f_off + idx;
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale, L.shift[j]), a.options & LAUNCH_STREAM_STORES);
+}
+
+// Lanes of a wave that take general-path list nodes.
+constexpr int kListLanes = 16;
+
+// Edge node number e of a lattice: rows 0 and ny-1, then columns 0 and nx-1 between them.
+__device__ __forceinline__ void edge_position(const LatticeDev& L, int e, int& x, int& y)
+{
+    const int rows = L.ny == 1 ? 1 : 2;
+    if (e < L.nx) {
+        x = e;
+        y = 0;
+    } else if (rows == 2 && e < 2 * L.nx) {
+        x = e - L.nx;
+        y = L.ny - 1;
+    } else {
+        const int k = e - rows * L.nx, h = L.ny - 2;
+        x = k < h ? 0 : L.nx - 1;
+        y = (k < h ? k : k - h) + 1;
+    }
+}
+
+// The general-path node t of a one-lattice launch: boundary-link record t (t < n_links;
+// the record gives the position, so links, flags, stored values and sources load in
+// two rounds), else edge node t - n_links (nodes with links are the records'; solid
+// ones are written only when marked).  One call site of node_general keeps one copy
+// of the general path in the kernel.
+template <int ST>
+__device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
+                                          typename Form<ST>::store* dst, double* accp, int t, ForcePart& part,
+                                          bool& bad)
+{
+    using R = typename Form<ST>::real;
+    const bool is_link = t < a.n_links;
+    LinkVals<R> own_links;
+    int x, y;
+    if (is_link) {
+        const LinkRec& rec = c.links[L.link_off + t];
+        load_link(rec, own_links);
+        const int32_t e = rec.cell;
+        y = e / L.pitch;
+        x = e - y * L.pitch;
+    } else {
+        edge_position(L, t - a.n_links, x, y);
+    }
+    const int gx = gathered_x<ST>(L, x);
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    const uint8_t fl = a.flags[L.cell_off + idx];
+    Pre<ST> p;
+    preload<ST>(c, gx, y, p);
+    const uint16_t wraw = L.les ? a.wdist[L.cell_off + idx] : uint16_t(0xffffu);
+    if (!is_link && (fl & CELL_LINKS)) return;
+    if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return;
+    LinkVals<R> lg = own_links;
+    if (!is_link || gx != x) links_of<ST>(c, p, lg);
+    node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, own_links, wraw, part, bad);
+}
+
+// One node, bulk or general.  SKIP_SPECIAL: general nodes are left to the lists.
+template <int ST, bool SKIP_SPECIAL>
 __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs& a,
                                             const Ctx<ST>& c, typename Form<ST>::store* dst,
                                             double* accp, int x, int y, ForcePart& part, bool& bad)
@@ -439,37 +590,41 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    // The flag, the bulk pull values and the wall distance in one round trip: the pull
+    // addresses of any node lie inside the population pool (which ends with a guard of
+    // more than one row), and the asm below keeps the loads ahead of the branch.
     const uint8_t fl = a.flags[L.cell_off + idx];
+    S raw[9];
+    const S* b = c.base + idx;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) raw[j] = b[j * L.plane - (CY[j] * L.pitch + CX[j])];
+    uint16_t wraw = 0xffffu;
+    if (L.les) wraw = a.wdist[L.cell_off + idx];
+    __asm__ volatile("" ::"v"(raw[0]), "v"(raw[1]), "v"(raw[2]), "v"(raw[3]), "v"(raw[4]), "v"(raw[5]),
+                     "v"(raw[6]), "v"(raw[7]), "v"(raw[8]), "v"(wraw));
+    if (fl & CELL_SPECIAL) {
+        if (SKIP_SPECIAL) return;
+        if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return;
+        const int gx = gathered_x<ST>(L, x);
+        Pre<ST> p;
+        preload<ST>(c, gx, y, p);
+        LinkVals<R> lg, ls_;
+        links_of<ST>(c, p, lg);
+        if (gx == x) {
+            node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, lg, wraw, part, bad);
+        } else {
+            if (fl & CELL_LINKS) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
+            node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, ls_, wraw, part, bad);
+        }
+        return;
+    }
     R d[9];
-    const bool solid = (fl & CELL_SOLID) != 0;
-    if (!(fl & CELL_SPECIAL)) {
-        const S* b = c.base + idx;
 #pragma unroll
-        for (int j = 0; j < 9; ++j)
-            d[j] = Form<ST>::dec(b[j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale, L.shift[j]);
-    } else {
-        if (solid && !(fl & CELL_WRITE)) return;
-        state_general<ST>(c, x, y, solid, d);
-        if ((fl & CELL_LINKS) && L.forces) {
-            // momentum exchange on the post-boundary state (src/lbm.hpp:542-581)
-            const LinkRec& rec = a.links[L.link_off + a.link_slot[L.cell_off + idx]];
-#pragma unroll
-            for (int i = 1; i < 9; ++i) {
-                if (!(rec.mask & (1u << i)) || rec.slot[i] < 0) continue;
-                const double fb = WD[OPP[i]] + static_cast<double>(d[OPP[i]]);
-                part.add(L.slot_base + rec.slot[i], CX[i] * 2.0 * fb, CY[i] * 2.0 * fb, accp);
-            }
-        }
-    }
-    if (!solid) {
-        R wd = R(0);
-        if (L.les) {
-            const uint16_t w = a.wdist[L.cell_off + idx];
-            wd = w == 0xffffu ? R(1e9) : static_cast<R>(w);
-        }
-        collide<R>(d, L, wd, R(1));
-        if (!(d[0] == d[0])) bad = true;
-    }
+    for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(raw[j], a.inv_scale, L.shift[j]);
+    R wd = R(0);
+    if (L.les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
+    collide<R>(d, L, wd, R(1));
+    if (!(d[0] == d[0])) bad = true;
     S* o = dst + L.f_off + idx;
 #pragma unroll
     for (int j = 0; j < 9; ++j)
@@ -526,6 +681,22 @@ __device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs
 #pragma unroll
     for (int j = 0; j < 9; ++j)
         in[j] = *reinterpret_cast<const P*>(c.base + j * L.plane + idx - CY[j] * L.pitch);
+    // the wave's first and last lanes fetch the values from outside the wave in the
+    // same round trip as the planes (interior row: every address is inside the pool)
+    const int lane = __lane_id();
+    S far[9];
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+            if (CX[j] == 1) far[j] = c.base[j * L.plane + idx - CY[j] * L.pitch - 1];
+    }
+    if (lane == 63) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+            if (CX[j] == -1) far[j] = c.base[j * L.plane + idx - CY[j] * L.pitch + V];
+    }
+    Pack<uint16_t, V> wd4;
+    if (L.les) wd4 = *reinterpret_cast<const Pack<uint16_t, V>*>(a.wdist + L.cell_off + idx);
     S edge[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
@@ -533,19 +704,16 @@ __device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs
         if (CX[j] == -1) edge[j] = shift_in<S>(in[j].v[0], false);
     }
     if (!bulk) return;
-    const int lane = __lane_id();
     if (lane == 0) {
 #pragma unroll
         for (int j = 0; j < 9; ++j)
-            if (CX[j] == 1) edge[j] = c.base[j * L.plane + idx - CY[j] * L.pitch - 1];
+            if (CX[j] == 1) edge[j] = far[j];
     }
     if (lane == 63) {
 #pragma unroll
         for (int j = 0; j < 9; ++j)
-            if (CX[j] == -1) edge[j] = c.base[j * L.plane + idx - CY[j] * L.pitch + V];
+            if (CX[j] == -1) edge[j] = far[j];
     }
-    Pack<uint16_t, V> wd4;
-    if (L.les) wd4 = *reinterpret_cast<const Pack<uint16_t, V>*>(a.wdist + L.cell_off + idx);
     P out[9];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
@@ -574,7 +742,11 @@ __device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs
         store_val(reinterpret_cast<P*>(dst + L.f_off + j * L.plane + idx), out[j], a.options & LAUNCH_STREAM_STORES);
 }
 
-template <int ST, int BX, int ROWS, int V>
+// One step of one lattice (SINGLE: descriptor, buffers and tile grid in the kernel
+// arguments, nothing loaded before the populations) or of every lattice of an ensemble
+// (tile table; per-lattice buffer parity from the device counters).  Forces go to this
+// launch's accumulator block, copy blockIdx % kAccCopies; flush_kernel sums them.
+template <int ST, int BX, int ROWS, int V, bool SINGLE>
 __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
 {
     using S = typename Form<ST>::store;
@@ -583,54 +755,75 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
     // takes one contiguous range of tiles, so lines shared by neighbouring tiles stay
     // in one L2.  The map is a bijection on [0, gridDim.x).
     int b = static_cast<int>(blockIdx.x);
+    const int nb = static_cast<int>(gridDim.x);
+    int lat_id, y0, x0;
+    const S* src;
+    S* dst;
+    LatticeDev L;  // by value: scalar registers, loaded before any store
+    if constexpr (SINGLE) {
+        L = a.L0;
+        lat_id = a.lattice0;
+        src = static_cast<const S*>(a.src);
+        dst = static_cast<S*>(a.dst);
+        // every argument the first memory round needs, fetched from the kernel argument
+        // segment in one group instead of between branches
+        __asm__ volatile("" ::"s"(src), "s"(dst), "s"(a.flags), "s"(a.wdist), "s"(a.acc), "s"(a.total_slots),
+                         "s"(a.tiles_x), "s"(a.options), "s"(nb), "s"(L.nx), "s"(L.ny), "s"(L.pitch),
+                         "s"(L.f_off), "s"(L.plane), "s"(L.cell_off), "s"(L.les), "s"(a.special),
+                         "s"(a.n_special), "s"(a.special_blocks), "s"(a.n_links), "s"(a.n_edges));
+        // The first blocks take the general-path nodes -- boundary-link records, edge
+        // nodes, written solids: dispatched first, their longer load chains overlap the
+        // bulk tiles, which skip those nodes.
+        if (b < a.special_blocks) {
+            double* accp = a.acc + static_cast<int64_t>(blockIdx.x % kAccCopies) * a.total_slots * 2;
+            const Ctx<ST> c{L, src + L.f_off, a.flags, a.link_slot, a.links, a.inv_scale};
+            ForcePart part;
+            bool bad = false;
+            // link and edge nodes: kListLanes per wave, since their loads scatter and a
+            // wave's memory requests issue one address at a time
+            constexpr int per_block = BX / 64 * kListLanes;
+            const int gb = (a.n_links + a.n_edges + per_block - 1) / per_block;
+            if (b < gb) {
+                const int lane = static_cast<int>(threadIdx.x) & 63;
+                const int t = b * per_block + (static_cast<int>(threadIdx.x) >> 6) * kListLanes + lane;
+                if (lane < kListLanes && t < a.n_links + a.n_edges) list_node<ST>(L, a, c, dst, accp, t, part, bad);
+            } else {
+                // written solids (Bouzidi sources): zero populations, f - w = -w
+                const int t = (b - gb) * BX + static_cast<int>(threadIdx.x);
+                if (t < a.n_special) {
+                    S* o = dst + L.f_off + a.special[t];
+#pragma unroll
+                    for (int j = 0; j < 9; ++j)
+                        o[j * L.plane] = Form<ST>::enc(-static_cast<typename Form<ST>::real>(WD[j]), a.scale,
+                                                       L.shift[j]);
+                }
+            }
+            if (bad) atomicOr(a.nan_flags + lat_id, 1);
+            flush_forces(part, accp);
+            return;
+        }
+        b -= a.special_blocks;
+    }
     if (a.options & LAUNCH_XCD_ORDER) {
-        const int nb = static_cast<int>(gridDim.x), q = nb >> 3, r = nb & 7, xcd = b & 7;
+        // over the tile blocks; (b & 7) still names one XCD's blocks after the offset
+        const int nt = nb - (SINGLE ? a.special_blocks : 0);
+        const int q = nt >> 3, r = nt & 7, xcd = b & 7;
         b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
     }
-    int lat_id, y0, x0;
-    if (a.tiles) {
+    if constexpr (SINGLE) {
+        y0 = (b / a.tiles_x) * ROWS;
+        x0 = (b % a.tiles_x) * (BX * V);
+    } else {
         const Tile t = a.tiles[b];
         lat_id = t.lattice;
         y0 = t.y0;
         x0 = t.x0;
-    } else {
-        lat_id = a.single;
-        const int tx = b % a.tiles_x;
-        const int ty = b / a.tiles_x;
-        y0 = ty * ROWS;
-        x0 = tx * BX * V;
+        L = a.lat[lat_id];
+        const int sp = a.par[lat_id] ^ (a.step_arg & 1);
+        src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
+        dst = static_cast<S*>(a.dst ? a.dst : (sp ? a.buf0 : a.buf1));
     }
-    const LatticeDev L = a.lat[lat_id];  // by value: scalar registers, loaded before any store
-    const int64_t k = a.abs_mode ? a.k_abs : a.steps[lat_id] + a.step_arg;
-    const int sp = a.abs_mode ? a.src_par : (a.par[lat_id] ^ (a.step_arg & 1));
-    const S* src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
-    S* dst = static_cast<S*>(a.dst_override ? a.dst_override : (sp ? a.buf0 : a.buf1));
-    double* accp = a.acc + (static_cast<int64_t>(k & 1) * kAccCopies + blockIdx.x % kAccCopies) * a.total_slots * 2;
-
-    // Step k-1 is complete: move its force totals to the history, clear its accumulator.
-    if (blockIdx.x == 0) {
-        for (int l = a.flush_begin + static_cast<int>(threadIdx.x); l < a.flush_begin + a.flush_count;
-             l += BX) {
-            const int64_t kl = a.abs_mode ? a.k_abs : a.steps[l] + a.step_arg;
-            if (kl < 1) continue;
-            const LatticeDev& Ll = a.lat[l];
-            double* prev = a.acc + static_cast<int64_t>((kl - 1) & 1) * kAccCopies * a.total_slots * 2;
-            const int64_t row = (kl - 1) % a.R;
-            for (int s = 0; s < Ll.n_slots; ++s) {
-                const int gs = Ll.slot_base + s;
-                double fx = 0.0, fy = 0.0;
-                for (int cp = 0; cp < kAccCopies; ++cp) {
-                    double* q = prev + (static_cast<int64_t>(cp) * a.total_slots + gs) * 2;
-                    fx += q[0];
-                    fy += q[1];
-                    q[0] = 0.0;
-                    q[1] = 0.0;
-                }
-                a.hist[(static_cast<int64_t>(gs) * a.R + row) * 2] = fx;
-                a.hist[(static_cast<int64_t>(gs) * a.R + row) * 2 + 1] = fy;
-            }
-        }
-    }
+    double* accp = a.acc + static_cast<int64_t>(blockIdx.x % kAccCopies) * a.total_slots * 2;
 
     const Ctx<ST> c{L, src + L.f_off, a.flags, a.link_slot, a.links, a.inv_scale};
     ForcePart part;
@@ -663,7 +856,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
             // one copy of the general path, looped (keeps the kernel small)
 #pragma unroll 1
             for (int q = 0; q < V; ++q)
-                if (x + q < L.nx) cell_update<ST>(L, a, c, dst, accp, x + q, y, part, bad);
+                if (x + q < L.nx) cell_update<ST, SINGLE>(L, a, c, dst, accp, x + q, y, part, bad);
         }
     }
     if (bad) atomicOr(a.nan_flags + lat_id, 1);
@@ -696,7 +889,11 @@ __global__ __launch_bounds__(256) void state_kernel(StateArgs a)
                 d[j] = Form<ST>::dec(base[idx + j * L.plane - (CY[j] * L.pitch + CX[j])], a.inv_scale, L.shift[j]);
         } else {
             const Ctx<ST> c{L, base, a.flags, a.link_slot, a.links, a.inv_scale};
-            state_general<ST>(c, x, y, solid, d);
+            Pre<ST> p;
+            preload<ST>(c, gathered_x<ST>(L, x), y, p);
+            LinkVals<R> lk;
+            links_of<ST>(c, p, lk);
+            state_general<ST>(c, x, y, solid, p, lk, d);
         }
         if (a.out) {
             double* o = a.out + (static_cast<int64_t>(y - a.y0) * L.nx + x) * 9;
@@ -798,16 +995,49 @@ __global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, in
     }
 }
 
+// Moves the force totals of a batch of n launches to the history ring and clears the
+// accumulators: thread (i, slot) sums the kAccCopies copies of launch i in copy order
+// and writes history row (steps[lattice] + i) % R.  Runs before add_steps_kernel.
+__global__ void flush_kernel(FlushArgs f)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= f.n * f.slot_count) return;
+    const int i = p / f.slot_count;
+    const int s = f.slot_begin + p % f.slot_count;
+    double* blk = f.acc + static_cast<int64_t>(i) * kAccCopies * f.total_slots * 2;
+    double v[2 * kAccCopies];
+#pragma unroll
+    for (int c = 0; c < kAccCopies; ++c) {
+        v[2 * c] = blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2];
+        v[2 * c + 1] = blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2 + 1];
+    }
+    double fx = 0.0, fy = 0.0;
+#pragma unroll
+    for (int c = 0; c < kAccCopies; ++c) {
+        fx += v[2 * c];
+        fy += v[2 * c + 1];
+    }
+#pragma unroll
+    for (int c = 0; c < kAccCopies; ++c) {
+        blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2] = 0.0;
+        blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2 + 1] = 0.0;
+    }
+    const int64_t row = (f.steps[f.slot_lattice[s]] + i) % f.R;
+    f.hist[(static_cast<int64_t>(s) * f.R + row) * 2] = fx;
+    f.hist[(static_cast<int64_t>(s) * f.R + row) * 2 + 1] = fy;
+}
+
 // A group of nodes moves at most 16 bytes per plane and thread (one dwordx4).
 template <int ST, int V> constexpr bool group_fits() { return sizeof(typename Form<ST>::store) * V <= 16; }
 
 template <int ST, int BX, int ROWS, int V>
-hipError_t launch_step_t(int n_blocks, const StepArgs& a, hipStream_t s)
+hipError_t launch_step_t(bool single, int n_blocks, const StepArgs& a, hipStream_t s)
 {
     if constexpr (!group_fits<ST, V>()) {
         return hipErrorInvalidValue;
     } else {
-        hipLaunchKernelGGL((step_kernel<ST, BX, ROWS, V>), dim3(n_blocks), dim3(BX), 0, s, a);
+        if (single) hipLaunchKernelGGL((step_kernel<ST, BX, ROWS, V, true>), dim3(n_blocks), dim3(BX), 0, s, a);
+        else hipLaunchKernelGGL((step_kernel<ST, BX, ROWS, V, false>), dim3(n_blocks), dim3(BX), 0, s, a);
         return hipGetLastError();
     }
 }
@@ -818,10 +1048,10 @@ hipError_t launch_step_t(int n_blocks, const StepArgs& a, hipStream_t s)
     X(64, 1, 4) X(128, 1, 4) X(256, 1, 4) X(64, 2, 4) X(128, 2, 4) X(64, 1, 8) X(128, 1, 8)
 
 template <int ST>
-hipError_t launch_step_st(int bx, int rows, int v, int n_blocks, const StepArgs& a, hipStream_t s)
+hipError_t launch_step_st(int bx, int rows, int v, bool single, int n_blocks, const StepArgs& a, hipStream_t s)
 {
 #define LBM_SHAPE(B, RW, VV) \
-    if (bx == B && rows == RW && v == VV) return launch_step_t<ST, B, RW, VV>(n_blocks, a, s);
+    if (bx == B && rows == RW && v == VV) return launch_step_t<ST, B, RW, VV>(single, n_blocks, a, s);
     LBM_SHAPES(LBM_SHAPE)
 #undef LBM_SHAPE
     return hipErrorInvalidValue;
@@ -858,15 +1088,30 @@ int step_shapes(int storage, int (*out)[3], int max)
     return n;
 }
 
-hipError_t launch_step(int storage, int bx, int rows, int v, int n_blocks, const StepArgs& a, hipStream_t s)
+hipError_t launch_step(int storage, int bx, int rows, int v, bool single, int n_blocks, const StepArgs& a,
+                       hipStream_t s)
 {
     switch (storage) {
-    case ST_F64: return launch_step_st<ST_F64>(bx, rows, v, n_blocks, a, s);
-    case ST_F32: return launch_step_st<ST_F32>(bx, rows, v, n_blocks, a, s);
-    case ST_F16S: return launch_step_st<ST_F16S>(bx, rows, v, n_blocks, a, s);
-    case ST_BF16S: return launch_step_st<ST_BF16S>(bx, rows, v, n_blocks, a, s);
+    case ST_F64: return launch_step_st<ST_F64>(bx, rows, v, single, n_blocks, a, s);
+    case ST_F32: return launch_step_st<ST_F32>(bx, rows, v, single, n_blocks, a, s);
+    case ST_F16S: return launch_step_st<ST_F16S>(bx, rows, v, single, n_blocks, a, s);
+    case ST_BF16S: return launch_step_st<ST_BF16S>(bx, rows, v, single, n_blocks, a, s);
     }
     return hipErrorInvalidValue;
+}
+
+int list_blocks(int block_x, int n)
+{
+    const int per_block = block_x / 64 * kListLanes;
+    return (n + per_block - 1) / per_block;
+}
+
+hipError_t launch_flush(const FlushArgs& f, hipStream_t s)
+{
+    const int n = f.n * f.slot_count;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(flush_kernel, dim3((n + 255) / 256), dim3(256), 0, s, f);
+    return hipGetLastError();
 }
 
 hipError_t launch_state(int storage, const StateArgs& a, int nx, hipStream_t s)
