@@ -581,8 +581,13 @@ int edge_nodes(int nx, int ny)
 
 // Grid for one launch over one lattice (single >= 0, or an ensemble of one) or over
 // every lattice; returns whether the one-lattice kernel applies.
-bool grid_for(Ensemble* E, int single, StepArgs& a, int& blocks)
+bool grid_for(Ensemble* E, int single, StepArgs& a, int& blocks, int bx = 0, int rows = 0, int vec = 0)
 {
+    if (bx == 0) {
+        bx = E->bx;
+        rows = E->rows;
+        vec = E->vec;
+    }
     if (single >= 0 || E->lat.size() == 1) {
         const int l = single >= 0 ? single : 0;
         const lbm_desc& d = E->lat[l].d;
@@ -590,13 +595,13 @@ bool grid_for(Ensemble* E, int single, StepArgs& a, int& blocks)
         a.L0 = E->host_lat[l];
         a.lattice0 = l;
         a.tiles = nullptr;
-        a.tiles_x = (d.nx + E->bx * E->vec - 1) / (E->bx * E->vec);
-        a.tiles_y = (d.ny + E->rows - 1) / E->rows;
+        a.tiles_x = (d.nx + bx * vec - 1) / (bx * vec);
+        a.tiles_y = (d.ny + rows - 1) / rows;
         a.n_links = static_cast<int32_t>(H.recs.size());
         a.n_edges = edge_nodes(d.nx, d.ny);
         a.special = E->d_special + H.special_off;
         a.n_special = static_cast<int32_t>(H.special.size());
-        a.special_blocks = list_blocks(E->bx, a.n_links + a.n_edges) + (a.n_special + E->bx - 1) / E->bx;
+        a.special_blocks = list_blocks(bx, a.n_links + a.n_edges) + (a.n_special + bx - 1) / bx;
         blocks = a.special_blocks + a.tiles_x * a.tiles_y;
         return true;
     }
@@ -634,7 +639,33 @@ int ensure_history_room(Ensemble* E, int l, int64_t n)
     return LBM_OK;
 }
 
-// n <= acc_blocks launches over lattice l (l < 0: every lattice), then the force flush
+bool one_lattice(Ensemble* E, int l) { return l >= 0 || E->lat.size() == 1; }
+
+// Two-step launches in a batch of n steps (one-lattice launches with LAUNCH_TWO_STEPS);
+// the batch always ends with a one-step launch, so the last stored C precedes the state.
+int two_step_pairs(Ensemble* E, int n, int l)
+{
+    if (!one_lattice(E, l) || !(E->options & LAUNCH_TWO_STEPS) || n < 2) return 0;
+    return (n - 1) / 2;
+}
+
+// Whether two steps per launch apply: one lattice, tile 64 x (4 or 8) with 256 threads
+// (block_x 64, one node per thread), and the outflow copy's source column (nx-2 and its
+// neighbours) inside the last tile's ring.
+int two_step_ok(Ensemble* E, int bx, int rows, int vec)
+{
+    if (E->lat.size() != 1) return fail(LBM_ERR_INVALID, "two steps per launch need a one-lattice handle");
+    if (bx != 64 || vec != 1 || (rows != 4 && rows != 8))
+        return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1) and (64, 8, 1)");
+    const lbm_desc& d = E->lat[0].d;
+    if (d.outlet && d.nx % 64 == 1) return fail(LBM_ERR_INVALID, "two steps per launch: nx % 64 == 1 with an outlet");
+    return LBM_OK;
+}
+
+// Buffer swaps of a batch: one per launch.
+int batch_flips(Ensemble* E, int n, int l) { return n - two_step_pairs(E, n, l); }
+
+// n <= acc_blocks steps over lattice l (l < 0: every lattice), then the force flush
 // and the counter update.  One-lattice launches take their buffers from the host's
 // parity; graph capture records them for that starting parity.
 int launch_batch(Ensemble* E, int n, int l)
@@ -644,8 +675,39 @@ int launch_batch(Ensemble* E, int n, int l)
     const bool single = grid_for(E, l, a, blocks);
     const int li = l >= 0 ? l : 0;
     const int32_t par = E->lat[li].par;
-    for (int i = 0; i < n; ++i) {
-        a.acc = E->d_acc + static_cast<int64_t>(i) * kAccCopies * E->total_slots * 2;
+    const bool two = single && (E->options & LAUNCH_TWO_STEPS);
+    const int pairs = two_step_pairs(E, n, l);
+    auto block_of = [&](int i) { return E->d_acc + static_cast<int64_t>(i) * kAccCopies * E->total_slots * 2; };
+    int swaps = 0;
+    if (two) {
+        // two-step launches on 64 x rows tiles, then one-step launches of shape (64, 1, 1)
+        const lbm_desc& d = E->lat[li].d;
+        StepArgs a2 = a;
+        a2.tiles_x = (d.nx + 63) / 64;
+        a2.tiles_y = (d.ny + E->rows - 1) / E->rows;
+        const int blocks2 = a2.tiles_x * a2.tiles_y;
+        for (int i = 0; i < pairs; ++i, ++swaps) {
+            const int sp = par ^ (swaps & 1);
+            a2.src = E->buf[sp];
+            a2.dst = E->buf[1 - sp];
+            a2.acc = block_of(2 * i);
+            a2.acc2 = block_of(2 * i + 1);
+            LBM_HIP(launch_step2(E->storage, E->rows, blocks2, a2, E->stream));
+        }
+        StepArgs a1 = base_args(E);
+        int blocks1;
+        grid_for(E, l, a1, blocks1, 64, 1, 1);
+        a1.options = E->options & (LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES);
+        for (int i = 2 * pairs; i < n; ++i, ++swaps) {
+            const int sp = par ^ (swaps & 1);
+            a1.src = E->buf[sp];
+            a1.dst = E->buf[1 - sp];
+            a1.acc = block_of(i);
+            LBM_HIP(launch_step(E->storage, 64, 1, 1, true, blocks1, a1, E->stream));
+        }
+    }
+    for (int i = 0; i < n && !two; ++i, ++swaps) {
+        a.acc = block_of(i);
         if (single) {
             const int sp = par ^ (i & 1);
             a.src = E->buf[sp];
@@ -667,7 +729,7 @@ int launch_batch(Ensemble* E, int n, int l)
     f.R = E->R;
     LBM_HIP(launch_flush(f, E->stream));
     const int begin = l >= 0 ? l : 0, count = l >= 0 ? 1 : static_cast<int>(E->lat.size());
-    LBM_HIP(launch_add_steps(E->d_steps, E->d_par, begin, count, n, E->stream));
+    LBM_HIP(launch_add_steps(E->d_steps, E->d_par, begin, count, n, swaps, E->stream));
     return LBM_OK;
 }
 
@@ -1209,9 +1271,10 @@ int lbm_step(lbm_handle h, int32_t n)
         } else {
             if ((rc = launch_batch(E, batch, -1)) != LBM_OK) return rc;
         }
+        const int flips = batch_flips(E, batch, -1);
         for (auto& H : E->lat) {
             H.steps += batch;
-            H.par ^= batch & 1;
+            H.par ^= flips & 1;
         }
         n -= batch;
     }
@@ -1233,7 +1296,7 @@ int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
         if ((rc = ensure_history_room(E, l, batch)) != LBM_OK) return rc;
         if ((rc = launch_batch(E, batch, l)) != LBM_OK) return rc;
         H.steps += batch;
-        H.par ^= batch & 1;
+        H.par ^= batch_flips(E, batch, l) & 1;
         n -= batch;
     }
     return LBM_OK;
@@ -1247,12 +1310,17 @@ int lbm_set_launch(lbm_handle h, int32_t bx, int32_t rows, int32_t nodes_per_thr
     Ensemble* E = h;
     const int nbx = bx > 0 ? bx : E->bx, nrows = rows > 0 ? rows : E->rows;
     const int nv = nodes_per_thread > 0 ? nodes_per_thread : E->vec;
-    if (!step_shape_supported(E->storage, nbx, nrows, nv))
+    const int nopt = options >= 0 ? options : E->options;
+    if (nopt & ~(LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES | LAUNCH_TWO_STEPS))
+        return fail(LBM_ERR_INVALID, "unknown launch option bits");
+    if (nopt & LAUNCH_TWO_STEPS) {
+        int rc = two_step_ok(E, nbx, nrows, nv);
+        if (rc) return rc;
+    } else if (!step_shape_supported(E->storage, nbx, nrows, nv)) {
         return fail(LBM_ERR_INVALID, "launch shape not compiled for this storage form");
+    }
     if (graph_steps < 0 || (graph_steps > 0 && (graph_steps % 2 != 0 || graph_steps > 2000)))
         return fail(LBM_ERR_INVALID, "graph_steps must be even and at most 2000");
-    const int nopt = options >= 0 ? options : E->options;
-    if (nopt & ~(LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES)) return fail(LBM_ERR_INVALID, "unknown launch option bits");
     if (nbx != E->bx || nrows != E->rows || nv != E->vec || nopt != E->options) {
         E->bx = nbx;
         E->rows = nrows;
@@ -1325,6 +1393,46 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
             bs[1] = E->rows;
             bs[2] = E->vec;
             bs[3] = E->options;
+        }
+    }
+    // two steps per launch (one-lattice handles): per-step time of each tile height
+    for (int si = 0; si < 8 && rc == LBM_OK && E->lat.size() == 1; ++si) {
+        const int rows = 4 << (si / 4), opt = LAUNCH_TWO_STEPS | (si & 3);
+        if (two_step_ok(E, 64, rows, 1) != LBM_OK) continue;
+        E->bx = 64;
+        E->rows = rows;
+        E->vec = 1;
+        E->options = opt;
+        StepArgs a = base_args(E);
+        int blocks;
+        grid_for(E, -1, a, blocks);
+        const lbm_desc& d = E->lat[0].d;
+        a.tiles_x = (d.nx + 63) / 64;
+        a.tiles_y = (d.ny + rows - 1) / rows;
+        blocks = a.tiles_x * a.tiles_y;
+        a.src = E->buf[E->lat[0].par];
+        a.dst = scratch;
+        a.acc = a.acc2 = E->d_dummy_acc;
+        a.nan_flags = E->d_dummy_nan;
+        const int reps = 5;
+        hipError_t le = hipSuccess;
+        for (int w = 0; w < 2 && le == hipSuccess; ++w) le = launch_step2(E->storage, rows, blocks, a, E->stream);
+        (void)hipEventRecord(e0, E->stream);
+        for (int r = 0; r < reps && le == hipSuccess; ++r) le = launch_step2(E->storage, rows, blocks, a, E->stream);
+        (void)hipEventRecord(e1, E->stream);
+        if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(LBM_ERR_DEVICE, "autotune launch failed");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= 2 * reps;
+        if (ms < best) {
+            best = ms;
+            bs[0] = 64;
+            bs[1] = rows;
+            bs[2] = 1;
+            bs[3] = opt;
         }
     }
     (void)hipEventDestroy(e0);

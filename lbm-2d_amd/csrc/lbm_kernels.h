@@ -14,6 +14,7 @@ namespace lbm {
 enum : int32_t {
     LAUNCH_XCD_ORDER = 1,       // consecutive tiles on one XCD (blocks go round-robin over 8)
     LAUNCH_STREAM_STORES = 2,   // nontemporal stores of the new populations
+    LAUNCH_TWO_STEPS = 4,       // one-lattice launches: two steps per launch (tile 64 x rows)
 };
 
 struct StepArgs {
@@ -48,6 +49,7 @@ struct StepArgs {
     const LinkRec* links;
     const uint16_t* wdist;
     double* acc;                // this launch's accumulator block [kAccCopies][total_slots][2]
+    double* acc2;               // two-step launches: the second step's block
     int32_t total_slots;
     int32_t* nan_flags;         // per lattice
     float scale, inv_scale;     // 2^k and 2^-k of the 16-bit forms
@@ -102,12 +104,15 @@ struct PrologueArgs {
 hipError_t launch_step(int storage, int block_x, int rows, int vec, bool single, int n_blocks, const StepArgs& a,
                        hipStream_t s);
 hipError_t launch_flush(const FlushArgs& f, hipStream_t s);
+// Two steps of one lattice per launch: tiles of 64 x tile_rows (4 or 8), 256 threads.
+hipError_t launch_step2(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s);
 // Blocks of block_x threads that take n general-path list nodes.
 int list_blocks(int block_x, int n);
 hipError_t launch_state(int storage, const StateArgs& a, int nx, hipStream_t s);
 hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream_t s);
+// steps += inc, buffer parity ^= flips & 1 for lattices [begin, begin+count)
 hipError_t launch_add_steps(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
-                            hipStream_t s);
+                            int32_t flips, hipStream_t s);
 
 bool step_shape_supported(int storage, int block_x, int rows, int vec);
 // Fills up to max compiled shapes {block_x, rows, nodes per thread} for a storage

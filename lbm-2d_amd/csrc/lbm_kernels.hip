@@ -119,6 +119,8 @@ template <> __device__ __forceinline__ double dexp<double>(double x) { return ex
 template <typename R>
 __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist, R sc)
 {
+    // unfused: the same rounding in every kernel and launch shape that inlines this
+#pragma clang fp contract(off)
     const R a13 = d[1] + d[3], s13 = d[1] - d[3];
     const R a24 = d[2] + d[4], s24 = d[2] - d[4];
     const R a57 = d[5] + d[7], s57 = d[5] - d[7];
@@ -276,6 +278,9 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
                                                const LinkVals<typename Form<ST>::real>& lk,
                                                typename Form<ST>::real d[9])
 {
+    // unfused multiply-adds: the boundary values round the same way in every kernel
+    // that inlines this (one or two steps per launch, any shape)
+#pragma clang fp contract(off)
     using R = typename Form<ST>::real;
     const LatticeDev& L = c.L;
     if (p.fl & CELL_SOLID) {
@@ -356,6 +361,7 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
 // so the closure keeps its form, and w0+w2+w4+2(w3+w6+w7) = 1.
 template <typename R> __device__ __forceinline__ void zou_he(R d[9], R u)
 {
+#pragma clang fp contract(off)
     const R rho = (R(1) + d[0] + d[2] + d[4] + R(2) * (d[3] + d[6] + d[7])) / (R(1) - u);
     d[1] = d[3] + R(2.0 / 3.0) * rho * u;
     d[5] = d[7] + R(1.0 / 6.0) * rho * u;
@@ -493,19 +499,16 @@ __device__ __forceinline__ void links_of(const Ctx<ST>& c, const Pre<ST>& p, Lin
 // lg), momentum exchange on it with the node's own links (ls_, src/lbm.hpp:542-581),
 // collision, store.
 template <int ST>
-__device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
-                                             typename Form<ST>::store* dst, double* accp, int x, int y,
-                                             uint8_t fl, const Pre<ST>& p,
-                                             const LinkVals<typename Form<ST>::real>& lg,
-                                             const LinkVals<typename Form<ST>::real>& ls_, uint16_t wraw,
-                                             ForcePart& part, bool& bad)
+__device__ __forceinline__ void node_general_d(const LatticeDev& L, const Ctx<ST>& c, double* accp, int x, int y,
+                                               uint8_t fl, const Pre<ST>& p,
+                                               const LinkVals<typename Form<ST>::real>& lg,
+                                               const LinkVals<typename Form<ST>::real>& ls_, uint16_t wraw,
+                                               ForcePart& part, bool& bad, typename Form<ST>::real d[9])
 {
     using R = typename Form<ST>::real;
-    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
     const bool solid = (fl & CELL_SOLID) != 0;
-    R d[9];
     state_general<ST>(c, x, y, solid, p, lg, d);
-    if ((fl & CELL_LINKS) && L.forces) {
+    if (accp && (fl & CELL_LINKS) && L.forces) {
 #pragma unroll
         for (int i = 1; i < 9; ++i) {
             if (!(ls_.mask & (1u << i)) || ls_.slot(i) < 0) continue;
@@ -519,7 +522,20 @@ __device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs
         collide<R>(d, L, wd, R(1));
         if (!(d[0] == d[0])) bad = true;
     }
-    auto* o = dst + L.f_off + idx;
+}
+
+template <int ST>
+__device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
+                                             typename Form<ST>::store* dst, double* accp, int x, int y,
+                                             uint8_t fl, const Pre<ST>& p,
+                                             const LinkVals<typename Form<ST>::real>& lg,
+                                             const LinkVals<typename Form<ST>::real>& ls_, uint16_t wraw,
+                                             ForcePart& part, bool& bad)
+{
+    using R = typename Form<ST>::real;
+    R d[9];
+    node_general_d<ST>(L, c, accp, x, y, fl, p, lg, ls_, wraw, part, bad, d);
+    auto* o = dst + L.f_off + static_cast<int64_t>(y) * L.pitch + x;
 #pragma unroll
     for (int j = 0; j < 9; ++j)
         store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale, L.shift[j]), a.options & LAUNCH_STREAM_STORES);
@@ -863,6 +879,182 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
     flush_forces(part, accp);
 }
 
+// ------------------------------------------------------------------ two steps per launch
+
+// Post-collision values of node (x,y) for the next state, read from the source buffer
+// like cell_update; forces counted only when accp is non-null.  false: a solid node
+// nothing reads (no value).
+template <int ST>
+__device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c, double* accp,
+                                            int x, int y, ForcePart& part, bool& bad,
+                                            typename Form<ST>::store out[9])
+{
+    using S = typename Form<ST>::store;
+    using R = typename Form<ST>::real;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    const uint8_t fl = a.flags[L.cell_off + idx];
+    S raw[9];
+    const S* b = c.base + idx;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) raw[j] = b[j * L.plane - (CY[j] * L.pitch + CX[j])];
+    uint16_t wraw = 0xffffu;
+    if (L.les) wraw = a.wdist[L.cell_off + idx];
+    __asm__ volatile("" ::"v"(raw[0]), "v"(raw[1]), "v"(raw[2]), "v"(raw[3]), "v"(raw[4]), "v"(raw[5]),
+                     "v"(raw[6]), "v"(raw[7]), "v"(raw[8]), "v"(wraw));
+    R d[9];
+    if (fl & CELL_SPECIAL) {
+        if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return false;
+        const int gx = gathered_x<ST>(L, x);
+        Pre<ST> p;
+        preload<ST>(c, gx, y, p);
+        LinkVals<R> lg, ls_;
+        links_of<ST>(c, p, lg);
+        if (gx == x) {
+            node_general_d<ST>(L, c, accp, x, y, fl, p, lg, lg, wraw, part, bad, d);
+        } else {
+            if (fl & CELL_LINKS) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
+            node_general_d<ST>(L, c, accp, x, y, fl, p, lg, ls_, wraw, part, bad, d);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(raw[j], a.inv_scale, L.shift[j]);
+        R wd = R(0);
+        if (L.les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
+        collide<R>(d, L, wd, R(1));
+        if (!(d[0] == d[0])) bad = true;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) out[j] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
+    return true;
+}
+
+// Steps k and k+1 of one lattice in one launch.  A block owns a 64 x TY tile: it
+// computes C_k on the tile and a one-node ring around it from C_{k-1} in the source
+// buffer (ring nodes wrap on periodic edges), keeps it in LDS in the stored form, then
+// computes C_{k+1} on the tile from LDS and writes it to the destination.  Every
+// arithmetic step is the one-step kernel's, so results are bit for bit the same; C_k
+// itself is never written, so a batch ends with a one-step launch (downloads re-gather
+// the state from the last stored C).  Forces of step k go to acc, of step k+1 to acc2.
+template <int ST, int TY>
+__global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
+{
+    using S = typename Form<ST>::store;
+    using R = typename Form<ST>::real;
+    constexpr int TX = 64, WX = TX + 2, WY = TY + 2, P2 = TX * TY / 256;
+    __shared__ S lds[9][WY][WX];
+
+    const LatticeDev L = a.L0;
+    const S* src = static_cast<const S*>(a.src);
+    S* dst = static_cast<S*>(a.dst);
+    const int nb = static_cast<int>(gridDim.x);
+    __asm__ volatile("" ::"s"(src), "s"(dst), "s"(a.flags), "s"(a.wdist), "s"(a.acc), "s"(a.acc2),
+                     "s"(a.total_slots), "s"(a.tiles_x), "s"(a.options), "s"(nb), "s"(L.nx), "s"(L.ny),
+                     "s"(L.pitch), "s"(L.f_off), "s"(L.plane), "s"(L.cell_off), "s"(L.les));
+    int b = static_cast<int>(blockIdx.x);
+    if (a.options & LAUNCH_XCD_ORDER) {
+        const int q = nb >> 3, r = nb & 7, xcd = b & 7;
+        b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+    }
+    const int x0 = (b % a.tiles_x) * TX, y0 = (b / a.tiles_x) * TY;
+    const int copy = static_cast<int>(blockIdx.x % kAccCopies);
+    double* acc0 = a.acc + static_cast<int64_t>(copy) * a.total_slots * 2;
+    double* acc1 = a.acc2 + static_cast<int64_t>(copy) * a.total_slots * 2;
+    const Ctx<ST> c{L, src + L.f_off, a.flags, a.link_slot, a.links, a.inv_scale};
+    const int tid = static_cast<int>(threadIdx.x);
+
+    // the second step's per-node data, fetched with the first step's loads
+    uint8_t fl2[P2];
+    int32_t ls2[P2];
+    uint16_t wd2[P2];
+#pragma unroll
+    for (int k = 0; k < P2; ++k) {
+        const int n = tid + 256 * k, x = x0 + n % TX, y = y0 + n / TX;
+        fl2[k] = 0;
+        ls2[k] = 0;
+        wd2[k] = 0xffffu;
+        if (x < L.nx && y < L.ny) {
+            const int64_t idx = L.cell_off + static_cast<int64_t>(y) * L.pitch + x;
+            fl2[k] = a.flags[idx];
+            ls2[k] = a.link_slot[idx];
+            if (L.les) wd2[k] = a.wdist[idx];
+        }
+    }
+
+    ForcePart part0, part1;
+    bool bad = false;
+    // step k on the tile and its ring
+    for (int r = tid; r < WX * WY; r += 256) {
+        const int lx = r % WX, ly = r / WX;
+        const int ux = x0 - 1 + lx, uy = y0 - 1 + ly;  // unwrapped position
+        int gx = ux, gy = uy;
+        if (gx < 0 || gx >= L.nx) {
+            if (L.x_edge != EDGE_PERIODIC) continue;
+            gx = gx < 0 ? gx + L.nx : gx - L.nx;
+        }
+        if (gy < 0 || gy >= L.ny) {
+            if (L.y_edge != EDGE_PERIODIC) continue;
+            gy = gy < 0 ? gy + L.ny : gy - L.ny;
+        }
+        const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny;
+        S out[9];
+        if (node_values<ST>(L, a, c, own ? acc0 : nullptr, gx, gy, part0, bad, out)) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
+        }
+    }
+    __syncthreads();
+
+    // step k+1 on the tile, from LDS
+#pragma unroll
+    for (int k = 0; k < P2; ++k) {
+        const int n = tid + 256 * k, tx = n % TX, ty = n / TX, x = x0 + tx, y = y0 + ty;
+        if (x >= L.nx || y >= L.ny) continue;
+        const uint8_t fl = fl2[k];
+        const int lx = tx + 1, ly = ty + 1;
+        R d[9];
+        if (fl & CELL_SPECIAL) {
+            if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) continue;
+            const int gx = gathered_x<ST>(L, x), glx = gx - x0 + 1;
+            Pre<ST> p;
+            if (gx == x) {
+                p.fl = fl;
+                p.ls = ls2[k];
+            } else {
+                const int64_t gidx = L.cell_off + static_cast<int64_t>(y) * L.pitch + gx;
+                p.fl = a.flags[gidx];
+                p.ls = a.link_slot[gidx];
+            }
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                p.own[j] = lds[j][ly][glx];
+                p.pull[j] = lds[j][ly - CY[j]][glx - CX[j]];
+            }
+            LinkVals<R> lg, ls_;
+            links_of<ST>(c, p, lg);
+            if (gx == x) {
+                node_general_d<ST>(L, c, acc1, x, y, fl, p, lg, lg, wd2[k], part1, bad, d);
+            } else {
+                if (fl & CELL_LINKS) load_link(c.links[L.link_off + ls2[k]], ls_);
+                node_general_d<ST>(L, c, acc1, x, y, fl, p, lg, ls_, wd2[k], part1, bad, d);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(lds[j][ly - CY[j]][lx - CX[j]], a.inv_scale, L.shift[j]);
+            R wd = R(0);
+            if (L.les) wd = wd2[k] == 0xffffu ? R(1e9) : static_cast<R>(wd2[k]);
+            collide<R>(d, L, wd, R(1));
+            if (!(d[0] == d[0])) bad = true;
+        }
+        S* o = dst + L.f_off + static_cast<int64_t>(y) * L.pitch + x;
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+            store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale, L.shift[j]), a.options & LAUNCH_STREAM_STORES);
+    }
+    if (bad) atomicOr(a.nan_flags + a.lattice0, 1);
+    flush_forces(part0, acc0);
+    flush_forces(part1, acc1);
+}
+
 // ------------------------------------------------------------------ state / prologue
 
 template <int ST>
@@ -984,14 +1176,15 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
     for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
 }
 
-// Advances the per-lattice step counters after a batch of launches; an odd batch
-// leaves the state in the other buffer.
-__global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc)
+// Advances the per-lattice step counters after a batch of launches; an odd number of
+// launches (a two-step launch swaps the buffers once) leaves the state in the other buffer.
+__global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
+                                 int32_t flips)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         steps[begin + i] += inc;
-        par[begin + i] ^= static_cast<int32_t>(inc & 1);
+        par[begin + i] ^= flips & 1;
     }
 }
 
@@ -1106,6 +1299,30 @@ int list_blocks(int block_x, int n)
     return (n + per_block - 1) / per_block;
 }
 
+template <int ST>
+hipError_t launch_step2_st(int ty, int n_blocks, const StepArgs& a, hipStream_t s)
+{
+    if (ty == 4) {
+        hipLaunchKernelGGL((step2_kernel<ST, 4>), dim3(n_blocks), dim3(256), 0, s, a);
+    } else if (ty == 8) {
+        hipLaunchKernelGGL((step2_kernel<ST, 8>), dim3(n_blocks), dim3(256), 0, s, a);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_step2(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s)
+{
+    switch (storage) {
+    case ST_F64: return launch_step2_st<ST_F64>(tile_rows, n_blocks, a, s);
+    case ST_F32: return launch_step2_st<ST_F32>(tile_rows, n_blocks, a, s);
+    case ST_F16S: return launch_step2_st<ST_F16S>(tile_rows, n_blocks, a, s);
+    case ST_BF16S: return launch_step2_st<ST_BF16S>(tile_rows, n_blocks, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_flush(const FlushArgs& f, hipStream_t s)
 {
     const int n = f.n * f.slot_count;
@@ -1141,11 +1358,11 @@ hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream
 }
 
 hipError_t launch_add_steps(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
-                            hipStream_t s)
+                            int32_t flips, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(add_steps_kernel, dim3((count + 255) / 256), dim3(256), 0, s, steps, par, begin, count,
-                       inc);
+                       inc, flips);
     return hipGetLastError();
 }
 

@@ -268,3 +268,29 @@ def test_storage_shift_round_trip_and_errors():
         o.step(c.tau, c.u)
     ens.step(c.steps)
     assert np.abs(ens.download(0) - o.f).max() <= TOL["f16s"]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "bf16s"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_two_steps_per_launch_is_bitwise_identical(name, storage):
+    """Two steps per launch (tile and ring in LDS) against one step per launch, on every
+    case's boundaries: the same populations bit for bit after odd and even step counts
+    (graph and single launches), forces to FP64 rounding."""
+    c = CASES[name]
+    a, _ = build_gpu(c, storage)
+    b, _ = build_gpu(c, storage)
+    a.set_launch(64, 1, 1, 0)
+    for rows, opt in ((4, 4), (8, 5), (8, 6), (4, 7)):
+        try:
+            b.set_launch(64, rows, 1, opt, graph_steps=10)
+        except G.LbmError:
+            pytest.skip("two steps per launch do not apply to this lattice")
+        for n in (1, 5, 12):
+            a.step(n)
+            b.step(n)
+            np.testing.assert_array_equal(b.download(0), a.download(0), err_msg=f"rows {rows} options {opt} n {n}")
+    fa, fya = a.forces(0)
+    fb, fyb = b.forces(0)
+    np.testing.assert_allclose(fb, fa, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(fyb, fya, rtol=1e-12, atol=1e-13)
+    assert a.steps_done(0) == b.steps_done(0)
