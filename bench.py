@@ -404,7 +404,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="", help="run one extra form only and print it (profiling): "
-                                                "f32_6400x2400_les, f16s_6400x2400_les, bf16s_6400x2400_les, "
+                                                "f32_6400x2400_les, f16s_6400x2400_les, bf16s_6400x2400_les "
+                                                "(6400x2400 forms take --launch), "
                                                 "batched_256, large_32768x16384")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (HBM counter calibration) and exit")
@@ -470,6 +471,11 @@ def main():
                                                                torch, stream.cuda_stream, les=True)
                 print(f"# {storage} 6400x2400 done", file=sys.stderr, flush=True)
             forms["fp16s_over_fp32"] = forms["f16s_6400x2400_les"]["GLUPS"] / forms["f32_6400x2400_les"]["GLUPS"]
+            # FP32 restricted to one step per launch (the autotuner may pick two)
+            forms["f32_6400x2400_les_one_step"] = bench_form(6400, 2400, 240, 0.644, "f32", big_steps, 20, torch,
+                                                             stream.cuda_stream, les=True, launch="256,1,1,0")
+            forms["fp16s_over_fp32_one_step"] = (forms["f16s_6400x2400_les"]["GLUPS"] /
+                                                 forms["f32_6400x2400_les_one_step"]["GLUPS"])
             forms["batched_256"] = bench_ensemble(1000, 100, torch, stream.cuda_stream)
             print("# batched ensemble done", file=sys.stderr, flush=True)
             if args.large:

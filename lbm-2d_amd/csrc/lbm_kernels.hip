@@ -624,14 +624,11 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
         const int gx = gathered_x<ST>(L, x);
         Pre<ST> p;
         preload<ST>(c, gx, y, p);
-        LinkVals<R> lg, ls_;
+        LinkVals<R> lg;
         links_of<ST>(c, p, lg);
-        if (gx == x) {
-            node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, lg, wraw, part, bad);
-        } else {
-            if (fl & CELL_LINKS) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
-            node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, ls_, wraw, part, bad);
-        }
+        LinkVals<R> ls_ = lg;
+        if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
+        node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, ls_, wraw, part, bad);
         return;
     }
     R d[9];
@@ -907,14 +904,11 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
         const int gx = gathered_x<ST>(L, x);
         Pre<ST> p;
         preload<ST>(c, gx, y, p);
-        LinkVals<R> lg, ls_;
+        LinkVals<R> lg;
         links_of<ST>(c, p, lg);
-        if (gx == x) {
-            node_general_d<ST>(L, c, accp, x, y, fl, p, lg, lg, wraw, part, bad, d);
-        } else {
-            if (fl & CELL_LINKS) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
-            node_general_d<ST>(L, c, accp, x, y, fl, p, lg, ls_, wraw, part, bad, d);
-        }
+        LinkVals<R> ls_ = lg;
+        if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
+        node_general_d<ST>(L, c, accp, x, y, fl, p, lg, ls_, wraw, part, bad, d);
     } else {
 #pragma unroll
         for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(raw[j], a.inv_scale, L.shift[j]);
@@ -1004,12 +998,21 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
     }
     __syncthreads();
 
-    // step k+1 on the tile, from LDS
-#pragma unroll
+    // step k+1 on the tile, from LDS (one copy of the loop body: the general path is long)
+#pragma unroll 1
     for (int k = 0; k < P2; ++k) {
         const int n = tid + 256 * k, tx = n % TX, ty = n / TX, x = x0 + tx, y = y0 + ty;
         if (x >= L.nx || y >= L.ny) continue;
-        const uint8_t fl = fl2[k];
+        uint8_t fl = fl2[0];
+        int32_t lsk = ls2[0];
+        uint16_t wdk = wd2[0];
+#pragma unroll
+        for (int q = 1; q < P2; ++q)
+            if (k == q) {
+                fl = fl2[q];
+                lsk = ls2[q];
+                wdk = wd2[q];
+            }
         const int lx = tx + 1, ly = ty + 1;
         R d[9];
         if (fl & CELL_SPECIAL) {
@@ -1018,7 +1021,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
             Pre<ST> p;
             if (gx == x) {
                 p.fl = fl;
-                p.ls = ls2[k];
+                p.ls = lsk;
             } else {
                 const int64_t gidx = L.cell_off + static_cast<int64_t>(y) * L.pitch + gx;
                 p.fl = a.flags[gidx];
@@ -1029,19 +1032,16 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
                 p.own[j] = lds[j][ly][glx];
                 p.pull[j] = lds[j][ly - CY[j]][glx - CX[j]];
             }
-            LinkVals<R> lg, ls_;
+            LinkVals<R> lg;
             links_of<ST>(c, p, lg);
-            if (gx == x) {
-                node_general_d<ST>(L, c, acc1, x, y, fl, p, lg, lg, wd2[k], part1, bad, d);
-            } else {
-                if (fl & CELL_LINKS) load_link(c.links[L.link_off + ls2[k]], ls_);
-                node_general_d<ST>(L, c, acc1, x, y, fl, p, lg, ls_, wd2[k], part1, bad, d);
-            }
+            LinkVals<R> ls_ = lg;
+            if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + lsk], ls_);
+            node_general_d<ST>(L, c, acc1, x, y, fl, p, lg, ls_, wdk, part1, bad, d);
         } else {
 #pragma unroll
             for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(lds[j][ly - CY[j]][lx - CX[j]], a.inv_scale, L.shift[j]);
             R wd = R(0);
-            if (L.les) wd = wd2[k] == 0xffffu ? R(1e9) : static_cast<R>(wd2[k]);
+            if (L.les) wd = wdk == 0xffffu ? R(1e9) : static_cast<R>(wdk);
             collide<R>(d, L, wd, R(1));
             if (!(d[0] == d[0])) bad = true;
         }
