@@ -88,9 +88,11 @@ def graph_step_ms(ens, torch, replays=20, graph_steps=100):
     return float(np.mean([evs[i].elapsed_time(evs[i + 1]) for i in range(replays)])) / graph_steps
 
 
-def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1, launch=""):
+def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1, launch="", sr=False):
     ens, mask = cylinder_lattice(nx, ny, nx // 4, ny // 2, r, tau, 0.1, storage, stream,
                                  les=les, cs=cs, perturb=False)
+    if sr:
+        ens.set_stochastic_rounding(True, 1)
     if launch:
         shape = tuple(int(v) for v in launch.split(","))
         ens.set_launch(*shape)
@@ -102,7 +104,7 @@ def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False,
     b = algorithmic_bytes(info, 2)
     m0 = ens.mass(0)
     out = {
-        "lattice": f"{nx}x{ny}", "storage": storage, "les": les,
+        "lattice": f"{nx}x{ny}", "storage": storage, "les": les, "rounding": "stochastic" if sr else "nearest-even",
         "GLUPS": info.cells * steps / t / 1e9,
         "ms_per_step": 1e3 * t / steps,
         "kernel_ms": kms,
@@ -404,7 +406,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="", help="run one extra form only and print it (profiling): "
-                                                "f32_6400x2400_les, f16s_6400x2400_les, bf16s_6400x2400_les "
+                                                "f32_6400x2400_les, f16s_6400x2400_les, f16s_sr_6400x2400_les, "
+                                                "bf16s_6400x2400_les "
                                                 "(6400x2400 forms take --launch), "
                                                 "batched_256, large_32768x16384")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -430,7 +433,7 @@ def main():
             else:
                 storage = args.form.split("_")[0]
                 out = bench_form(6400, 2400, 240, 0.644, storage, 100, 10, torch, stream.cuda_stream, les=True,
-                                 launch=args.launch)
+                                 launch=args.launch, sr="_sr_" in args.form)
         print(json.dumps({args.form: out}))
         return
     if args.gpus != 1:
@@ -470,6 +473,8 @@ def main():
                 forms[f"{storage}_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, storage, big_steps, 20,
                                                                torch, stream.cuda_stream, les=True)
                 print(f"# {storage} 6400x2400 done", file=sys.stderr, flush=True)
+            forms["f16s_sr_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "f16s", big_steps, 20, torch,
+                                                        stream.cuda_stream, les=True, sr=True)
             forms["fp16s_over_fp32"] = forms["f16s_6400x2400_les"]["GLUPS"] / forms["f32_6400x2400_les"]["GLUPS"]
             # FP32 restricted to one step per launch (the autotuner may pick two)
             forms["f32_6400x2400_les_one_step"] = bench_form(6400, 2400, 240, 0.644, "f32", big_steps, 20, torch,

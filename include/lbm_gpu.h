@@ -152,6 +152,12 @@ int lbm_init_equilibrium(lbm_handle h, int32_t lattice, double rho, double u, do
  * the flow's own variations there.  No counterpart in the reference, whose state is
  * FP64.  Invalidates the stored state: upload or initialise afterwards. */
 int lbm_set_storage_shift(lbm_handle h, int32_t lattice, double u, double v);
+/* 16-bit storage forms only: round each stored population stochastically (add uniform
+ * random bits below the last kept bit, then truncate) instead of to nearest even.  The
+ * bits come from a hash of the seed, the lattice's step and the node, so runs repeat
+ * exactly; rounding errors have zero mean, which removes the drift that identical
+ * nodes rounding identically cause.  No counterpart in the reference (FP64 state). */
+int lbm_set_stochastic_rounding(lbm_handle h, int32_t on, uint32_t seed);
 /* Density and velocity of rows [y0, y1) of the current state, row-major, nx*(y1-y0)
  * FP64 each (any output may be null): compute_macros (src/lbm_types.hpp:278-288) on
  * what lbm_download would return, computed on the device -- the fields the frame

@@ -161,6 +161,8 @@ struct Ensemble {
     int total_slots = 0;
     int R = kHistoryRows;
     int bx = 256, rows = 1, vec = 1, options = 0, graph_steps = 100;
+    int sr = 0;                    // stochastic rounding (16-bit forms)
+    uint32_t sr_seed = 0;
     bool geom_dirty = true, desc_dirty = true, tiles_dirty = true, counters_stale = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // by the starting buffer parity (one lattice)
     bool graph_dirty = true;
@@ -570,6 +572,9 @@ StepArgs base_args(Ensemble* E)
     a.scale = E->scale;
     a.inv_scale = E->inv_scale;
     a.options = E->options;
+    a.steps = E->d_steps;
+    a.sr = E->sr;
+    a.sr_seed = E->sr_seed;
     return a;
 }
 
@@ -692,6 +697,7 @@ int launch_batch(Ensemble* E, int n, int l)
             a2.dst = E->buf[1 - sp];
             a2.acc = block_of(2 * i);
             a2.acc2 = block_of(2 * i + 1);
+            a2.step_arg = 2 * i;
             LBM_HIP(launch_step2(E->storage, E->rows, blocks2, a2, E->stream));
         }
         StepArgs a1 = base_args(E);
@@ -703,17 +709,17 @@ int launch_batch(Ensemble* E, int n, int l)
             a1.src = E->buf[sp];
             a1.dst = E->buf[1 - sp];
             a1.acc = block_of(i);
+            a1.step_arg = i;
             LBM_HIP(launch_step(E->storage, 64, 1, 1, true, blocks1, a1, E->stream));
         }
     }
     for (int i = 0; i < n && !two; ++i, ++swaps) {
         a.acc = block_of(i);
+        a.step_arg = i;
         if (single) {
             const int sp = par ^ (i & 1);
             a.src = E->buf[sp];
             a.dst = E->buf[1 - sp];
-        } else {
-            a.step_arg = i;
         }
         LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream));
     }
@@ -1049,6 +1055,17 @@ int lbm_set_storage_shift(lbm_handle h, int32_t l, double u, double v)
     H.shift_v = v;
     h->desc_dirty = true;
     H.uploaded = false;  // the stored state is encoded against the old shift
+    return LBM_OK;
+}
+
+int lbm_set_stochastic_rounding(lbm_handle h, int32_t on, uint32_t seed)
+{
+    if (!h) return fail(LBM_ERR_INVALID, "null handle");
+    if (h->storage != lbm::ST_F16S && h->storage != lbm::ST_BF16S)
+        return fail(LBM_ERR_INVALID, "stochastic rounding applies to the 16-bit storage forms");
+    h->sr = on ? 1 : 0;
+    h->sr_seed = seed;
+    h->graph_dirty = true;
     return LBM_OK;
 }
 

@@ -50,6 +50,9 @@ struct StepArgs {
     const uint16_t* wdist;
     double* acc;                // this launch's accumulator block [kAccCopies][total_slots][2]
     double* acc2;               // two-step launches: the second step's block
+    const int64_t* steps;       // per-lattice step count at the start of the batch
+    int32_t sr;                 // 16-bit forms: stochastic rounding of stored values
+    uint32_t sr_seed;
     int32_t total_slots;
     int32_t* nan_flags;         // per lattice
     float scale, inv_scale;     // 2^k and 2^-k of the 16-bit forms

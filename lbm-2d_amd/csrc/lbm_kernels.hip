@@ -70,6 +70,14 @@ template <> struct Form<ST_F16S> {
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(h)); }
     static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
     static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
+    // Stochastic rounding: add the top 13 bits of rnd below the last kept mantissa bit,
+    // then truncate; exact in the binary16 normal range (rounded to nearest below it).
+    static __device__ __forceinline__ store enc_sr_raw(real h, uint32_t rnd)
+    {
+        uint32_t b = __float_as_uint(h);
+        if ((b & 0x7f800000u) != 0x7f800000u) b += rnd >> 19;
+        return enc_raw(__uint_as_float(b & ~0x1fffu));
+    }
 };
 template <> struct Form<ST_BF16S> {
     using store = uint16_t;
@@ -80,7 +88,68 @@ template <> struct Form<ST_BF16S> {
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(h)); }
     static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
     static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
+    // Stochastic rounding: add the top 16 bits of rnd below the last kept bit, truncate.
+    static __device__ __forceinline__ store enc_sr_raw(real h, uint32_t rnd)
+    {
+        uint32_t b = __float_as_uint(h);
+        if ((b & 0x7f800000u) != 0x7f800000u) b += rnd >> 16;
+        return static_cast<uint16_t>(b >> 16);
+    }
 };
+
+// Random bits for stochastic rounding: a per-launch key (seed and step) mixed once with
+// the node's global index (lowbias32); each further word is the previous one times an
+// odd constant, and a population takes one 16-bit half of a word.
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+struct Rounding {
+    bool on;        // stochastic rounding of the 16-bit forms
+    uint32_t key;   // seed and step
+    __device__ __forceinline__ uint32_t node(int64_t cell) const
+    {
+        return mix32(key ^ (static_cast<uint32_t>(cell) * 0x9e3779b1u));
+    }
+};
+
+// Random bits for the next population in their top 16 bits: the high and low halves of
+// a word in turn, a new word every second call.  h carries the word and a call count
+// in its caller's loop index, so the sequence is fixed per node.
+__device__ __forceinline__ uint32_t next_rand(uint32_t& h, int j)
+{
+    if (j & 1) return h << 16;
+    if (j) h *= 0x85ebca6bu;
+    return h & 0xffff0000u;
+}
+
+// Encode deviation d for storage, rounding to nearest even or stochastically.
+template <int ST>
+__device__ __forceinline__ typename Form<ST>::store enc_node(typename Form<ST>::real d, float scale, float sh,
+                                                             const Rounding& rd, uint32_t& h, int j)
+{
+    if constexpr (Form<ST>::shifted) {
+        if (rd.on) return Form<ST>::enc_sr_raw((d - sh) * scale, next_rand(h, j));
+    }
+    return Form<ST>::enc(d, scale, sh);
+}
+
+// The same for a value already in stored units (less the shift).
+template <int ST>
+__device__ __forceinline__ typename Form<ST>::store enc_node_raw(typename Form<ST>::real h_, const Rounding& rd,
+                                                                 uint32_t& h, int j)
+{
+    if constexpr (Form<ST>::shifted) {
+        if (rd.on) return Form<ST>::enc_sr_raw(h_, next_rand(h, j));
+    }
+    return Form<ST>::enc_raw(h_);
+}
 
 template <typename R> __device__ __forceinline__ R wgt(int i) { return static_cast<R>(WD[i]); }
 
@@ -530,15 +599,17 @@ __device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs
                                              uint8_t fl, const Pre<ST>& p,
                                              const LinkVals<typename Form<ST>::real>& lg,
                                              const LinkVals<typename Form<ST>::real>& ls_, uint16_t wraw,
-                                             ForcePart& part, bool& bad)
+                                             const Rounding& rd, ForcePart& part, bool& bad)
 {
     using R = typename Form<ST>::real;
     R d[9];
     node_general_d<ST>(L, c, accp, x, y, fl, p, lg, ls_, wraw, part, bad, d);
-    auto* o = dst + L.f_off + static_cast<int64_t>(y) * L.pitch + x;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    auto* o = dst + L.f_off + idx;
+    uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
 #pragma unroll
     for (int j = 0; j < 9; ++j)
-        store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale, L.shift[j]), a.options & LAUNCH_STREAM_STORES);
+        store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j), a.options & LAUNCH_STREAM_STORES);
 }
 
 // Lanes of a wave that take general-path list nodes.
@@ -568,8 +639,8 @@ __device__ __forceinline__ void edge_position(const LatticeDev& L, int e, int& x
 // of the general path in the kernel.
 template <int ST>
 __device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
-                                          typename Form<ST>::store* dst, double* accp, int t, ForcePart& part,
-                                          bool& bad)
+                                          typename Form<ST>::store* dst, double* accp, int t, const Rounding& rd,
+                                          ForcePart& part, bool& bad)
 {
     using R = typename Form<ST>::real;
     const bool is_link = t < a.n_links;
@@ -594,14 +665,15 @@ __device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a
     if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return;
     LinkVals<R> lg = own_links;
     if (!is_link || gx != x) links_of<ST>(c, p, lg);
-    node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, own_links, wraw, part, bad);
+    node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, own_links, wraw, rd, part, bad);
 }
 
 // One node, bulk or general.  SKIP_SPECIAL: general nodes are left to the lists.
 template <int ST, bool SKIP_SPECIAL>
 __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs& a,
                                             const Ctx<ST>& c, typename Form<ST>::store* dst,
-                                            double* accp, int x, int y, ForcePart& part, bool& bad)
+                                            double* accp, int x, int y, const Rounding& rd, ForcePart& part,
+                                            bool& bad)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
@@ -628,7 +700,7 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
         links_of<ST>(c, p, lg);
         LinkVals<R> ls_ = lg;
         if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
-        node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, ls_, wraw, part, bad);
+        node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, ls_, wraw, rd, part, bad);
         return;
     }
     R d[9];
@@ -639,9 +711,10 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     collide<R>(d, L, wd, R(1));
     if (!(d[0] == d[0])) bad = true;
     S* o = dst + L.f_off + idx;
+    uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
 #pragma unroll
     for (int j = 0; j < 9; ++j)
-        store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale, L.shift[j]), a.options & LAUNCH_STREAM_STORES);
+        store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j), a.options & LAUNCH_STREAM_STORES);
 }
 
 // Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane i-1 /
@@ -684,7 +757,7 @@ template <typename S, int V> struct alignas(sizeof(S) * V) Pack {
 template <int ST, int V>
 __device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
                                              typename Form<ST>::store* dst, int x, int y, bool bulk,
-                                             bool& bad)
+                                             const Rounding& rd, bool& bad)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
@@ -744,15 +817,24 @@ __device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs
         if (L.les) wd = wd4.v[k] == 0xffffu ? R(1e9) : static_cast<R>(wd4.v[k]);
         collide<R>(d, L, wd, static_cast<R>(a.scale));  // in stored units
         if (!(d[0] == d[0])) bad = true;
+        uint32_t h = rd.on ? rd.node(L.cell_off + idx + k) : 0u;
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
             if constexpr (Form<ST>::shifted) d[j] -= L.shift_s[j];
-            out[j].v[k] = Form<ST>::enc_raw(d[j]);
+            out[j].v[k] = enc_node_raw<ST>(d[j], rd, h, j);
         }
     }
 #pragma unroll
     for (int j = 0; j < 9; ++j)
         store_val(reinterpret_cast<P*>(dst + L.f_off + j * L.plane + idx), out[j], a.options & LAUNCH_STREAM_STORES);
+}
+
+// Stochastic-rounding key of lattice l's step (its step count at the start of the batch
+// + the launch's index in it + extra): the same in every launch shape and path.
+__device__ __forceinline__ uint32_t step_key(const StepArgs& a, int l, int extra)
+{
+    const int64_t k = a.steps[l] + a.step_arg + extra;
+    return mix32(a.sr_seed ^ mix32(static_cast<uint32_t>(k) ^ mix32(static_cast<uint32_t>(k >> 32) + 0x9e3779b9u)));
 }
 
 // One step of one lattice (SINGLE: descriptor, buffers and tile grid in the kernel
@@ -784,6 +866,10 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
                          "s"(a.tiles_x), "s"(a.options), "s"(nb), "s"(L.nx), "s"(L.ny), "s"(L.pitch),
                          "s"(L.f_off), "s"(L.plane), "s"(L.cell_off), "s"(L.les), "s"(a.special),
                          "s"(a.n_special), "s"(a.special_blocks), "s"(a.n_links), "s"(a.n_edges));
+    }
+    Rounding rd{a.sr != 0, 0u};
+    if constexpr (SINGLE) {
+        if (rd.on) rd.key = step_key(a, a.lattice0, 0);
         // The first blocks take the general-path nodes -- boundary-link records, edge
         // nodes, written solids: dispatched first, their longer load chains overlap the
         // bulk tiles, which skip those nodes.
@@ -799,7 +885,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
             if (b < gb) {
                 const int lane = static_cast<int>(threadIdx.x) & 63;
                 const int t = b * per_block + (static_cast<int>(threadIdx.x) >> 6) * kListLanes + lane;
-                if (lane < kListLanes && t < a.n_links + a.n_edges) list_node<ST>(L, a, c, dst, accp, t, part, bad);
+                if (lane < kListLanes && t < a.n_links + a.n_edges) list_node<ST>(L, a, c, dst, accp, t, rd, part, bad);
             } else {
                 // written solids (Bouzidi sources): zero populations, f - w = -w
                 const int t = (b - gb) * BX + static_cast<int>(threadIdx.x);
@@ -832,6 +918,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
         y0 = t.y0;
         x0 = t.x0;
         L = a.lat[lat_id];
+        if (rd.on) rd.key = step_key(a, lat_id, 0);
         const int sp = a.par[lat_id] ^ (a.step_arg & 1);
         src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
         dst = static_cast<S*>(a.dst ? a.dst : (sp ? a.buf0 : a.buf1));
@@ -861,7 +948,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
                     bulk = !(any & CELL_SPECIAL);
                 }
                 // every lane inside the padded row takes part in the lane shifts
-                if (x < L.pitch) group_update<ST, V>(L, a, c, dst, x, y, bulk, bad);
+                if (x < L.pitch) group_update<ST, V>(L, a, c, dst, x, y, bulk, rd, bad);
                 done = bulk;
             }
         }
@@ -869,7 +956,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
             // one copy of the general path, looped (keeps the kernel small)
 #pragma unroll 1
             for (int q = 0; q < V; ++q)
-                if (x + q < L.nx) cell_update<ST, SINGLE>(L, a, c, dst, accp, x + q, y, part, bad);
+                if (x + q < L.nx) cell_update<ST, SINGLE>(L, a, c, dst, accp, x + q, y, rd, part, bad);
         }
     }
     if (bad) atomicOr(a.nan_flags + lat_id, 1);
@@ -883,7 +970,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
 // nothing reads (no value).
 template <int ST>
 __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c, double* accp,
-                                            int x, int y, ForcePart& part, bool& bad,
+                                            int x, int y, const Rounding& rd, ForcePart& part, bool& bad,
                                             typename Form<ST>::store out[9])
 {
     using S = typename Form<ST>::store;
@@ -917,8 +1004,9 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
         collide<R>(d, L, wd, R(1));
         if (!(d[0] == d[0])) bad = true;
     }
+    uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) out[j] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
+    for (int j = 0; j < 9; ++j) out[j] = enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j);
     return true;
 }
 
@@ -976,6 +1064,11 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
 
     ForcePart part0, part1;
     bool bad = false;
+    Rounding rd0{a.sr != 0, 0u}, rd1{a.sr != 0, 0u};
+    if (rd0.on) {
+        rd0.key = step_key(a, a.lattice0, 0);
+        rd1.key = step_key(a, a.lattice0, 1);
+    }
     // step k on the tile and its ring
     for (int r = tid; r < WX * WY; r += 256) {
         const int lx = r % WX, ly = r / WX;
@@ -991,7 +1084,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
         }
         const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny;
         S out[9];
-        if (node_values<ST>(L, a, c, own ? acc0 : nullptr, gx, gy, part0, bad, out)) {
+        if (node_values<ST>(L, a, c, own ? acc0 : nullptr, gx, gy, rd0, part0, bad, out)) {
 #pragma unroll
             for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
         }
@@ -1045,10 +1138,12 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
             collide<R>(d, L, wd, R(1));
             if (!(d[0] == d[0])) bad = true;
         }
-        S* o = dst + L.f_off + static_cast<int64_t>(y) * L.pitch + x;
+        const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+        S* o = dst + L.f_off + idx;
+        uint32_t h = rd1.on ? rd1.node(L.cell_off + idx) : 0u;
 #pragma unroll
         for (int j = 0; j < 9; ++j)
-            store_val(o + j * L.plane, Form<ST>::enc(d[j], a.scale, L.shift[j]), a.options & LAUNCH_STREAM_STORES);
+            store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j), a.options & LAUNCH_STREAM_STORES);
     }
     if (bad) atomicOr(a.nan_flags + a.lattice0, 1);
     flush_forces(part0, acc0);

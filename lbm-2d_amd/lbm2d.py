@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
         "lbm_download": (C.c_int, [_h, i32, _dp]),
         "lbm_init_equilibrium": (C.c_int, [_h, i32, dbl, dbl, dbl]),
         "lbm_set_storage_shift": (C.c_int, [_h, i32, dbl, dbl]),
+        "lbm_set_stochastic_rounding": (C.c_int, [_h, i32, C.c_uint32]),
         "lbm_macros": (C.c_int, [_h, i32, i32, i32, _dp, _dp, _dp]),
         "lbm_step": (C.c_int, [_h, i32]),
         "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
@@ -214,6 +215,10 @@ class Ensemble:
     def set_storage_shift(self, l: int, u: float, v: float = 0.0):
         """16-bit forms: store f - feq(1, u, v) rather than f - w (call before upload)."""
         _check(lib().lbm_set_storage_shift(self._h, l, u, v))
+
+    def set_stochastic_rounding(self, on: bool = True, seed: int = 0):
+        """16-bit forms: stochastic instead of round-to-nearest-even storage."""
+        _check(lib().lbm_set_stochastic_rounding(self._h, int(on), seed & 0xFFFFFFFF))
 
     def init_equilibrium(self, l: int, rho: float = 1.0, u: float = 0.0, v: float = 0.0):
         _check(lib().lbm_init_equilibrium(self._h, l, rho, u, v))

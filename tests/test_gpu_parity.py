@@ -294,3 +294,32 @@ def test_two_steps_per_launch_is_bitwise_identical(name, storage):
     np.testing.assert_allclose(fb, fa, rtol=1e-12, atol=1e-13)
     np.testing.assert_allclose(fyb, fya, rtol=1e-12, atol=1e-13)
     assert a.steps_done(0) == b.steps_done(0)
+
+
+@pytest.mark.parametrize("storage", ["f16s", "bf16s"])
+def test_stochastic_rounding_repeats_in_every_launch_form(storage):
+    """With stochastic rounding the bits come from (seed, step, node): launch shapes,
+    two steps per launch, graphs and single launches all give the same populations bit
+    for bit; a different seed gives different ones, still within the form's bound."""
+    c = CASES["cylinder"]
+    ref = None
+    for launch in ((256, 1, 1, 0, 100), (64, 1, 4, 3, 100), (64, 8, 1, 5, 6), (128, 1, 1, 2, 0)):
+        ens, o = build_gpu(c, storage)
+        ens.set_stochastic_rounding(True, 1234)
+        ens.set_launch(*launch[:4], graph_steps=launch[4]) if launch[4] else ens.set_launch(*launch[:4])
+        for n in (3, 8):
+            ens.step(n)
+        f = ens.download(0)
+        if ref is None:
+            ref = f
+            for _ in range(11):
+                o.step(c.tau, c.u)
+            assert np.abs(f - o.f).max() <= TOL[storage]
+        else:
+            np.testing.assert_array_equal(f, ref, err_msg=f"launch {launch}")
+    other, _ = build_gpu(c, storage)
+    other.set_stochastic_rounding(True, 99)
+    other.step(11)
+    assert not np.array_equal(other.download(0), ref)
+    with pytest.raises(G.LbmError):
+        G.Ensemble([G.desc_for_case(G.CYLINDER, 64, 32)], storage="f32").set_stochastic_rounding(True)
