@@ -293,10 +293,13 @@ def bench_large(steps, warmup, torch, stream, horizon=100, seed=42):
     t_geom = time.perf_counter() - t0
     n_slots = 1 + max(k[3] for k in links)
     lat = {}
-    for storage in ("f64", "f32", "f16s", "bf16s"):
+    for form in ("f64", "f32", "f16s", "bf16s", "f16s_sr", "bf16s_sr"):
+        storage = form.split("_")[0]
         t1 = time.perf_counter()
         d = G.desc_for_case(G.CYLINDER, nx, ny, tau=tau, u_in=u, les=True, cs=cs)
         e = G.Ensemble([d], storage=storage, stream=stream)
+        if form.endswith("_sr"):
+            e.set_stochastic_rounding(True, 1)
         e.set_obstacle(0, mask)
         e.set_force_slots(0, n_slots)
         e.set_links(0, links)
@@ -305,15 +308,15 @@ def bench_large(steps, warmup, torch, stream, horizon=100, seed=42):
             m0 = e.mass(0)
         e.step(horizon)
         e.sync()
-        lat[storage] = e
-        print(f"# 32768x16384 {storage}: built and stepped {horizon} in {time.perf_counter() - t1:.1f} s",
+        lat[form] = e
+        print(f"# 32768x16384 {form}: built and stepped {horizon} in {time.perf_counter() - t1:.1f} s",
               file=sys.stderr, flush=True)
     out = {"cells": nx * ny, "obstacles": "64 circles r 20-60 (Mei links) + 64 plates 155x2 (staircase)",
            "seed": seed, "links": len(links), "host_geometry_s": t_geom, "horizon": horizon}
     strip = 1024
     fluid = mask == 0
     out["fp64_mass_drift"] = lat["f64"].mass(0) / m0 - 1.0
-    pairs = [(s, r) for r in ("f64", "f32") for s in ("f32", "f16s", "bf16s") if s != r]
+    pairs = [(s, r) for r in ("f64", "f32") for s in ("f32", "f16s", "bf16s", "f16s_sr", "bf16s_sr") if s != r]
     acc = {p: [0.0, 0.0, 0] for p in pairs}
     for y0 in range(0, ny, strip):
         y1 = min(ny, y0 + strip)
@@ -330,7 +333,7 @@ def bench_large(steps, warmup, torch, stream, horizon=100, seed=42):
     for (s, r), (sq, mx, n) in acc.items():
         out[f"vel_err_{s}_vs_{r}"] = {"rms": float(np.sqrt(sq / n)) / u, "max": mx / u}
     lat.pop("f64").close()
-    for s in ("f32", "f16s", "bf16s"):
+    for s in ("f32", "f16s", "bf16s", "f16s_sr", "bf16s_sr"):
         e = lat.pop(s)
         shape, _ = e.autotune()
         info = e.info()
