@@ -35,6 +35,8 @@ struct Config {
     int scale_exp = 8;            // 16-bit forms
     int graph_steps = 100;        // run_steps: hipGraph length
     double shift_u = 0.0;         // 16-bit forms: store f - feq(1, shift_u, 0), e.g. the inflow
+    bool stochastic_rounding = false;  // 16-bit forms: zero-mean rounding of stored values
+    uint32_t rounding_seed = 0;
 };
 
 inline Config& config()
@@ -54,6 +56,8 @@ struct Context {
     // what the handle was built for
     int nx = 0, ny = 0, case_type = -1, collision = -1, les = -1, storage = -1;
     double cs = 0.0, body_force = 0.0, shift_u = 0.0;
+    bool sr = false;
+    uint32_t sr_seed = 0;
     // geometry last given to the handle
     std::vector<uint8_t> obstacle;
     BounceBackGeometry geom;
@@ -116,7 +120,8 @@ inline void prepare(LBMCapabilities& sys, double tau, double u_inflow)
     const int collision = g_collision == CollisionType::MRT ? LBM_MRT : LBM_BGK;
     const int les = g_use_les ? 1 : 0;
     if (!C.h || C.nx != NX || C.ny != NY || C.case_type != case_type || C.collision != collision || C.les != les ||
-        C.cs != g_cs || C.body_force != sys.body_force_x || C.storage != cfg.storage || C.shift_u != cfg.shift_u) {
+        C.cs != g_cs || C.body_force != sys.body_force_x || C.storage != cfg.storage || C.shift_u != cfg.shift_u ||
+        C.sr != cfg.stochastic_rounding || C.sr_seed != cfg.rounding_seed) {
         if (C.h) {
             lbm_destroy(C.h);
             C.h = nullptr;
@@ -131,7 +136,11 @@ inline void prepare(LBMCapabilities& sys, double tau, double u_inflow)
         d.u_in = u_inflow;
         check(lbm_create(&d, 1, cfg.storage, cfg.scale_exp, nullptr, &C.h), "lbm_create");
         if (cfg.shift_u != 0.0) check(lbm_set_storage_shift(C.h, 0, cfg.shift_u, 0.0), "lbm_set_storage_shift");
+        if (cfg.stochastic_rounding)
+            check(lbm_set_stochastic_rounding(C.h, 1, cfg.rounding_seed), "lbm_set_stochastic_rounding");
         C.shift_u = cfg.shift_u;
+        C.sr = cfg.stochastic_rounding;
+        C.sr_seed = cfg.rounding_seed;
         C.nx = NX;
         C.ny = NY;
         C.case_type = case_type;
