@@ -185,10 +185,15 @@ template <> __device__ __forceinline__ double dexp<double>(double x) { return ex
 //   pxx - pxx_eq = (d1 + d3) - (d2 + d4) - (jx u - jy v)
 //   pxy - pxy_eq = (d5 + d7) - (d6 + d8) - jx v
 // and the rows of M^-1 regroup into the few sums below.
+template <typename R> __device__ __forceinline__ R fma_(R a, R b, R c);
+template <> __device__ __forceinline__ float fma_<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fma_<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename R>
 __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist, R sc)
 {
-    // unfused: the same rounding in every kernel and launch shape that inlines this
+    // Fused multiply-adds are written out and nothing else is fused: the same rounding in
+    // every kernel and launch shape that inlines this.
 #pragma clang fp contract(off)
     const R a13 = d[1] + d[3], s13 = d[1] - d[3];
     const R a24 = d[2] + d[4], s24 = d[2] - d[4];
@@ -205,19 +210,20 @@ __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist
     if (L.collision != 0) {
         // BGK: d_i -= (d_i - sc (feq_i - w_i)) / tau
         const R it = static_cast<R>(L.inv_tau);
-        const R usq = u * u + v * v;
+        const R usq = fma_<R>(u, u, v * v);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             const R eu = R(CX[i]) * u + R(CY[i]) * v;
-            const R feq_dev = wgt<R>(i) * (sum + rho_s * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
-            d[i] -= it * (d[i] - feq_dev);
+            const R shape = fma_<R>(eu, fma_<R>(R(4.5), eu, R(3)), R(-1.5) * usq);
+            const R feq_dev = wgt<R>(i) * fma_<R>(rho_s, shape, sum);
+            d[i] = fma_<R>(-it, d[i] - feq_dev, d[i]);
         }
     } else {
-        const R J2 = jx * u + jy * v;                                   // sc rho |u|^2
-        const R ne = R(-2) * d[0] + sax + R(4) * sdg - R(3) * J2;        // e - e_eq
-        const R e3 = d[0] - sax + J2;                                    // (eps - eps_eq) / 3
-        const R npxx = (a13 - a24) - (jx * u - jy * v);
-        const R npxy = (a57 - a68) - jx * v;
+        const R J2 = fma_<R>(jx, u, jy * v);                                      // sc rho |u|^2
+        const R ne = fma_<R>(R(-2), d[0], fma_<R>(R(4), sdg, fma_<R>(R(-3), J2, sax)));  // e - e_eq
+        const R e3 = d[0] - sax + J2;                                             // (eps - eps_eq) / 3
+        const R npxx = (a13 - a24) - fma_<R>(jx, u, -(jy * v));
+        const R npxy = fma_<R>(-jx, v, a57 - a68);
         R ss = static_cast<R>(L.s_shear);
         if (L.les) {
             // Smagorinsky with Van Driest damping (src/lbm.hpp:155-182, y+ :314-325).
@@ -225,7 +231,7 @@ __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist
             R cs2 = static_cast<R>(L.cs2);
             const R nu = static_cast<R>(L.nu);
             if (nu > R(0)) {
-                const R y_plus = wall_dist * dsqrt<R>(u * u + v * v + R(1e-12)) * static_cast<R>(L.inv_nu);
+                const R y_plus = wall_dist * dsqrt<R>(fma_<R>(u, u, fma_<R>(v, v, R(1e-12)))) * static_cast<R>(L.inv_nu);
                 if (y_plus > R(0)) {
                     const R damp = R(1) - dexp<R>(y_plus * R(-1.0 / 26.0));
                     const R csd = static_cast<R>(L.cs) * damp;
@@ -233,22 +239,21 @@ __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist
                 }
             }
             // A = 9 cs^2 Q / (2 rho); Q/rho is scale free
-            const R Q = dsqrt<R>(R(2) * (npxx * npxx + npxy * npxy));
+            const R Q = dsqrt<R>(R(2) * fma_<R>(npxx, npxx, npxy * npxy));
             const R A = R(4.5) * cs2 * Q * ir;
             const R tau = static_cast<R>(L.tau);
-            const R tau_eff = (tau + dsqrt<R>(tau * tau + R(4) * A)) * R(0.5);
+            const R tau_eff = (tau + dsqrt<R>(fma_<R>(tau, tau, R(4) * A))) * R(0.5);
             ss = clamp_rate<R>(recip<R>(tau_eff));
         }
         // M^-1 S (m - m_eq), s_bulk = 1.2 and s_normal = 1
-        const R a = ne * R(-1.2 / 36.0) - e3 * R(3.6 / 18.0);
-        const R b = ne * R(1.2 / 18.0) + e3 * R(3.6 / 36.0);
-        const R c0 = e3 * R(3.6 / 9.0) - ne * R(1.2 / 9.0);
-        const R qx6 = jx * R(1.0 / 3.0) - s13 * R(0.5);  // (qx - qx_eq) / 6
-        const R qy6 = jy * R(1.0 / 3.0) - s24 * R(0.5);
+        const R a = fma_<R>(ne, R(-1.2 / 36.0), e3 * R(-3.6 / 18.0));
+        const R b = fma_<R>(ne, R(1.2 / 18.0), e3 * R(3.6 / 36.0));
+        const R c0 = fma_<R>(e3, R(3.6 / 9.0), ne * R(-1.2 / 9.0));
+        const R qx6 = fma_<R>(jx, R(1.0 / 3.0), s13 * R(-0.5));  // (qx - qx_eq) / 6
+        const R qy6 = fma_<R>(jy, R(1.0 / 3.0), s24 * R(-0.5));
         const R ss4 = ss * R(0.25);
-        const R p4 = ss4 * npxx, x4 = ss4 * npxy;
-        const R t = a + p4, t2 = a - p4;
-        const R u1 = b + x4, u2 = b - x4;
+        const R t = fma_<R>(ss4, npxx, a), t2 = fma_<R>(-ss4, npxx, a);
+        const R u1 = fma_<R>(ss4, npxy, b), u2 = fma_<R>(-ss4, npxy, b);
         const R qp = (qx6 + qy6) * R(0.5), qm = (qx6 - qy6) * R(0.5);
         d[0] -= c0;
         d[1] = d[1] - t + qx6;
