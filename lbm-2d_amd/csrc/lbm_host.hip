@@ -150,6 +150,11 @@ struct Ensemble {
     int32_t* d_slot_lat = nullptr; // lattice of each force slot
     int32_t* d_special = nullptr;  // every lattice's special-node list
     size_t special_cap = 0;
+    double* d_nf = nullptr;        // [acc_blocks][n_recs][2] force terms per link record
+    double* d_dummy_nf = nullptr;
+    int32_t* d_slot_start = nullptr;  // records per force slot (CSR), in record order
+    int32_t* d_slot_recs = nullptr;
+    int n_recs = 0;
     std::vector<LatticeDev> host_lat;
     double* d_hist = nullptr;
     double* d_dummy_acc = nullptr;
@@ -473,6 +478,31 @@ int commit(Ensemble* E)
                 if (!H.wdist.empty()) std::copy(H.wdist.begin(), H.wdist.end(), wd.begin() + H.cell_off);
             }
             if ((rc = h2d(E, E->d_links, all.data(), all.size() * sizeof(LinkRec))) != LBM_OK) return rc;
+            // each record's force terms go to its primary slot: that of its first counted link
+            std::vector<std::vector<int32_t>> per_slot(slots);
+            for (auto& H : E->lat) {
+                if (!H.d.forces) continue;
+                for (size_t r = 0; r < H.recs.size(); ++r)
+                    for (int i = 1; i < 9; ++i)
+                        if ((H.recs[r].mask & (1u << i)) && H.recs[r].slot[i] >= 0) {
+                            per_slot[H.slot_base + H.recs[r].slot[i]].push_back(H.link_off + static_cast<int32_t>(r));
+                            break;
+                        }
+            }
+            std::vector<int32_t> start(1, 0), recs_of;
+            for (auto& v : per_slot) {
+                recs_of.insert(recs_of.end(), v.begin(), v.end());
+                start.push_back(static_cast<int32_t>(recs_of.size()));
+            }
+            if ((rc = dev_alloc(E, &E->d_slot_start, start.size())) != LBM_OK) return rc;
+            if ((rc = dev_alloc(E, &E->d_slot_recs, recs_of.size())) != LBM_OK) return rc;
+            if ((rc = h2d(E, E->d_slot_start, start.data(), start.size() * 4)) != LBM_OK) return rc;
+            if ((rc = h2d(E, E->d_slot_recs, recs_of.data(), recs_of.size() * 4)) != LBM_OK) return rc;
+            if (static_cast<int>(nrec) != E->n_recs || !E->d_dummy_nf) {
+                E->n_recs = static_cast<int>(nrec);
+                if ((rc = dev_alloc(E, &E->d_dummy_nf, nrec * 2)) != LBM_OK) return rc;
+                E->acc_blocks = 0;  // the per-record force blocks are reallocated below
+            }
             std::vector<int32_t> special;
             for (auto& H : E->lat) {
                 H.special_off = static_cast<int64_t>(special.size());
@@ -512,6 +542,7 @@ int commit(Ensemble* E)
         const size_t acc_n = static_cast<size_t>(want_blocks) * kAccCopies * E->total_slots * 2;
         if ((rc = dev_alloc(E, &E->d_acc, acc_n)) != LBM_OK) return rc;
         LBM_HIP(hipMemsetAsync(E->d_acc, 0, acc_n * 8, E->stream));
+        if ((rc = dev_alloc(E, &E->d_nf, static_cast<size_t>(want_blocks) * E->n_recs * 2)) != LBM_OK) return rc;
         E->acc_blocks = want_blocks;
         E->graph_dirty = true;
     }
@@ -683,6 +714,7 @@ int launch_batch(Ensemble* E, int n, int l)
     const bool two = single && (E->options & LAUNCH_TWO_STEPS);
     const int pairs = two_step_pairs(E, n, l);
     auto block_of = [&](int i) { return E->d_acc + static_cast<int64_t>(i) * kAccCopies * E->total_slots * 2; };
+    auto terms_of = [&](int i) { return E->d_nf + static_cast<int64_t>(i) * E->n_recs * 2; };
     int swaps = 0;
     if (two) {
         // two-step launches on 64 x rows tiles, then one-step launches of shape (64, 1, 1)
@@ -697,6 +729,8 @@ int launch_batch(Ensemble* E, int n, int l)
             a2.dst = E->buf[1 - sp];
             a2.acc = block_of(2 * i);
             a2.acc2 = block_of(2 * i + 1);
+            a2.nf = terms_of(2 * i);
+            a2.nf2 = terms_of(2 * i + 1);
             a2.step_arg = 2 * i;
             LBM_HIP(launch_step2(E->storage, E->rows, blocks2, a2, E->stream));
         }
@@ -709,12 +743,14 @@ int launch_batch(Ensemble* E, int n, int l)
             a1.src = E->buf[sp];
             a1.dst = E->buf[1 - sp];
             a1.acc = block_of(i);
+            a1.nf = terms_of(i);
             a1.step_arg = i;
             LBM_HIP(launch_step(E->storage, 64, 1, 1, true, blocks1, a1, E->stream));
         }
     }
     for (int i = 0; i < n && !two; ++i, ++swaps) {
         a.acc = block_of(i);
+        a.nf = terms_of(i);
         a.step_arg = i;
         if (single) {
             const int sp = par ^ (i & 1);
@@ -725,6 +761,10 @@ int launch_batch(Ensemble* E, int n, int l)
     }
     FlushArgs f{};
     f.acc = E->d_acc;
+    f.nf = E->d_nf;
+    f.n_recs = E->n_recs;
+    f.slot_start = E->d_slot_start;
+    f.slot_recs = E->d_slot_recs;
     f.hist = E->d_hist;
     f.steps = E->d_steps;
     f.slot_lattice = E->d_slot_lat;
@@ -1016,7 +1056,8 @@ int lbm_destroy(lbm_handle h)
     lbm::drop_graphs(E);
     void* ptrs[] = {E->buf[0], E->buf[1], E->d_flags, E->d_lslot, E->d_wdist, E->d_links, E->d_lat,
                     E->d_tiles, E->d_steps, E->d_par, E->d_acc, E->d_hist, E->d_dummy_acc, E->d_nan,
-                    E->d_dummy_nan, E->d_mass, E->d_stage, E->d_slot_lat, E->d_special};
+                    E->d_dummy_nan, E->d_mass, E->d_stage, E->d_slot_lat, E->d_special, E->d_nf,
+                    E->d_dummy_nf, E->d_slot_start, E->d_slot_recs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
@@ -1388,6 +1429,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         a.src = E->buf[E->lat[0].par];
         a.dst = scratch;
         a.acc = E->d_dummy_acc;
+        a.nf = E->d_dummy_nf;
         a.nan_flags = E->d_dummy_nan;
         const int reps = 10;
         hipError_t le = hipSuccess;
@@ -1430,6 +1472,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         a.src = E->buf[E->lat[0].par];
         a.dst = scratch;
         a.acc = a.acc2 = E->d_dummy_acc;
+        a.nf = a.nf2 = E->d_dummy_nf;
         a.nan_flags = E->d_dummy_nan;
         const int reps = 5;
         hipError_t le = hipSuccess;

@@ -48,8 +48,10 @@ struct StepArgs {
     const int32_t* link_slot;
     const LinkRec* links;
     const uint16_t* wdist;
-    double* acc;                // this launch's accumulator block [kAccCopies][total_slots][2]
-    double* acc2;               // two-step launches: the second step's block
+    double* acc;                // this launch's fallback accumulators [kAccCopies][total_slots][2]
+    double* acc2;               // two-step launches: the second step's
+    double* nf;                 // this launch's force terms per link record [records][2]
+    double* nf2;                // two-step launches: the second step's
     const int64_t* steps;       // per-lattice step count at the start of the batch
     int32_t sr;                 // 16-bit forms: stochastic rounding of stored values
     uint32_t sr_seed;
@@ -61,6 +63,10 @@ struct StepArgs {
 // Force totals of a batch of n launches: accumulator blocks -> history ring.
 struct FlushArgs {
     double* acc;                // [n][kAccCopies][total_slots][2], cleared on the way
+    const double* nf;           // [n][n_recs][2] force terms per link record
+    int32_t n_recs;
+    const int32_t* slot_start;  // records of slot s: slot_recs[slot_start[s] .. slot_start[s+1])
+    const int32_t* slot_recs;
     double* hist;               // [total_slots][R][2]
     const int64_t* steps;       // per-lattice step count at the start of the batch
     const int32_t* slot_lattice;  // lattice of each force slot

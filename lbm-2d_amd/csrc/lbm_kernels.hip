@@ -278,6 +278,14 @@ __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist
 
 // ------------------------------------------------------------------ general path
 
+// Where a launch's link records put their force terms: nf[2 r], nf[2 r + 1] for record r
+// (this launch's block), accp for terms of a record's secondary slots (atomic).  A null
+// nf: forces not counted.
+struct ForceSink {
+    double* nf;
+    double* accp;
+};
+
 template <int ST> struct Ctx {
     using S = typename Form<ST>::store;
     const LatticeDev& L;
@@ -497,43 +505,6 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Per-thread force partial keyed by slot; a second slot goes straight to memory.
-struct ForcePart {
-    int slot = -1;
-    double fx = 0.0, fy = 0.0;
-    __device__ __forceinline__ void add(int s, double x, double y, double* accp)
-    {
-        if (slot < 0 || slot == s) {
-            slot = s;
-            fx += x;
-            fy += y;
-        } else {
-            atomicAdd(accp + 2 * s, x);
-            atomicAdd(accp + 2 * s + 1, y);
-        }
-    }
-};
-
-// Wave-level reduction of the per-thread partials, one atomic per distinct slot.
-__device__ __forceinline__ void flush_forces(ForcePart p, double* accp)
-{
-    int my = p.slot;
-    const int lane = __lane_id();
-    for (;;) {
-        const uint64_t m = __ballot(my >= 0);
-        if (m == 0) break;
-        const int lead = __ffsll(static_cast<long long>(m)) - 1;
-        const int s = __shfl(my, lead, 64);
-        const double vx = wave_sum(my == s ? p.fx : 0.0);
-        const double vy = wave_sum(my == s ? p.fy : 0.0);
-        if (lane == lead) {
-            atomicAdd(accp + 2 * s, vx);
-            atomicAdd(accp + 2 * s + 1, vy);
-        }
-        if (my == s) my = -1;
-    }
-}
-
 // ------------------------------------------------------------------ step kernel
 
 // Store of a value or an aligned pack of values, optionally nontemporal.
@@ -573,21 +544,40 @@ __device__ __forceinline__ void links_of(const Ctx<ST>& c, const Pre<ST>& p, Lin
 // lg), momentum exchange on it with the node's own links (ls_, src/lbm.hpp:542-581),
 // collision, store.
 template <int ST>
-__device__ __forceinline__ void node_general_d(const LatticeDev& L, const Ctx<ST>& c, double* accp, int x, int y,
-                                               uint8_t fl, const Pre<ST>& p,
+__device__ __forceinline__ void node_general_d(const LatticeDev& L, const Ctx<ST>& c, const ForceSink& fs, int x,
+                                               int y, uint8_t fl, const Pre<ST>& p,
                                                const LinkVals<typename Form<ST>::real>& lg,
                                                const LinkVals<typename Form<ST>::real>& ls_, uint16_t wraw,
-                                               ForcePart& part, bool& bad, typename Form<ST>::real d[9])
+                                               bool& bad, typename Form<ST>::real d[9])
 {
     using R = typename Form<ST>::real;
     const bool solid = (fl & CELL_SOLID) != 0;
     state_general<ST>(c, x, y, solid, p, lg, d);
-    if (accp && (fl & CELL_LINKS) && L.forces) {
+    if (fs.nf && (fl & CELL_LINKS) && L.forces) {
+        // momentum exchange on the post-boundary state (src/lbm.hpp:542-581): the terms of
+        // the record's primary slot (that of its first counted link) summed in direction
+        // order and stored for the record; terms of any other slot go to the fallback
+        // accumulator
+        int ps = -1;
+        double fx = 0.0, fy = 0.0;
 #pragma unroll
         for (int i = 1; i < 9; ++i) {
-            if (!(ls_.mask & (1u << i)) || ls_.slot(i) < 0) continue;
+            const int si = ls_.slot(i);
+            if (!(ls_.mask & (1u << i)) || si < 0) continue;
             const double fb = WD[OPP[i]] + static_cast<double>(d[OPP[i]]);
-            part.add(L.slot_base + ls_.slot(i), CX[i] * 2.0 * fb, CY[i] * 2.0 * fb, accp);
+            if (ps < 0) ps = si;
+            if (si == ps) {
+                fx += CX[i] * 2.0 * fb;
+                fy += CY[i] * 2.0 * fb;
+            } else {
+                atomicAdd(fs.accp + 2 * (L.slot_base + si), CX[i] * 2.0 * fb);
+                atomicAdd(fs.accp + 2 * (L.slot_base + si) + 1, CY[i] * 2.0 * fb);
+            }
+        }
+        if (ps >= 0) {
+            const int64_t r = ls_.rec - c.links;
+            fs.nf[2 * r] = fx;
+            fs.nf[2 * r + 1] = fy;
         }
     }
     if (!solid) {
@@ -600,15 +590,15 @@ __device__ __forceinline__ void node_general_d(const LatticeDev& L, const Ctx<ST
 
 template <int ST>
 __device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
-                                             typename Form<ST>::store* dst, double* accp, int x, int y,
+                                             typename Form<ST>::store* dst, const ForceSink& fs, int x, int y,
                                              uint8_t fl, const Pre<ST>& p,
                                              const LinkVals<typename Form<ST>::real>& lg,
                                              const LinkVals<typename Form<ST>::real>& ls_, uint16_t wraw,
-                                             const Rounding& rd, ForcePart& part, bool& bad)
+                                             const Rounding& rd, bool& bad)
 {
     using R = typename Form<ST>::real;
     R d[9];
-    node_general_d<ST>(L, c, accp, x, y, fl, p, lg, ls_, wraw, part, bad, d);
+    node_general_d<ST>(L, c, fs, x, y, fl, p, lg, ls_, wraw, bad, d);
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
     auto* o = dst + L.f_off + idx;
     uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
@@ -644,8 +634,8 @@ __device__ __forceinline__ void edge_position(const LatticeDev& L, int e, int& x
 // of the general path in the kernel.
 template <int ST>
 __device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
-                                          typename Form<ST>::store* dst, double* accp, int t, const Rounding& rd,
-                                          ForcePart& part, bool& bad)
+                                          typename Form<ST>::store* dst, const ForceSink& fs, int t,
+                                          const Rounding& rd, bool& bad)
 {
     using R = typename Form<ST>::real;
     const bool is_link = t < a.n_links;
@@ -670,15 +660,14 @@ __device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a
     if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return;
     LinkVals<R> lg = own_links;
     if (!is_link || gx != x) links_of<ST>(c, p, lg);
-    node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, own_links, wraw, rd, part, bad);
+    node_general<ST>(L, a, c, dst, fs, x, y, fl, p, lg, own_links, wraw, rd, bad);
 }
 
 // One node, bulk or general.  SKIP_SPECIAL: general nodes are left to the lists.
 template <int ST, bool SKIP_SPECIAL>
 __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs& a,
                                             const Ctx<ST>& c, typename Form<ST>::store* dst,
-                                            double* accp, int x, int y, const Rounding& rd, ForcePart& part,
-                                            bool& bad)
+                                            const ForceSink& fs, int x, int y, const Rounding& rd, bool& bad)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
@@ -705,7 +694,7 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
         links_of<ST>(c, p, lg);
         LinkVals<R> ls_ = lg;
         if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
-        node_general<ST>(L, a, c, dst, accp, x, y, fl, p, lg, ls_, wraw, rd, part, bad);
+        node_general<ST>(L, a, c, dst, fs, x, y, fl, p, lg, ls_, wraw, rd, bad);
         return;
     }
     R d[9];
@@ -881,7 +870,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
         if (b < a.special_blocks) {
             double* accp = a.acc + static_cast<int64_t>(blockIdx.x % kAccCopies) * a.total_slots * 2;
             const Ctx<ST> c{L, src + L.f_off, a.flags, a.link_slot, a.links, a.inv_scale};
-            ForcePart part;
+            const ForceSink fs{a.nf, accp};
             bool bad = false;
             // link and edge nodes: kListLanes per wave, since their loads scatter and a
             // wave's memory requests issue one address at a time
@@ -890,7 +879,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
             if (b < gb) {
                 const int lane = static_cast<int>(threadIdx.x) & 63;
                 const int t = b * per_block + (static_cast<int>(threadIdx.x) >> 6) * kListLanes + lane;
-                if (lane < kListLanes && t < a.n_links + a.n_edges) list_node<ST>(L, a, c, dst, accp, t, rd, part, bad);
+                if (lane < kListLanes && t < a.n_links + a.n_edges) list_node<ST>(L, a, c, dst, fs, t, rd, bad);
             } else {
                 // written solids (Bouzidi sources): zero populations, f - w = -w
                 const int t = (b - gb) * BX + static_cast<int>(threadIdx.x);
@@ -903,7 +892,6 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
                 }
             }
             if (bad) atomicOr(a.nan_flags + lat_id, 1);
-            flush_forces(part, accp);
             return;
         }
         b -= a.special_blocks;
@@ -931,7 +919,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
     double* accp = a.acc + static_cast<int64_t>(blockIdx.x % kAccCopies) * a.total_slots * 2;
 
     const Ctx<ST> c{L, src + L.f_off, a.flags, a.link_slot, a.links, a.inv_scale};
-    ForcePart part;
+    const ForceSink fs{a.nf, accp};
     bool bad = false;
     const int x = x0 + V * static_cast<int>(threadIdx.x);
 
@@ -961,11 +949,10 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
             // one copy of the general path, looped (keeps the kernel small)
 #pragma unroll 1
             for (int q = 0; q < V; ++q)
-                if (x + q < L.nx) cell_update<ST, SINGLE>(L, a, c, dst, accp, x + q, y, rd, part, bad);
+                if (x + q < L.nx) cell_update<ST, SINGLE>(L, a, c, dst, fs, x + q, y, rd, bad);
         }
     }
     if (bad) atomicOr(a.nan_flags + lat_id, 1);
-    flush_forces(part, accp);
 }
 
 // ------------------------------------------------------------------ two steps per launch
@@ -974,8 +961,8 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
 // like cell_update; forces counted only when accp is non-null.  false: a solid node
 // nothing reads (no value).
 template <int ST>
-__device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c, double* accp,
-                                            int x, int y, const Rounding& rd, ForcePart& part, bool& bad,
+__device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
+                                            const ForceSink& fs, int x, int y, const Rounding& rd, bool& bad,
                                             typename Form<ST>::store out[9])
 {
     using S = typename Form<ST>::store;
@@ -1000,7 +987,7 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
         links_of<ST>(c, p, lg);
         LinkVals<R> ls_ = lg;
         if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
-        node_general_d<ST>(L, c, accp, x, y, fl, p, lg, ls_, wraw, part, bad, d);
+        node_general_d<ST>(L, c, fs, x, y, fl, p, lg, ls_, wraw, bad, d);
     } else {
 #pragma unroll
         for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(raw[j], a.inv_scale, L.shift[j]);
@@ -1067,7 +1054,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
         }
     }
 
-    ForcePart part0, part1;
+    const ForceSink fs0{a.nf, acc0}, fs1{a.nf2, acc1}, none{nullptr, nullptr};
     bool bad = false;
     Rounding rd0{a.sr != 0, 0u}, rd1{a.sr != 0, 0u};
     if (rd0.on) {
@@ -1089,7 +1076,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
         }
         const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny;
         S out[9];
-        if (node_values<ST>(L, a, c, own ? acc0 : nullptr, gx, gy, rd0, part0, bad, out)) {
+        if (node_values<ST>(L, a, c, own ? fs0 : none, gx, gy, rd0, bad, out)) {
 #pragma unroll
             for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
         }
@@ -1134,7 +1121,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
             links_of<ST>(c, p, lg);
             LinkVals<R> ls_ = lg;
             if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + lsk], ls_);
-            node_general_d<ST>(L, c, acc1, x, y, fl, p, lg, ls_, wdk, part1, bad, d);
+            node_general_d<ST>(L, c, fs1, x, y, fl, p, lg, ls_, wdk, bad, d);
         } else {
 #pragma unroll
             for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(lds[j][ly - CY[j]][lx - CX[j]], a.inv_scale, L.shift[j]);
@@ -1151,8 +1138,6 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
             store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j), a.options & LAUNCH_STREAM_STORES);
     }
     if (bad) atomicOr(a.nan_flags + a.lattice0, 1);
-    flush_forces(part0, acc0);
-    flush_forces(part1, acc1);
 }
 
 // ------------------------------------------------------------------ state / prologue
@@ -1288,36 +1273,50 @@ __global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, in
     }
 }
 
-// Moves the force totals of a batch of n launches to the history ring and clears the
-// accumulators: thread (i, slot) sums the kAccCopies copies of launch i in copy order
-// and writes history row (steps[lattice] + i) % R.  Runs before add_steps_kernel.
-__global__ void flush_kernel(FlushArgs f)
+// Moves the force totals of a batch of n launches to the history ring.  Block (i, s)
+// sums, for launch i and force slot s, the terms its link records stored (records of
+// the slot in record order, strided over the block, then a fixed LDS tree) and the
+// kAccCopies fallback copies in copy order, clears those copies and writes history row
+// (steps[lattice] + i) % R.  The order of every addition is fixed: the totals repeat bit
+// for bit.  Runs before add_steps_kernel.
+__global__ __launch_bounds__(256) void flush_kernel(FlushArgs f)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= f.n * f.slot_count) return;
-    const int i = p / f.slot_count;
-    const int s = f.slot_begin + p % f.slot_count;
-    double* blk = f.acc + static_cast<int64_t>(i) * kAccCopies * f.total_slots * 2;
-    double v[2 * kAccCopies];
-#pragma unroll
-    for (int c = 0; c < kAccCopies; ++c) {
-        v[2 * c] = blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2];
-        v[2 * c + 1] = blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2 + 1];
-    }
+    __shared__ double sx[256], sy[256];
+    const int pair = static_cast<int>(blockIdx.x);
+    const int i = pair / f.slot_count;
+    const int s = f.slot_begin + pair % f.slot_count;
+    const int tid = static_cast<int>(threadIdx.x);
+    const double* nfi = f.nf + static_cast<int64_t>(i) * f.n_recs * 2;
     double fx = 0.0, fy = 0.0;
-#pragma unroll
-    for (int c = 0; c < kAccCopies; ++c) {
-        fx += v[2 * c];
-        fy += v[2 * c + 1];
+    for (int r = f.slot_start[s] + tid; r < f.slot_start[s + 1]; r += 256) {
+        const int64_t rec = f.slot_recs[r];
+        fx += nfi[2 * rec];
+        fy += nfi[2 * rec + 1];
     }
-#pragma unroll
-    for (int c = 0; c < kAccCopies; ++c) {
-        blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2] = 0.0;
-        blk[(static_cast<int64_t>(c) * f.total_slots + s) * 2 + 1] = 0.0;
+    sx[tid] = fx;
+    sy[tid] = fy;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            sx[tid] += sx[tid + o];
+            sy[tid] += sy[tid + o];
+        }
+        __syncthreads();
     }
-    const int64_t row = (f.steps[f.slot_lattice[s]] + i) % f.R;
-    f.hist[(static_cast<int64_t>(s) * f.R + row) * 2] = fx;
-    f.hist[(static_cast<int64_t>(s) * f.R + row) * 2 + 1] = fy;
+    if (tid == 0) {
+        double* blk = f.acc + static_cast<int64_t>(i) * kAccCopies * f.total_slots * 2;
+        double gx = sx[0], gy = sy[0];
+        for (int c = 0; c < kAccCopies; ++c) {
+            double* q = blk + (static_cast<int64_t>(c) * f.total_slots + s) * 2;
+            gx += q[0];
+            gy += q[1];
+            q[0] = 0.0;
+            q[1] = 0.0;
+        }
+        const int64_t row = (f.steps[f.slot_lattice[s]] + i) % f.R;
+        f.hist[(static_cast<int64_t>(s) * f.R + row) * 2] = gx;
+        f.hist[(static_cast<int64_t>(s) * f.R + row) * 2 + 1] = gy;
+    }
 }
 
 // A group of nodes moves at most 16 bytes per plane and thread (one dwordx4).
@@ -1427,7 +1426,7 @@ hipError_t launch_flush(const FlushArgs& f, hipStream_t s)
 {
     const int n = f.n * f.slot_count;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(flush_kernel, dim3((n + 255) / 256), dim3(256), 0, s, f);
+    hipLaunchKernelGGL(flush_kernel, dim3(n), dim3(256), 0, s, f);
     return hipGetLastError();
 }
 
