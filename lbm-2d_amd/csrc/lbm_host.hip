@@ -6,7 +6,8 @@
 //   link_slot    int32 per node: index of the node's boundary-link record
 //   links        one LinkRec per fluid node with links (q, scheme, wall velocity, slot)
 //   wdist        uint16 wall distance per node (LES lattices)
-//   forces       FP64 accumulators [2][slots] and a per-slot history ring
+//   forces       per launch of a batch: FP64 force terms per link record and fallback
+//                accumulators per slot; a per-slot history ring filled by the flush
 // and builds them from the reference's obstacle/geometry description
 // (src/lbm_types.hpp:90-182, src/lbm.hpp:79-87, 931-964).
 #include <hip/hip_runtime.h>

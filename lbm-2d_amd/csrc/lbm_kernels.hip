@@ -6,16 +6,19 @@
 //   1. gathers C_k(x - e_j) for every direction j (streaming, src/lbm.hpp:405-484),
 //      or the bounce-back value of the link when x - e_j is solid (src/lbm.hpp:226-288),
 //   2. applies the open boundaries to that node (src/lbm.hpp:530-539),
-//   3. adds its momentum-exchange terms to the FP64 force sum (src/lbm.hpp:542-581),
-//   4. collides (MRT / BGK, LES, body force) and writes C_{k+1}(x) in place of nothing:
-//      reads come from one buffer, writes go to the other (A-B pattern).
-// Storage keeps deviations h_i = (f_i - w_i) * 2^k (k = 0 for F64/F32) in SoA planes
-// of pitch-aligned rows; arithmetic is done on deviations in FP32 (FP64 for F64).
+//   3. stores its momentum-exchange terms in FP64 for its link record
+//      (src/lbm.hpp:542-581), summed per slot in a fixed order after the batch,
+//   4. collides (MRT / BGK, LES, body force) and writes C_{k+1}(x): reads come from one
+//      buffer, writes go to the other (A-B pattern).
+// Storage keeps deviations h_i = (f_i - w_i - s_i) * 2^k (k = 0 and s = 0 for F64/F32)
+// in SoA planes of pitch-aligned rows; arithmetic is done on deviations in FP32 (FP64
+// for F64).
 //
 // Bulk nodes (no edge, no solid neighbour) take a branch-free path of 9 shifted,
 // coalesced row loads, 9 aligned stores and one flag byte.  Every other node takes
 // the general path, which handles links, edges and boundary conditions exactly as
-// the reference's branches do.
+// the reference's branches do.  step2_kernel advances one lattice two steps per launch
+// through LDS.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
