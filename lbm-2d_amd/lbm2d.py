@@ -433,25 +433,29 @@ def strouhal_welch(cl, length: float, u: float):
     return float(f[band][np.argmax(psd[band])] * length / u)
 
 
-def strouhal_crossings(cl, length: float, u: float, tail: float = 0.5, hysteresis: float = 0.1):
+def strouhal_crossings(cl, length: float, u: float, tail: float = 0.5, hysteresis: float = 0.5):
     """Strouhal number from the mean period between upward crossings of Cl's mean over the
-    last `tail` of the history.  A crossing counts once the signal has been below
-    -hysteresis * rms since the previous one, so step-to-step noise around the mean adds
-    no crossings; each is placed by linear interpolation.  Resolves St far below the
-    Welch bin width 1/nperseg when the history holds few periods."""
+    last `tail` of the history.  The crossings are those of a Schmitt trigger: one counts
+    when the signal, having been below -hysteresis * rms, rises above +hysteresis * rms,
+    and it is placed (by linear interpolation) at the last upward zero crossing before
+    that rise, so noise near the mean or near either threshold adds no crossings.
+    Resolves St far below the Welch bin width 1/nperseg when the history holds few
+    periods."""
     c = np.asarray(cl, float)
     c = c[int(len(c) * (1.0 - tail)):]
     c = c - c.mean()
     h = hysteresis * float(np.sqrt(np.mean(c * c)))
-    below = c < -h
-    up = np.nonzero((c[:-1] < 0.0) & (c[1:] >= 0.0))[0]
-    # keep an upward crossing only if the signal went below -h since the last kept one
-    last_low = np.maximum.accumulate(np.where(below, np.arange(len(c)), -1))
-    times, prev = [], -1
-    for i in up:
-        if last_low[i] > prev:
+    up = (c[:-1] < 0.0) & (c[1:] >= 0.0)
+    times, armed, last_up = [], False, -1
+    for j in range(len(c)):
+        if j > 0 and up[j - 1]:
+            last_up = j - 1
+        if c[j] < -h:
+            armed = True
+        elif armed and c[j] > h:
+            i = last_up
             times.append(i + (-c[i]) / (c[i + 1] - c[i]))
-            prev = i
+            armed = False
     if len(times) < 2:
         return None
     return float(length / ((times[-1] - times[0]) / (len(times) - 1)) / u)

@@ -74,7 +74,10 @@ def fp64_run():
     return z, _summary(z["fx"], z["fy"])
 
 
-@pytest.mark.parametrize("form,tol", [("f64", 1e-9), ("f32", 1e-4)])
+# bounds on the relative errors of (Cd, Cl amplitude, St) over steps 25,000-50,000.  The
+# shedding amplitude is still growing there, so Cl amplitude moves with the onset time and
+# is bounded ten times looser than Cd and St for FP32.
+@pytest.mark.parametrize("form,tol", [("f64", (1e-9, 1e-9, 1e-9)), ("f32", (1e-4, 1e-3, 1e-4))])
 def test_cylinder_re100_50k_steps(form, tol, fp64_run):
     z, ref = fp64_run
     steps = int(z["steps"])
@@ -88,7 +91,8 @@ def test_cylinder_re100_50k_steps(form, tol, fp64_run):
     print(f"\n{form}: Cd {got['cd']:.6f} (FP64 {ref['cd']:.6f}, rel {rel['cd']:.2e})  "
           f"Cl amp {got['cl_amp']:.6f} (rel {rel['cl_amp']:.2e})  St {got['st_x']:.5f} "
           f"(FP64 {ref['st_x']:.5f}, rel {rel['st_x']:.2e})  Welch St {got['st_w']} (FP64 {ref['st_w']})")
-    assert max(rel.values()) <= tol
+    for key, t in zip(("cd", "cl_amp", "st_x"), tol):
+        assert rel[key] <= t, key
     assert got["st_w"] == pytest.approx(ref["st_w"], rel=1e-12)
     if form == "f64":
         np.testing.assert_allclose(fx, z["fx"], rtol=0, atol=1e-9 * np.abs(z["fx"]).max())

@@ -81,3 +81,15 @@ def test_strouhal_crossings_ignore_noise_at_the_mean():
     t = np.arange(80000)
     cl = 0.3 * np.sin(2 * np.pi * st * U / L * t) + 2e-3 * rng.standard_normal(len(t))
     assert abs(G.strouhal_crossings(cl, L, U) - st) / st < 2e-3
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_strouhal_crossings_ignore_noise_near_the_thresholds(seed):
+    # white noise at 7% of the amplitude: a zero crossing that noise makes while the
+    # signal still sits near -h must not count (a plain "below -h since the last one"
+    # rule counts it and the next real crossing too)
+    L, U, st = 60.0, 0.1, 0.2
+    rng = np.random.default_rng(seed)
+    t = np.arange(80000)
+    cl = 0.3 * np.sin(2 * np.pi * st * U / L * t) + 0.02 * rng.standard_normal(len(t))
+    assert abs(G.strouhal_crossings(cl, L, U) - st) / st < 2e-3
