@@ -96,6 +96,13 @@ def lib() -> C.CDLL:
         L.orc_mt_seed.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_mt_next.restype = C.c_uint32
         L.orc_mt_next.argtypes = [C.c_void_p]
+        L.orc_g.restype = _dp
+        L.orc_g.argtypes = [C.c_void_p]
+        L.orc_set_thermal.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 4
+        L.orc_thermal_equilibrium.restype = C.c_double
+        L.orc_thermal_equilibrium.argtypes = [C.c_int] + [C.c_double] * 3
+        L.orc_thermal_isothermal_solids.argtypes = [C.c_void_p, C.c_double]
+        L.orc_thermal_cavity_walls.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_uniform_real.restype = C.c_double
         L.orc_uniform_real.argtypes = [C.c_void_p, C.c_double, C.c_double]
         _lib = L
@@ -114,6 +121,10 @@ def _dptr(a: np.ndarray):
 
 def equilibrium(i, rho, u, v) -> float:
     return lib().orc_equilibrium(i, rho, u, v)
+
+
+def thermal_equilibrium(i, T, u, v) -> float:
+    return lib().orc_thermal_equilibrium(i, T, u, v)
 
 
 def macros(f9):
@@ -257,6 +268,23 @@ class Oracle:
     def step(self, tau, u_in, n=1):
         lib().orc_steps(self._h, n, tau, u_in)
 
+    @property
+    def g(self) -> np.ndarray:
+        """AoS thermal populations, shape (ny, nx, 9); a live view."""
+        return _arr(lib().orc_g(self._h), self.n * 9).reshape(self.ny, self.nx, 9)
+
+    def set_thermal(self, omega_k, beta=0.0, g_buoyancy=0.0, t_ref=1.0, on=True):
+        lib().orc_set_thermal(self._h, int(on), omega_k, beta, g_buoyancy, t_ref)
+
+    def set_g(self, g):
+        self.g[...] = np.asarray(g, dtype=np.float64).reshape(self.ny, self.nx, 9)
+
+    def thermal_isothermal_solids(self, t_wall):
+        lib().orc_thermal_isothermal_solids(self._h, t_wall)
+
+    def thermal_cavity_walls(self, t_hot, t_cold):
+        lib().orc_thermal_cavity_walls(self._h, t_hot, t_cold)
+
     def total_force(self):
         fx, fy = C.c_double(), C.c_double()
         lib().orc_total_force(self._h, C.byref(fx), C.byref(fy))
@@ -295,6 +323,10 @@ def _ref_lib(fast: bool) -> C.CDLL:
             getattr(L, name).restype = _dp
         L.ref_obstacle.restype = _u8p
         L.ref_mrt_collide.argtypes = [_dp] + [C.c_double] * 6
+        L.ref_set_thermal.argtypes = [C.c_int] + [C.c_double] * 4
+        L.ref_g.restype = _dp
+        L.ref_thermal_isothermal_solids.argtypes = [C.c_double]
+        L.ref_thermal_cavity_walls.argtypes = [C.c_double, C.c_double]
         L.ref_q.restype = C.c_double
         L.ref_q.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_int,
                             C.c_double, C.c_double, C.c_int]
@@ -364,6 +396,22 @@ class RefLattice:
 
     def set_f(self, f):
         self.f[...] = np.asarray(f, dtype=np.float64).reshape(self.ny, self.nx, 9)
+
+    @property
+    def g(self):
+        return _arr(self.L.ref_g(), self.n * 9).reshape(self.ny, self.nx, 9)
+
+    def set_thermal(self, omega_k, beta=0.0, g_buoyancy=0.0, t_ref=1.0, on=True):
+        self.L.ref_set_thermal(int(on), omega_k, beta, g_buoyancy, t_ref)
+
+    def set_g(self, g):
+        self.g[...] = np.asarray(g, dtype=np.float64).reshape(self.ny, self.nx, 9)
+
+    def thermal_isothermal_solids(self, t_wall):
+        self.L.ref_thermal_isothermal_solids(t_wall)
+
+    def thermal_cavity_walls(self, t_hot, t_cold):
+        self.L.ref_thermal_cavity_walls(t_hot, t_cold)
 
     def step(self, tau, u_in, n=1):
         self.L.ref_step(n, tau, u_in)

@@ -29,6 +29,10 @@
  *   mt19937 + uniform_real             10000th output of mt19937 seeded 5489 equals
  *                                      4123659995 (C++ standard [rand.predef]);
  *                                      golden fixture of the seed-42 perturbation
+ *   thermal DDF (collision, Boussinesq thermal equilibrium moments (sum = T, first
+ *   forcing, adiabatic streaming,      moment = T u); collision keeps T; closed domain
+ *   drivers' thermal wall loops)       keeps total heat; forcing adds exactly F per node
+ *                                      to the momentum; bitwise vs oracle/_ref
  */
 #include "lbm_oracle.h"
 
@@ -70,6 +74,11 @@ struct orc_lattice {
     double* wall_dist;
     double* fx_body;
     double* fy_body;
+    /* thermal double distribution (src/lbm_types.hpp:193-201) */
+    int use_thermal;
+    double* g;
+    double* g_next;
+    double t_ref, omega_k, beta, g_buoyancy;
 };
 
 int orc_cx(int i) { return CX[i]; }
@@ -249,6 +258,12 @@ orc_lattice* orc_create(int nx, int ny)
     L->wall_dist = (double*)calloc(n, sizeof(double));
     L->fx_body = (double*)calloc(n, sizeof(double));
     L->fy_body = (double*)calloc(n, sizeof(double));
+    /* g starts at 1.0 in every slot (LBMCapabilities ctor, src/lbm_types.hpp:215-216) */
+    L->g = (double*)malloc(n * 9 * sizeof(double));
+    L->g_next = (double*)malloc(n * 9 * sizeof(double));
+    for (size_t k = 0; k < n * 9; ++k) L->g[k] = L->g_next[k] = 1.0;
+    L->t_ref = 1.0;
+    L->omega_k = 1.0;
     return L;
 }
 
@@ -261,11 +276,69 @@ void orc_destroy(orc_lattice* L)
     free(L->wall_dist);
     free(L->fx_body);
     free(L->fy_body);
+    free(L->g);
+    free(L->g_next);
     free(L->poly);
     free(L);
 }
 
 int orc_nx(const orc_lattice* L) { return L->nx; }
+double* orc_g(orc_lattice* L) { return L->g; }
+
+void orc_set_thermal(orc_lattice* L, int on, double omega_k, double beta, double g_buoyancy, double t_ref)
+{
+    L->use_thermal = on;
+    L->omega_k = omega_k;
+    L->beta = beta;
+    L->g_buoyancy = g_buoyancy;
+    L->t_ref = t_ref;
+}
+
+/* thermal_equilibrium, src/thermal.hpp:24-30 (T_0 = 1). */
+double orc_thermal_equilibrium(int i, double T, double u, double v)
+{
+    double edotc = (double)CX[i] * u + (double)CY[i] * v;
+    double u2 = u * u + v * v;
+    return W[i] * T * (1.0 + 3.0 * edotc + 4.5 * edotc * edotc - 1.5 * u2);
+}
+
+/* The heated-cylinder driver's wall loop after every step (src/heated_cylinder.cpp:
+ * 155-164, apply_thermal_isothermal src/thermal.hpp:70-75): every solid node takes the
+ * thermal equilibrium at t_wall and rest. */
+void orc_thermal_isothermal_solids(orc_lattice* L, double t_wall)
+{
+    size_t n = (size_t)L->nx * L->ny;
+    for (size_t c = 0; c < n; ++c) {
+        if (!L->obstacle[c]) continue;
+        for (int i = 0; i < 9; ++i) L->g[c * 9 + i] = orc_thermal_equilibrium(i, t_wall, 0.0, 0.0);
+    }
+}
+
+/* The thermal-cavity driver's loop after every step (src/thermal_cavity.cpp:153-188):
+ * fluid columns x = 1 and x = nx-2 (rows 1..ny-2) take the thermal equilibrium at t_hot
+ * / t_cold with the node's own velocity; then rows y = 1 and y = ny-2 (columns
+ * 1..nx-2) copy g[opp(i)] into g[i] in place, in slot order. */
+void orc_thermal_cavity_walls(orc_lattice* L, double t_hot, double t_cold)
+{
+    const int nx = L->nx, ny = L->ny;
+    for (int y = 1; y < ny - 1; ++y) {
+        const int cols[2] = {1, nx - 2};
+        const double temps[2] = {t_hot, t_cold};
+        for (int k = 0; k < 2; ++k) {
+            size_t c = (size_t)y * nx + cols[k];
+            double rho, u, v;
+            orc_macros(&L->f[c * 9], &rho, &u, &v);
+            for (int i = 0; i < 9; ++i) L->g[c * 9 + i] = orc_thermal_equilibrium(i, temps[k], u, v);
+        }
+    }
+    for (int x = 1; x < nx - 1; ++x) {
+        const int rows[2] = {1, ny - 2};
+        for (int k = 0; k < 2; ++k) {
+            double* gc = &L->g[((size_t)rows[k] * nx + x) * 9];
+            for (int i = 0; i < 9; ++i) gc[i] = gc[OPP[i]];
+        }
+    }
+}
 int orc_ny(const orc_lattice* L) { return L->ny; }
 double* orc_f(orc_lattice* L) { return L->f; }
 uint8_t* orc_obstacle(orc_lattice* L) { return L->obstacle; }
@@ -480,8 +553,7 @@ static int has_geometry(const orc_lattice* L) { return L->g_radius > 0.0 || L->i
 
 /* ---------------------------------------------------------------- time step */
 
-/* execute_time_step, src/lbm.hpp:293-582 (thermal hooks excluded: the thermal
- * DDF stays outside this path). */
+/* execute_time_step, src/lbm.hpp:293-582. */
 void orc_step(orc_lattice* L, double tau, double u_in)
 {
     const int nx = L->nx, ny = L->ny;
@@ -542,8 +614,43 @@ void orc_step(orc_lattice* L, double tau, double u_in)
             }
     }
 
-    /* zero f_next, src/lbm.hpp:400 */
+    /* thermal collision and Boussinesq forcing, src/lbm.hpp:356-397: macros of the
+     * post-collision f, T of the pre-collision g (compute_temperature, src/thermal.hpp:
+     * 55-60), thermal_collide (src/thermal.hpp:35-41), boussinesq_force (:45-52) with
+     * g = (0, g_buoyancy), added to f as w_i (3 F.e_i + 9 (e_i.u)(e_i.F)) */
+    if (L->use_thermal) {
+        const double rho_0 = 1.0;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                size_t c = (size_t)y * nx + x;
+                if (obs[c]) continue;
+                double* fc = &f[c * 9];
+                double* gc = &L->g[c * 9];
+                double rho, u, v;
+                orc_macros(fc, &rho, &u, &v);
+                double T = 0.0;
+                for (int i = 0; i < 9; ++i) T += gc[i];
+                for (int i = 0; i < 9; ++i) {
+                    double g_eq = orc_thermal_equilibrium(i, T, u, v);
+                    gc[i] -= L->omega_k * (gc[i] - g_eq);
+                }
+                if (L->beta > 0.0 && L->g_buoyancy != 0.0) {
+                    double dT = T - L->t_ref;
+                    double fxb = -rho_0 * L->beta * dT * 0.0;
+                    double fyb = -rho_0 * L->beta * dT * L->g_buoyancy;
+                    for (int i = 0; i < 9; ++i) {
+                        double edotu = (double)CX[i] * u + (double)CY[i] * v;
+                        double Fdot = (double)CX[i] * fxb + (double)CY[i] * fyb;
+                        fc[i] += W[i] * (3.0 * Fdot + 9.0 * edotu * ((double)CX[i] * fxb + (double)CY[i] * fyb));
+                    }
+                }
+            }
+    }
+
+    /* zero f_next (and g_next), src/lbm.hpp:400-403 */
     memset(fn, 0, sizeof(double) * n * 9);
+    if (L->use_thermal) memset(L->g_next, 0, sizeof(double) * n * 9);
 
     /* streaming + bounce-back, src/lbm.hpp:406-484 */
     const int interp = has_geometry(L);
@@ -592,6 +699,33 @@ void orc_step(orc_lattice* L, double tau, double u_in)
     L->f = fn;
     L->f_next = f;
     f = L->f;
+
+    /* thermal streaming, src/lbm.hpp:489-528: y periodic (x too for ribs / hills); a
+     * population leaving the lattice or hitting a solid node bounces back into the
+     * opposite slot of its own node (adiabatic) */
+    if (L->use_thermal) {
+        const double* g = L->g;
+        double* gn = L->g_next;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                size_t c = (size_t)y * nx + x;
+                if (obs[c]) continue;
+                for (int i = 0; i < 9; ++i) {
+                    int tx = x + CX[i], ty = y + CY[i];
+                    if (is_periodic_x_case(cse)) {
+                        if (tx < 0) tx += nx;
+                        if (tx >= nx) tx -= nx;
+                    }
+                    if (ty < 0) ty += ny;
+                    if (ty >= ny) ty -= ny;
+                    if (tx < 0 || tx >= nx || obs[(size_t)ty * nx + tx]) gn[c * 9 + OPP[i]] = g[c * 9 + i];
+                    else gn[((size_t)ty * nx + tx) * 9 + i] = g[c * 9 + i];
+                }
+            }
+        L->g = gn;
+        L->g_next = (double*)g;
+    }
 
     /* open boundaries, src/lbm.hpp:530-539 */
     if (cse == ORC_CAVITY) {

@@ -91,6 +91,15 @@ void   orc_total_force(const orc_lattice* L, double* fx, double* fy);
 /* FP64 mass over non-solid nodes (src/lbm_test.cpp:173-180). */
 double orc_mass(const orc_lattice* L);
 
+/* ---- thermal double distribution (src/lbm.hpp:356-397, 489-528; src/thermal.hpp) ---- */
+double* orc_g(orc_lattice* L);           /* thermal populations, AoS n*9 (start at 1.0) */
+void   orc_set_thermal(orc_lattice* L, int on, double omega_k, double beta, double g_buoyancy, double t_ref);
+double orc_thermal_equilibrium(int i, double T, double u, double v);
+/* The thermal drivers' loops run after each step (src/heated_cylinder.cpp:155-164,
+ * src/thermal_cavity.cpp:153-188). */
+void   orc_thermal_isothermal_solids(orc_lattice* L, double t_wall);
+void   orc_thermal_cavity_walls(orc_lattice* L, double t_hot, double t_cold);
+
 /* Wall distance: reference relaxation sweeps (src/lbm_types.hpp:231-264) and BFS. */
 void orc_wall_distance_sweep(const uint8_t* obstacle, int nx, int ny, double* dist);
 void orc_wall_distance_bfs(const uint8_t* obstacle, int nx, int ny, double* dist);

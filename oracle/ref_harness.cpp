@@ -102,6 +102,49 @@ double* ref_fx_body() { return g_sys->fx_body.data(); }
 double* ref_fy_body() { return g_sys->fy_body.data(); }
 double* ref_wall_dist() { return g_sys->wall_dist.data(); }
 
+// Thermal double distribution: the LBMCapabilities fields (src/lbm_types.hpp:193-201).
+void ref_set_thermal(int on, double omega_k, double beta, double g_buoyancy, double t_ref)
+{
+    g_sys->use_thermal = on != 0;
+    g_sys->omega_k = omega_k;
+    g_sys->beta = beta;
+    g_sys->g_buoyancy = g_buoyancy;
+    g_sys->T_ref = t_ref;
+}
+
+double* ref_g() { return g_sys->g_thermal.data(); }
+
+// The heated-cylinder driver's wall loop (src/heated_cylinder.cpp:155-164), through
+// apply_thermal_isothermal.
+void ref_thermal_isothermal_solids(double t_wall)
+{
+    for (int n = 0; n < NX * NY; ++n)
+        if (g_sys->obstacle[n]) apply_thermal_isothermal(&g_sys->g_thermal[n * 9], t_wall);
+}
+
+// The thermal-cavity driver's loop after each step (src/thermal_cavity.cpp:153-188),
+// through compute_macros, thermal_equilibrium and bounce_back.
+void ref_thermal_cavity_walls(double t_hot, double t_cold)
+{
+    for (int y = 1; y < NY - 1; ++y) {
+        const int col[2] = {1, NX - 2};
+        const double t[2] = {t_hot, t_cold};
+        for (int k = 0; k < 2; ++k) {
+            const int n = node_index(col[k], y);
+            double rho, u, v;
+            compute_macros(&g_sys->f[n * 9], rho, u, v);
+            for (int i = 0; i < 9; ++i) g_sys->g_thermal[n * 9 + i] = thermal_equilibrium(i, t[k], u, v);
+        }
+    }
+    for (int x = 1; x < NX - 1; ++x) {
+        const int row[2] = {1, NY - 2};
+        for (int k = 0; k < 2; ++k) {
+            double* g = &g_sys->g_thermal[node_index(x, row[k]) * 9];
+            for (int i = 0; i < 9; ++i) g[i] = g[bounce_back[i]];
+        }
+    }
+}
+
 void ref_mrt_collide(double* f9, double rho, double u, double v, double tau, double cs_sq, double y_plus)
 {
     MRTParams p = MRTParams::from_tau(tau);

@@ -167,3 +167,120 @@ def build_gpu(c: Case, storage: str, o=None, scale_exp: int = 8):
     apply_gpu_geometry(ens, 0, c, o.obstacle.copy())
     ens.upload(0, o.f.copy())
     return ens, o
+
+
+# ---- thermal double distribution ----------------------------------------------------
+
+@dataclasses.dataclass
+class ThermalCase:
+    """A momentum case with the thermal populations on (src/lbm.hpp:356-397, 489-528).
+
+    t_field(x, y) is the initial temperature of fluid nodes (g = thermal equilibrium at
+    the node's initial velocity); solid nodes start as the drivers leave them before their
+    loop.  walls is the driver's loop after each step: ("solids", t_wall) as
+    src/heated_cylinder.cpp:155-164, ("cavity", t_hot, t_cold) as
+    src/thermal_cavity.cpp:153-188, or None."""
+    name: str
+    base: Case
+    omega_k: float
+    beta: float
+    g_buoyancy: float
+    t_ref: float
+    t_field: Callable
+    walls: Optional[tuple] = None
+    steps: int = 60
+
+
+def _blob(x0, y0, amp, width):
+    return lambda x, y: 1.0 + amp * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / width)
+
+
+def thermal_eq(T, u, v):
+    """Thermal equilibrium of every slot, (..., 9) (src/thermal.hpp:24-30)."""
+    cx = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], float)
+    cy = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], float)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    T, u, v = (np.asarray(a, float)[..., None] for a in (T, u, v))
+    eu = cx * u + cy * v
+    return w * T * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * (u * u + v * v))
+
+
+def _alpha(tau):
+    return (tau - 0.5) / 3.0 / 0.71  # Pr = 0.71 (src/heated_cylinder.cpp:103)
+
+
+def _cavity_walls(nx, ny):
+    m = np.zeros((ny, nx), np.uint8)
+    m[:, 0] = m[:, nx - 1] = 1
+    m[0, :] = m[ny - 1, :] = 1
+    return m
+
+
+CAVW = _cavity_walls(32, 32)
+_RA, _TC_TAU = 1e5, 0.8
+_GB = _RA * ((_TC_TAU - 0.5) / 3.0) * _alpha(_TC_TAU) / (1.0 * 30.0 ** 3)
+
+THERMAL_CASES = {
+    # heated cylinder with a warm blob and strong buoyancy (src/heated_cylinder.cpp)
+    "heated": ThermalCase("heated", CASES["cylinder"], 1.0 / (0.5 + 3.0 * _alpha(0.56)), 0.01, -1e-3, 1.0,
+                          _blob(45, 20, 0.5, 60.0), walls=("solids", 1.5)),
+    "heated_les": ThermalCase("heated_les", CASES["les"], 1.0 / (0.5 + 3.0 * _alpha(0.52)), 0.01, -1e-3, 1.0,
+                              _blob(40, 30, 0.5, 40.0), walls=("solids", 1.5), steps=40),
+    "heated_bgk": ThermalCase("heated_bgk", CASES["bgk"], 1.0 / (0.5 + 3.0 * _alpha(0.7)), 0.0, 0.0, 1.0,
+                              _blob(40, 24, 0.3, 80.0), walls=("solids", 1.2), steps=40),
+    # natural convection in a walled box on the default (cylinder) case, no inflow
+    # (src/thermal_cavity.cpp: Ra = 1e5, tau = 0.8, beta = 0.001)
+    "convection": ThermalCase("convection",
+                              Case("convection", 32, 32, O.CYLINDER, _TC_TAU, 0.0, 200, _mask_setup(CAVW),
+                                   init=(1.0, 0.0, 0.0)),
+                              1.0 / (0.5 + 3.0 * _alpha(_TC_TAU)), 0.001, -_GB / 0.001, 1.0,
+                              lambda x, y: np.ones_like(x, float), walls=("cavity", 1.5, 0.5), steps=200),
+    # lid-driven cavity: momentum walls bounce, the thermal populations wrap in y
+    "lid": ThermalCase("lid", CASES["cavity"], 1.4, 0.005, -2e-3, 1.0, _blob(16, 12, 0.4, 20.0), steps=80),
+    # periodic hills: the thermal populations wrap in x too
+    "hills": ThermalCase("hills", CASES["hills"], 1.2, 0.01, -1e-3, 1.0, _blob(30, 26, 0.5, 30.0)),
+}
+
+
+def thermal_initial_g(tc: ThermalCase, obstacle: np.ndarray) -> np.ndarray:
+    c = tc.base
+    y, x = np.mgrid[0:c.ny, 0:c.nx].astype(float)
+    rho0, u0, v0 = c.init
+    g = thermal_eq(tc.t_field(x, y), np.full(x.shape, u0), np.full(x.shape, v0))
+    solid = obstacle != 0
+    if tc.walls and tc.walls[0] == "solids":
+        g[solid] = thermal_eq(tc.walls[1], 0.0, 0.0)
+    elif tc.walls and tc.walls[0] == "cavity":
+        # the driver's setup before its loop (src/thermal_cavity.cpp:127-144): hot / cold
+        # side walls, top and bottom walls mirrored in place in slot order
+        th, tcold = tc.walls[1], tc.walls[2]
+        g[:, 0] = thermal_eq(th, 0.0, 0.0)
+        g[:, c.nx - 1] = thermal_eq(tcold, 0.0, 0.0)
+        opp = [0, 3, 4, 1, 2, 7, 8, 5, 6]
+        for yy in (0, c.ny - 1):
+            for xx in range(1, c.nx - 1):
+                for i in range(9):
+                    g[yy, xx, i] = g[yy, xx, opp[i]]
+    return g
+
+
+def build_thermal_oracle(tc: ThermalCase, cls=O.Oracle):
+    o = build_oracle(tc.base, cls=cls)
+    o.set_thermal(tc.omega_k, tc.beta, tc.g_buoyancy, tc.t_ref)
+    o.set_g(thermal_initial_g(tc, o.obstacle))
+    return o
+
+
+def thermal_walls(o, tc: ThermalCase):
+    """The driver's loop after a step, on an Oracle or RefLattice."""
+    if not tc.walls:
+        return
+    if tc.walls[0] == "solids":
+        o.thermal_isothermal_solids(tc.walls[1])
+    else:
+        o.thermal_cavity_walls(tc.walls[1], tc.walls[2])
+
+
+def thermal_step(o, tc: ThermalCase):
+    o.step(tc.base.tau, tc.base.u)
+    thermal_walls(o, tc)

@@ -6,7 +6,8 @@ reference's execute_time_step:
   - the total force (sum of fx_body, fy_body in node order) after every step,
   - the FP64 mass over fluid nodes after the last step,
   - every 4th row of the final populations and per-direction column sums and
-    sums of squares of all of them.
+    sums of squares of all of them;
+for every thermal case, per-step forces and total heat and the final f and g rows;
 plus the cylinder-wake lattice at full size (800x300, the driver's (200,151)
 circle and the (200,150) one) and the 256x256 cavity over a short horizon.
 
@@ -22,7 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "tests"), os.path.join(ROOT, "lbm-2d_amd")]
 
 import oracle as O  # noqa: E402
-from cases import CASES, build_oracle  # noqa: E402
+from cases import CASES, THERMAL_CASES, build_oracle, build_thermal_oracle, thermal_step  # noqa: E402
 
 
 def record(obj, steps, tau, u):
@@ -42,6 +43,22 @@ def record(obj, steps, tau, u):
     }
 
 
+def record_thermal(obj, tc):
+    """Per-step total force and heat (sum of g over every node) with the driver's wall
+    loop after each step; every 4th row of the final f and g and their column sums."""
+    fx, fy, heat = [], [], []
+    for _ in range(tc.steps):
+        thermal_step(obj, tc)
+        a, b = obj.total_force()
+        fx.append(a)
+        fy.append(b)
+        heat.append(float(np.cumsum(obj.g.ravel())[-1]))
+    f, g = obj.f.copy(), obj.g.copy()
+    return {"fx": np.array(fx), "fy": np.array(fy), "heat": np.array(heat),
+            "rows": f[::4].copy(), "grows": g[::4].copy(),
+            "colsum": f.sum(axis=(0, 1)), "gcolsum": g.sum(axis=(0, 1))}
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libref_check.so is missing: run `make -C oracle` where the reference is present")
@@ -50,6 +67,11 @@ def main():
         rec = record(r, c.steps, c.tau, c.u)
         np.savez_compressed(os.path.join(HERE, f"case_{name}.npz"), steps=c.steps, **rec)
         print(name, c.steps, rec["fx"][-1], rec["mass"])
+    for name, tc in sorted(THERMAL_CASES.items()):
+        r = build_thermal_oracle(tc, cls=O.RefLattice)
+        rec = record_thermal(r, tc)
+        np.savez_compressed(os.path.join(HERE, f"thermal_{name}.npz"), steps=tc.steps, **rec)
+        print("thermal", name, tc.steps, rec["heat"][-1])
     # full-size cylinder wake (configs' 800x300, tau = 0.68), 20 steps, both centres
     for tag, cy in (("driver", 151), ("spec", 150)):
         r = O.RefLattice(800, 300, O.CYLINDER)
