@@ -198,6 +198,45 @@ int lbm_get_info(lbm_handle h, lbm_info* out);
 /* Waits for all work of the handle. */
 int lbm_sync(lbm_handle h);
 
+/* ---- Thermal double distribution (src/lbm.hpp:356-397, 489-528; src/thermal.hpp:16-60).
+ * Nine more populations g per node carry the temperature T = sum g_i; they replace
+ * LBMCapabilities::use_thermal, g_thermal, T_ref, omega_k, beta and g_buoyancy
+ * (src/lbm_types.hpp:193-201).  In each step, after the momentum collision, g relaxes at
+ * omega_k to the thermal equilibrium w_i T (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2) with
+ * the velocity of the post-collision f, and, when beta > 0 and g_buoyancy != 0, f gains
+ * w_i (3 F.e_i + 9 (e_i.u)(e_i.F)) with F = (0, -beta (T - t_ref) g_buoyancy) and T
+ * taken before the thermal collision; g then streams periodic in y (and in x on a
+ * periodic x edge) and bounces back into its own node's opposite slot off the other x
+ * edges and off solid nodes, which receive no g.  One-lattice handles, F64 or F32. */
+/* Turns the thermal populations on (g = 1.0 in every slot, as the reference's
+ * constructor) or changes their parameters.  Turning them on needs a lattice that has
+ * not stepped since its last upload. */
+int lbm_set_thermal(lbm_handle h, int32_t lattice, double omega_k, double beta, double g_buoyancy,
+                    double t_ref);
+/* Per-node conditions on the thermal state after every step: what the thermal drivers
+ * do on the host between steps (src/heated_cylinder.cpp:155-164,
+ * src/thermal_cavity.cpp:153-188).  code[y*nx + x]: low 4 bits k > 0 set g to the
+ * thermal equilibrium at rules[k-1].t with the node's own velocity (LBM_THERMAL_EQ_FLOW)
+ * or at rest (LBM_THERMAL_EQ_REST); bit LBM_THERMAL_MIRROR then sets g[i] = g[opp(i)]
+ * in slot order, in place.  code null: none.  The conditions do not touch an uploaded
+ * state; they act from the first step on. */
+#define LBM_THERMAL_MIRROR 16
+enum lbm_thermal_kind { LBM_THERMAL_EQ_FLOW = 0, LBM_THERMAL_EQ_REST = 1 };
+typedef struct lbm_thermal_rule {
+    double t;
+    int32_t kind;
+    int32_t pad;
+} lbm_thermal_rule;
+int lbm_set_thermal_rules(lbm_handle h, int32_t lattice, const uint8_t* code, const lbm_thermal_rule* rules,
+                          int32_t n_rules);
+/* Thermal state in the reference layout (AoS FP64 like f).  lbm_upload and
+ * lbm_init_equilibrium reset g to 1.0; lbm_upload_thermal follows them, before any step. */
+int lbm_upload_thermal(lbm_handle h, int32_t lattice, const double* g_aos);
+int lbm_download_thermal(lbm_handle h, int32_t lattice, double* g_aos);
+/* Temperature sum_i g_i (compute_temperature, src/thermal.hpp:55-60) of rows [y0, y1),
+ * nx*(y1-y0) FP64, computed on the device. */
+int lbm_temperature(lbm_handle h, int32_t lattice, int32_t y0, int32_t y1, double* t);
+
 /* Wall distance used by the LES (src/lbm_types.hpp:231-264), computed once per
  * geometry: out[nx*ny] in lattice units, 1e9 where no solid is reachable. */
 int lbm_wall_distance(lbm_handle h, int32_t lattice, double* out);

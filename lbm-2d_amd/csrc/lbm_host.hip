@@ -173,6 +173,15 @@ struct Ensemble {
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // by the starting buffer parity (one lattice)
     bool graph_dirty = true;
     int64_t device_bytes = 0;
+    // thermal double distribution (one-lattice handles, F64 / F32 storage): g planes laid
+    // out as the f planes, the same buffer parity; per-node conditions and their rules
+    bool thermal = false;
+    void* gbuf[2] = {nullptr, nullptr};
+    uint8_t* d_tcode = nullptr;
+    bool has_rules = false;
+    double omega_k = 1.0, beta = 0.0, g_buoyancy = 0.0, t_ref = 1.0;
+    double rule_t[kThermalRules] = {};
+    uint32_t rule_rest = 0;
 };
 
 namespace {
@@ -682,7 +691,7 @@ bool one_lattice(Ensemble* E, int l) { return l >= 0 || E->lat.size() == 1; }
 // the batch always ends with a one-step launch, so the last stored C precedes the state.
 int two_step_pairs(Ensemble* E, int n, int l)
 {
-    if (!one_lattice(E, l) || !(E->options & LAUNCH_TWO_STEPS) || n < 2) return 0;
+    if (E->thermal || !one_lattice(E, l) || !(E->options & LAUNCH_TWO_STEPS) || n < 2) return 0;
     return (n - 1) / 2;
 }
 
@@ -702,6 +711,25 @@ int two_step_ok(Ensemble* E, int bx, int rows, int vec)
 // Buffer swaps of a batch: one per launch.
 int batch_flips(Ensemble* E, int n, int l) { return n - two_step_pairs(E, n, l); }
 
+// Thermal kernel arguments from the handle's parameters and node conditions.
+ThermalArgs thermal_args(Ensemble* E, int mode, int from_state, int y0, int y1, double* out)
+{
+    ThermalArgs t{};
+    t.code = E->has_rules ? E->d_tcode : nullptr;
+    for (int k = 0; k < kThermalRules; ++k) t.rule_t[k] = E->rule_t[k];
+    t.rule_rest = E->rule_rest;
+    t.mode = mode;
+    t.from_state = from_state;
+    t.y0 = y0;
+    t.y1 = y1;
+    t.omega_k = E->omega_k;
+    t.beta = E->beta;
+    t.g_buoyancy = E->g_buoyancy;
+    t.t_ref = E->t_ref;
+    t.out = out;
+    return t;
+}
+
 // n <= acc_blocks steps over lattice l (l < 0: every lattice), then the force flush
 // and the counter update.  One-lattice launches take their buffers from the host's
 // parity; graph capture records them for that starting parity.
@@ -712,12 +740,26 @@ int launch_batch(Ensemble* E, int n, int l)
     const bool single = grid_for(E, l, a, blocks);
     const int li = l >= 0 ? l : 0;
     const int32_t par = E->lat[li].par;
-    const bool two = single && (E->options & LAUNCH_TWO_STEPS);
+    const bool two = !E->thermal && single && (E->options & LAUNCH_TWO_STEPS);
     const int pairs = two_step_pairs(E, n, l);
     auto block_of = [&](int i) { return E->d_acc + static_cast<int64_t>(i) * kAccCopies * E->total_slots * 2; };
     auto terms_of = [&](int i) { return E->d_nf + static_cast<int64_t>(i) * E->n_recs * 2; };
     int swaps = 0;
-    if (two) {
+    if (E->thermal) {
+        // one launch of the thermal kernel per step (one lattice; f and g share the parity)
+        for (int i = 0; i < n; ++i, ++swaps) {
+            const int sp = par ^ (i & 1);
+            a.src = E->buf[sp];
+            a.dst = E->buf[1 - sp];
+            a.acc = block_of(i);
+            a.nf = terms_of(i);
+            a.step_arg = i;
+            ThermalArgs t = thermal_args(E, THERMAL_STEP, 0, 0, E->lat[0].d.ny, nullptr);
+            t.gsrc = E->gbuf[sp];
+            t.gdst = E->gbuf[1 - sp];
+            LBM_HIP(launch_thermal(E->storage, a, t, E->stream));
+        }
+    } else if (two) {
         // two-step launches on 64 x rows tiles, then one-step launches of shape (64, 1, 1)
         const lbm_desc& d = E->lat[li].d;
         StepArgs a2 = a;
@@ -749,7 +791,7 @@ int launch_batch(Ensemble* E, int n, int l)
             LBM_HIP(launch_step(E->storage, 64, 1, 1, true, blocks1, a1, E->stream));
         }
     }
-    for (int i = 0; i < n && !two; ++i, ++swaps) {
+    for (int i = 0; i < n && !two && !E->thermal; ++i, ++swaps) {
         a.acc = block_of(i);
         a.nf = terms_of(i);
         a.step_arg = i;
@@ -901,6 +943,91 @@ int fields(Ensemble* E, int l, int y0, int y1, double* rho, double* u, double* v
         for (int k = 0; k < 3; ++k)
             if (dst[k] && (rc = d2h(E, dst[k] + off, E->d_stage + k * n, n * 8)) != LBM_OK) return rc;
     }
+    return LBM_OK;
+}
+
+// Thermal state rows of lattice 0 from AoS FP64 g (null: 1.0 in every slot) into the
+// state planes gbuf[1 - par].
+int thermal_load(Ensemble* E, const double* g)
+{
+    HostLattice& H = E->lat[0];
+    const lbm_desc& d = H.d;
+    const int64_t row_bytes = static_cast<int64_t>(d.nx) * 72;
+    const int chunk = g ? static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(d.ny, E->stage_bytes / row_bytes)))
+                        : d.ny;
+    for (int y0 = 0; y0 < d.ny; y0 += chunk) {
+        const int y1 = std::min(d.ny, y0 + chunk);
+        int rc;
+        if (g && (rc = h2d(E, E->d_stage, g + static_cast<int64_t>(y0) * d.nx * 9, (y1 - y0) * row_bytes)) != LBM_OK)
+            return rc;
+        ThermalLoadArgs t{};
+        t.lat = E->d_lat;
+        t.lattice = 0;
+        t.y0 = y0;
+        t.y1 = y1;
+        t.in = g ? E->d_stage : nullptr;
+        t.state_dst = E->gbuf[1 - H.par];
+        LBM_HIP(launch_thermal_load(E->storage, t, d.nx, E->stream));
+    }
+    LBM_HIP(hipStreamSynchronize(E->stream));
+    return LBM_OK;
+}
+
+// Rewrites the stored post-collision f and g of lattice 0 from its current state (after
+// an upload, or a change of the parameters or node conditions): the pull form keeps
+// the next step's collisions, done with the old values.
+int thermal_recollide(Ensemble* E)
+{
+    int rc;
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    HostLattice& H = E->lat[0];
+    StepArgs a = base_args(E);
+    int blocks;
+    grid_for(E, 0, a, blocks);
+    a.src = E->buf[1 - H.par];
+    a.dst = E->buf[H.par];
+    a.nf = nullptr;
+    ThermalArgs t = thermal_args(E, THERMAL_COLLIDE, H.steps == 0 ? 1 : 0, 0, H.d.ny, nullptr);
+    t.gsrc = E->gbuf[1 - H.par];
+    t.gdst = E->gbuf[H.par];
+    LBM_HIP(launch_thermal(E->storage, a, t, E->stream));
+    LBM_HIP(hipStreamSynchronize(E->stream));
+    return LBM_OK;
+}
+
+// Rows [y0, y1) of lattice 0's thermal state: AoS FP64 g (mode THERMAL_OUT_G) or T
+// (THERMAL_OUT_T) into out, in chunks through the staging buffer.
+int thermal_out(Ensemble* E, int mode, int y0, int y1, double* out)
+{
+    int rc;
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    HostLattice& H = E->lat[0];
+    const int per_node = mode == THERMAL_OUT_G ? 9 : 1;
+    const int64_t row_bytes = static_cast<int64_t>(H.d.nx) * per_node * 8;
+    const int chunk = static_cast<int>(std::min<int64_t>(y1 - y0, E->stage_bytes / row_bytes));
+    if (chunk < 1) return fail(LBM_ERR_INVALID, "staging buffer smaller than one row");
+    for (int c0 = y0; c0 < y1; c0 += chunk) {
+        const int c1 = std::min(y1, c0 + chunk);
+        StepArgs a = base_args(E);
+        int blocks;
+        grid_for(E, 0, a, blocks);
+        a.src = E->buf[1 - H.par];
+        a.nf = nullptr;
+        ThermalArgs t = thermal_args(E, mode, H.steps == 0 ? 1 : 0, c0, c1, E->d_stage);
+        t.gsrc = E->gbuf[1 - H.par];
+        LBM_HIP(launch_thermal(E->storage, a, t, E->stream));
+        rc = d2h(E, out + static_cast<int64_t>(c0 - y0) * H.d.nx * per_node, E->d_stage,
+                 static_cast<size_t>(c1 - c0) * row_bytes);
+        if (rc != LBM_OK) return rc;
+    }
+    return LBM_OK;
+}
+
+int thermal_ok(Ensemble* E, int l)
+{
+    int rc = lattice_ok(E, l);
+    if (rc) return rc;
+    if (!E->thermal) return fail(LBM_ERR_INVALID, "the thermal populations are off (lbm_set_thermal)");
     return LBM_OK;
 }
 
@@ -1058,7 +1185,7 @@ int lbm_destroy(lbm_handle h)
     void* ptrs[] = {E->buf[0], E->buf[1], E->d_flags, E->d_lslot, E->d_wdist, E->d_links, E->d_lat,
                     E->d_tiles, E->d_steps, E->d_par, E->d_acc, E->d_hist, E->d_dummy_acc, E->d_nan,
                     E->d_dummy_nan, E->d_mass, E->d_stage, E->d_slot_lat, E->d_special, E->d_nf,
-                    E->d_dummy_nf, E->d_slot_start, E->d_slot_recs};
+                    E->d_dummy_nf, E->d_slot_start, E->d_slot_recs, E->gbuf[0], E->gbuf[1], E->d_tcode};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
@@ -1206,7 +1333,8 @@ int lbm_set_params(lbm_handle h, int32_t l, double tau, double u_in)
         // produced with the old inlet velocity, so that one changes afterwards.
         H.d.tau = tau;
         h->desc_dirty = true;
-        if ((rc = lbm::materialize(h, l, nullptr, nullptr, true)) != LBM_OK) return rc;
+        rc = h->thermal ? lbm::thermal_recollide(h) : lbm::materialize(h, l, nullptr, nullptr, true);
+        if (rc != LBM_OK) return rc;
     }
     if (H.d.tau != tau || H.d.u_in != u_in) {
         H.d.tau = tau;
@@ -1260,6 +1388,12 @@ int load_state(Ensemble* E, int l, const double* f, double rho, double u, double
     const int32_t zero = 0;
     if ((rc = h2d(E, E->d_nan + l, &zero, 4)) != LBM_OK) return rc;
     E->counters_stale = true;
+    if (E->thermal) {
+        // g returns to the reference's initial 1.0 (src/lbm_types.hpp:215): an upload of
+        // g follows; the post-collision f then carries the forcing of that g
+        if ((rc = thermal_load(E, nullptr)) != LBM_OK) return rc;
+        if ((rc = thermal_recollide(E)) != LBM_OK) return rc;
+    }
     return LBM_OK;
 }
 }  // namespace
@@ -1402,6 +1536,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     if (!h) return fail(LBM_ERR_INVALID, "null handle");
     Ensemble* E = h;
     int rc;
+    if (E->thermal) return fail(LBM_ERR_INVALID, "a thermal lattice has one launch shape: nothing to tune");
     if ((rc = require_uploaded(E)) != LBM_OK) return rc;
     if ((rc = commit(E)) != LBM_OK) return rc;
     void* scratch = nullptr;
@@ -1588,6 +1723,116 @@ int lbm_sync(lbm_handle h)
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail(LBM_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
     return LBM_OK;
+}
+
+int lbm_set_thermal(lbm_handle h, int32_t l, double omega_k, double beta, double g_buoyancy, double t_ref)
+{
+    using namespace lbm;
+    int rc = lattice_ok(h, l);
+    if (rc) return rc;
+    Ensemble* E = h;
+    if (E->lat.size() != 1) return fail(LBM_ERR_INVALID, "the thermal populations need a handle of one lattice");
+    if (E->storage != ST_F64 && E->storage != ST_F32)
+        return fail(LBM_ERR_INVALID, "the thermal populations need F64 or F32 storage");
+    if (!std::isfinite(omega_k) || !std::isfinite(beta) || !std::isfinite(g_buoyancy) || !std::isfinite(t_ref))
+        return fail(LBM_ERR_INVALID, "thermal parameters must be finite");
+    HostLattice& H = E->lat[0];
+    const bool first = !E->thermal;
+    if (first) {
+        if (H.uploaded && H.steps > 0)
+            return fail(LBM_ERR_INVALID, "turn the thermal populations on before stepping (or upload again)");
+        const int64_t guard = H.pitch + 256;
+        for (int b = 0; b < 2; ++b) {
+            hipError_t e = hipMalloc(&E->gbuf[b], static_cast<size_t>(E->pool_elems + guard) * E->bpp);
+            if (e != hipSuccess) return fail(LBM_ERR_NOMEM, "thermal buffers: " + std::string(hipGetErrorString(e)));
+            E->device_bytes += (E->pool_elems + guard) * E->bpp;
+        }
+        if ((rc = dev_alloc(E, &E->d_tcode, E->cell_elems)) != LBM_OK) return rc;
+        E->thermal = true;
+    }
+    E->omega_k = omega_k;
+    E->beta = beta;
+    E->g_buoyancy = g_buoyancy;
+    E->t_ref = t_ref;
+    E->graph_dirty = true;
+    if (H.uploaded) {
+        if (first && (rc = thermal_load(E, nullptr)) != LBM_OK) return rc;
+        return thermal_recollide(E);
+    }
+    return LBM_OK;
+}
+
+int lbm_set_thermal_rules(lbm_handle h, int32_t l, const uint8_t* code, const lbm_thermal_rule* rules,
+                          int32_t n_rules)
+{
+    using namespace lbm;
+    int rc = thermal_ok(h, l);
+    if (rc) return rc;
+    Ensemble* E = h;
+    if (n_rules < 0 || n_rules > kThermalRules) return fail(LBM_ERR_INVALID, "at most 15 thermal rules");
+    if (n_rules > 0 && !rules) return fail(LBM_ERR_INVALID, "null rule array");
+    for (int k = 0; k < n_rules; ++k) {
+        if (rules[k].kind != LBM_THERMAL_EQ_FLOW && rules[k].kind != LBM_THERMAL_EQ_REST)
+            return fail(LBM_ERR_INVALID, "unknown thermal rule kind");
+        if (!std::isfinite(rules[k].t)) return fail(LBM_ERR_INVALID, "rule temperature must be finite");
+    }
+    HostLattice& H = E->lat[0];
+    if (code) {
+        std::vector<uint8_t> pc(E->cell_elems, 0);
+        for (int y = 0; y < H.d.ny; ++y)
+            for (int x = 0; x < H.d.nx; ++x) {
+                const uint8_t c = code[static_cast<int64_t>(y) * H.d.nx + x];
+                if ((c & ~(15u | LBM_THERMAL_MIRROR)) || (c & 15) > n_rules)
+                    return fail(LBM_ERR_INVALID, "node condition code names no rule");
+                pc[H.cell_off + static_cast<int64_t>(y) * H.pitch + x] = c;
+            }
+        if ((rc = h2d(E, E->d_tcode, pc.data(), pc.size())) != LBM_OK) return rc;
+    }
+    E->has_rules = code != nullptr;
+    E->rule_rest = 0;
+    for (int k = 0; k < kThermalRules; ++k) {
+        E->rule_t[k] = k < n_rules ? rules[k].t : 0.0;
+        if (k < n_rules && rules[k].kind == LBM_THERMAL_EQ_REST) E->rule_rest |= 1u << k;
+    }
+    E->graph_dirty = true;
+    // the conditions act on the state after a step: the current one changes with them
+    if (H.uploaded && H.steps > 0) return thermal_recollide(E);
+    return LBM_OK;
+}
+
+int lbm_upload_thermal(lbm_handle h, int32_t l, const double* g)
+{
+    using namespace lbm;
+    int rc = thermal_ok(h, l);
+    if (rc) return rc;
+    if (!g) return fail(LBM_ERR_INVALID, "null population array");
+    HostLattice& H = h->lat[0];
+    if (!H.uploaded || H.steps > 0)
+        return fail(LBM_ERR_INVALID, "upload g right after f (lbm_upload or lbm_init_equilibrium), before stepping");
+    if ((rc = commit(h)) != LBM_OK) return rc;
+    if ((rc = thermal_load(h, g)) != LBM_OK) return rc;
+    return thermal_recollide(h);
+}
+
+int lbm_download_thermal(lbm_handle h, int32_t l, double* g)
+{
+    using namespace lbm;
+    int rc = thermal_ok(h, l);
+    if (rc) return rc;
+    if (!g) return fail(LBM_ERR_INVALID, "null output array");
+    if (!h->lat[0].uploaded) return fail(LBM_ERR_INVALID, "lattice has no state");
+    return thermal_out(h, THERMAL_OUT_G, 0, h->lat[0].d.ny, g);
+}
+
+int lbm_temperature(lbm_handle h, int32_t l, int32_t y0, int32_t y1, double* t)
+{
+    using namespace lbm;
+    int rc = thermal_ok(h, l);
+    if (rc) return rc;
+    if (!t) return fail(LBM_ERR_INVALID, "null output array");
+    if (!h->lat[0].uploaded) return fail(LBM_ERR_INVALID, "lattice has no state");
+    if (y0 < 0 || y1 > h->lat[0].d.ny || y0 >= y1) return fail(LBM_ERR_INVALID, "row range out of bounds");
+    return thermal_out(h, THERMAL_OUT_T, y0, y1, t);
 }
 
 int lbm_wall_distance(lbm_handle h, int32_t l, double* out)

@@ -109,6 +109,42 @@ struct PrologueArgs {
     float scale, inv_scale;
 };
 
+// Thermal double distribution of one lattice (src/lbm.hpp:356-397, 489-528): what the
+// thermal kernel does over rows [y0, y1) of StepArgs.L0.
+enum : int32_t {
+    THERMAL_STEP = 0,     // one step: f and g from src / gsrc (post-collision) to dst / gdst, forces
+    THERMAL_COLLIDE = 1,  // the stored post-collision f and g of the current state (no forces)
+    THERMAL_OUT_G = 2,    // the state's g as AoS FP64 rows into out
+    THERMAL_OUT_T = 3,    // the state's temperature sum(g) into out
+};
+constexpr int kThermalRules = 15;
+constexpr uint8_t THERMAL_MIRROR = 16;  // per-node code bit: g[i] = g[opp(i)] in slot order
+
+struct ThermalArgs {
+    const void* gsrc;           // thermal planes: post-collision of the previous step, or the state
+    void* gdst;
+    const uint8_t* code;        // per-node conditions after each step (pitch layout), or null
+    double rule_t[kThermalRules];
+    uint32_t rule_rest;         // bit k: rule k is the equilibrium at rest (else at the node's velocity)
+    int32_t mode;               // THERMAL_*
+    int32_t from_state;         // the sources hold the state itself (no step taken yet)
+    int32_t y0, y1;
+    double omega_k, beta, g_buoyancy, t_ref;
+    double* out;
+};
+
+// Rows [y0, y1) of AoS FP64 thermal state (null: 1.0 in every slot) into the state planes.
+struct ThermalLoadArgs {
+    const LatticeDev* lat;
+    int32_t lattice;
+    int32_t y0, y1;
+    const double* in;
+    void* state_dst;
+};
+
+hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s);
+hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s);
+
 // single: the one-lattice kernel (StepArgs.L0, src, dst); else the ensemble kernel.
 hipError_t launch_step(int storage, int block_x, int rows, int vec, bool single, int n_blocks, const StepArgs& a,
                        hipStream_t s);

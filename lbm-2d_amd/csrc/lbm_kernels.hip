@@ -543,9 +543,40 @@ __device__ __forceinline__ void links_of(const Ctx<ST>& c, const Pre<ST>& p, Lin
     if (p.fl & CELL_LINKS) load_link(c.links[c.L.link_off + p.ls], lk);
 }
 
+// Momentum exchange of a node with boundary links on its post-boundary state d
+// (src/lbm.hpp:542-581): the terms of the record's primary slot (that of its first
+// counted link) summed in direction order and stored for the record; terms of any other
+// slot go to the fallback accumulator.  ls_: the node's own links.
+template <int ST>
+__device__ __forceinline__ void node_forces(const LatticeDev& L, const Ctx<ST>& c, const ForceSink& fs,
+                                            const LinkVals<typename Form<ST>::real>& ls_,
+                                            const typename Form<ST>::real d[9])
+{
+    int ps = -1;
+    double fx = 0.0, fy = 0.0;
+#pragma unroll
+    for (int i = 1; i < 9; ++i) {
+        const int si = ls_.slot(i);
+        if (!(ls_.mask & (1u << i)) || si < 0) continue;
+        const double fb = WD[OPP[i]] + static_cast<double>(d[OPP[i]]);
+        if (ps < 0) ps = si;
+        if (si == ps) {
+            fx += CX[i] * 2.0 * fb;
+            fy += CY[i] * 2.0 * fb;
+        } else {
+            atomicAdd(fs.accp + 2 * (L.slot_base + si), CX[i] * 2.0 * fb);
+            atomicAdd(fs.accp + 2 * (L.slot_base + si) + 1, CY[i] * 2.0 * fb);
+        }
+    }
+    if (ps >= 0) {
+        const int64_t r = ls_.rec - c.links;
+        fs.nf[2 * r] = fx;
+        fs.nf[2 * r + 1] = fy;
+    }
+}
+
 // The general path of one node: boundary state from the preloaded gathered node (p,
-// lg), momentum exchange on it with the node's own links (ls_, src/lbm.hpp:542-581),
-// collision, store.
+// lg), momentum exchange on it with the node's own links (ls_), collision.
 template <int ST>
 __device__ __forceinline__ void node_general_d(const LatticeDev& L, const Ctx<ST>& c, const ForceSink& fs, int x,
                                                int y, uint8_t fl, const Pre<ST>& p,
@@ -556,33 +587,7 @@ __device__ __forceinline__ void node_general_d(const LatticeDev& L, const Ctx<ST
     using R = typename Form<ST>::real;
     const bool solid = (fl & CELL_SOLID) != 0;
     state_general<ST>(c, x, y, solid, p, lg, d);
-    if (fs.nf && (fl & CELL_LINKS) && L.forces) {
-        // momentum exchange on the post-boundary state (src/lbm.hpp:542-581): the terms of
-        // the record's primary slot (that of its first counted link) summed in direction
-        // order and stored for the record; terms of any other slot go to the fallback
-        // accumulator
-        int ps = -1;
-        double fx = 0.0, fy = 0.0;
-#pragma unroll
-        for (int i = 1; i < 9; ++i) {
-            const int si = ls_.slot(i);
-            if (!(ls_.mask & (1u << i)) || si < 0) continue;
-            const double fb = WD[OPP[i]] + static_cast<double>(d[OPP[i]]);
-            if (ps < 0) ps = si;
-            if (si == ps) {
-                fx += CX[i] * 2.0 * fb;
-                fy += CY[i] * 2.0 * fb;
-            } else {
-                atomicAdd(fs.accp + 2 * (L.slot_base + si), CX[i] * 2.0 * fb);
-                atomicAdd(fs.accp + 2 * (L.slot_base + si) + 1, CY[i] * 2.0 * fb);
-            }
-        }
-        if (ps >= 0) {
-            const int64_t r = ls_.rec - c.links;
-            fs.nf[2 * r] = fx;
-            fs.nf[2 * r + 1] = fy;
-        }
-    }
+    if (fs.nf && (fl & CELL_LINKS) && L.forces) node_forces<ST>(L, c, fs, ls_, d);
     if (!solid) {
         R wd = R(0);
         if (L.les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
@@ -1264,6 +1269,210 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
     for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
 }
 
+// ------------------------------------------------------------------ thermal populations
+
+// Post-stream thermal populations (as deviations g - w) of node (x,y), pulled from the
+// post-collision planes gs (src/lbm.hpp:489-528): y always periodic, x periodic on a
+// periodic x edge; a source off the lattice or solid means the population bounced back
+// into this node's opposite slot.  Solid nodes are never streamed into (zero).
+template <int ST>
+__device__ __forceinline__ void thermal_gather(const LatticeDev& L, const typename Form<ST>::store* gs,
+                                               const uint8_t* flags, int x, int y, bool solid,
+                                               typename Form<ST>::real gd[9])
+{
+    using R = typename Form<ST>::real;
+    if (solid) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) gd[j] = -wgt<R>(j);
+        return;
+    }
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        int sx = x - CX[j], sy = y - CY[j];
+        sy = sy < 0 ? sy + L.ny : (sy >= L.ny ? sy - L.ny : sy);
+        if (L.x_edge == EDGE_PERIODIC) sx = sx < 0 ? sx + L.nx : (sx >= L.nx ? sx - L.nx : sx);
+        const int64_t sidx = static_cast<int64_t>(sy) * L.pitch + sx;
+        const bool blocked = sx < 0 || sx >= L.nx || (flags[L.cell_off + sidx] & CELL_SOLID);
+        gd[j] = Form<ST>::dec(blocked ? gs[OPP[j] * L.plane + idx] : gs[j * L.plane + sidx], 1.f, 0.f);
+    }
+}
+
+// The thermal drivers' conditions on the state after a step (src/heated_cylinder.cpp:
+// 155-164, src/thermal_cavity.cpp:153-188): rule k sets the thermal equilibrium at its
+// temperature (src/thermal.hpp:24-30) with the node's velocity from the state d, or at
+// rest; THERMAL_MIRROR then copies g[opp(i)] into g[i] in slot order.
+template <typename R>
+__device__ __forceinline__ void thermal_rules(const ThermalArgs& t, uint8_t code, const R d[9], R gd[9])
+{
+    const int k = (code & 15) - 1;
+    if (k >= 0) {
+        const R T = static_cast<R>(t.rule_t[k]);
+        R u = R(0), v = R(0);
+        if (!((t.rule_rest >> k) & 1u)) {
+            // compute_macros on the state (src/lbm_types.hpp:278-288); sum w_i c_i = 0
+            R s = R(0), mx = R(0), my = R(0);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                s += d[i];
+                mx += R(CX[i]) * d[i];
+                my += R(CY[i]) * d[i];
+            }
+            const R rho = R(1) + s;
+            u = mx / rho;
+            v = my / rho;
+        }
+        const R uu = u * u + v * v;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const R eu = R(CX[i]) * u + R(CY[i]) * v;
+            gd[i] = wgt<R>(i) * T * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * uu) - wgt<R>(i);
+        }
+    }
+    if (code & THERMAL_MIRROR) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gd[i] = gd[OPP[i]];
+    }
+}
+
+// Thermal collision and Boussinesq forcing of a fluid node (src/lbm.hpp:356-397), on the
+// post-collision f deviations d and the pre-collision g deviations gd: the velocity is
+// that of the post-collision f, T = sum g; g relaxes at omega_k to the thermal
+// equilibrium (src/thermal.hpp:35-41); with beta > 0 and g_buoyancy != 0,
+// F = (0, -beta (T - t_ref) g_buoyancy) (src/thermal.hpp:45-52) enters f as
+// w_i (3 F.e_i + 9 (e_i.u)(e_i.F)).
+template <typename R>
+__device__ __forceinline__ void thermal_collide(const ThermalArgs& t, R d[9], R gd[9])
+{
+    R s = R(0), mx = R(0), my = R(0), dT = R(0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        s += d[i];
+        mx += R(CX[i]) * d[i];
+        my += R(CY[i]) * d[i];
+        dT += gd[i];
+    }
+    const R rho = R(1) + s;
+    const R u = mx / rho, v = my / rho;
+    const R T = R(1) + dT;  // sum w_i = 1
+    const R uu = u * u + v * v;
+    const R om = static_cast<R>(t.omega_k);
+    const bool buoy = t.beta > 0.0 && t.g_buoyancy != 0.0;
+    const R fy = static_cast<R>(-t.beta) * (T - static_cast<R>(t.t_ref)) * static_cast<R>(t.g_buoyancy);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const R eu = R(CX[i]) * u + R(CY[i]) * v;
+        // g_eq - w = w (T - 1) + w T (3 eu + 4.5 eu^2 - 1.5 u^2)
+        const R geq = wgt<R>(i) * (dT + T * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * uu));
+        gd[i] -= om * (gd[i] - geq);
+        if (buoy) {
+            const R ef = R(CY[i]) * fy;
+            d[i] += wgt<R>(i) * (R(3) * ef + R(9) * eu * ef);
+        }
+    }
+}
+
+// One lattice with thermal populations, one node per thread over 64 x 4 tiles of rows
+// [t.y0, t.y1): the momentum state from the general path (every node), the thermal
+// state from thermal_gather and the node conditions; then, by t.mode, the collisions
+// and stores of a step (forces into the launch's record terms) or of the current state,
+// or the thermal state / temperature as FP64.  With t.from_state the sources hold the
+// state itself (uploaded, no step taken): no gather and no conditions.
+template <int ST>
+__global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
+{
+    using S = typename Form<ST>::store;
+    using R = typename Form<ST>::real;
+    const LatticeDev& L = a.L0;
+    const int x = static_cast<int>(blockIdx.x) * 64 + (static_cast<int>(threadIdx.x) & 63);
+    const int y = t.y0 + static_cast<int>(blockIdx.y) * 4 + (static_cast<int>(threadIdx.x) >> 6);
+    if (x >= L.nx || y >= t.y1) return;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    const uint8_t fl = a.flags[L.cell_off + idx];
+    const bool solid = (fl & CELL_SOLID) != 0;
+    const S* fsrc = static_cast<const S*>(a.src) + L.f_off;
+    const S* gsrc = static_cast<const S*>(t.gsrc) + L.f_off;
+    R d[9], gd[9];
+    if (t.from_state) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            d[j] = Form<ST>::dec(fsrc[j * L.plane + idx], 1.f, 0.f);
+            gd[j] = Form<ST>::dec(gsrc[j * L.plane + idx], 1.f, 0.f);
+        }
+    } else {
+        const Ctx<ST> c{L, fsrc, a.flags, a.link_slot, a.links, a.inv_scale};
+        const int gx = gathered_x<ST>(L, x);
+        Pre<ST> p;
+        preload<ST>(c, gx, y, p);
+        LinkVals<R> lg;
+        links_of<ST>(c, p, lg);
+        state_general<ST>(c, x, y, solid, p, lg, d);
+        if (t.mode == THERMAL_STEP && a.nf && (fl & CELL_LINKS) && L.forces) {
+            LinkVals<R> ls_ = lg;
+            if (gx != x) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
+            double* accp = a.acc + static_cast<int64_t>((blockIdx.y * gridDim.x + blockIdx.x) % kAccCopies) *
+                                       a.total_slots * 2;
+            node_forces<ST>(L, c, ForceSink{a.nf, accp}, ls_, d);
+        }
+        thermal_gather<ST>(L, gsrc, a.flags, x, y, solid, gd);
+        if (t.code) thermal_rules<R>(t, t.code[L.cell_off + idx], d, gd);
+    }
+    if (t.mode == THERMAL_OUT_G) {
+        double* o = t.out + (static_cast<int64_t>(y - t.y0) * L.nx + x) * 9;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) o[j] = WD[j] + static_cast<double>(gd[j]);
+        return;
+    }
+    if (t.mode == THERMAL_OUT_T) {
+        // compute_temperature (src/thermal.hpp:55-60) in slot order
+        double T = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) T += WD[j] + static_cast<double>(gd[j]);
+        t.out[static_cast<int64_t>(y - t.y0) * L.nx + x] = T;
+        return;
+    }
+    if (!solid) {
+        R wd = R(0);
+        if (L.les) {
+            const uint16_t w = a.wdist[L.cell_off + idx];
+            wd = w == 0xffffu ? R(1e9) : static_cast<R>(w);
+        }
+        collide<R>(d, L, wd, R(1));
+        if (!(d[0] == d[0])) atomicOr(a.nan_flags + a.lattice0, 1);
+        thermal_collide<R>(t, d, gd);
+    }
+    // as the one-step kernels: solid nodes keep a stored value only where it is read
+    // (written solids); the post-upload collision stores every node
+    if (!solid || t.from_state || (fl & CELL_WRITE)) {
+        S* o = static_cast<S*>(a.dst) + L.f_off + idx;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) o[j * L.plane] = Form<ST>::enc(d[j], 1.f, 0.f);
+    }
+    if (!solid || t.from_state) {
+        S* o = static_cast<S*>(t.gdst) + L.f_off + idx;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) o[j * L.plane] = Form<ST>::enc(gd[j], 1.f, 0.f);
+    }
+}
+
+// Rows of AoS FP64 thermal state (or 1.0 in every slot, the reference's initial value,
+// src/lbm_types.hpp:215) into the state planes, as deviations from w.
+template <int ST>
+__global__ __launch_bounds__(256) void thermal_load_kernel(ThermalLoadArgs a)
+{
+    using S = typename Form<ST>::store;
+    using R = typename Form<ST>::real;
+    const LatticeDev L = a.lat[a.lattice];
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = a.y0 + blockIdx.y;
+    if (x >= L.nx || y >= a.y1) return;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    const double* in = a.in ? a.in + (static_cast<int64_t>(y - a.y0) * L.nx + x) * 9 : nullptr;
+    S* o = static_cast<S*>(a.state_dst) + L.f_off + idx;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) o[j * L.plane] = Form<ST>::enc(static_cast<R>((in ? in[j] : 1.0) - WD[j]), 1.f, 0.f);
+}
+
 // Advances the per-lattice step counters after a batch of launches; an odd number of
 // launches (a two-step launch swaps the buffers once) leaves the state in the other buffer.
 __global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
@@ -1454,6 +1663,28 @@ hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream
     case ST_F32: hipLaunchKernelGGL(prologue_kernel<ST_F32>, grid, dim3(256), 0, s, a); break;
     case ST_F16S: hipLaunchKernelGGL(prologue_kernel<ST_F16S>, grid, dim3(256), 0, s, a); break;
     case ST_BF16S: hipLaunchKernelGGL(prologue_kernel<ST_BF16S>, grid, dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s)
+{
+    dim3 grid((a.L0.nx + 63) / 64, (t.y1 - t.y0 + 3) / 4);
+    switch (storage) {
+    case ST_F64: hipLaunchKernelGGL(thermal_kernel<ST_F64>, grid, dim3(256), 0, s, a, t); break;
+    case ST_F32: hipLaunchKernelGGL(thermal_kernel<ST_F32>, grid, dim3(256), 0, s, a, t); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s)
+{
+    dim3 grid((nx + 255) / 256, a.y1 - a.y0);
+    switch (storage) {
+    case ST_F64: hipLaunchKernelGGL(thermal_load_kernel<ST_F64>, grid, dim3(256), 0, s, a); break;
+    case ST_F32: hipLaunchKernelGGL(thermal_load_kernel<ST_F32>, grid, dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -6,8 +6,10 @@
 // sys.fx_body / sys.fy_body the per-node momentum exchange.  Each call moves sys.f to
 // the GPU, takes one step there and brings the populations back, so drivers that read
 // or write sys.f between steps keep working unchanged.  lbm_gpu::run_steps() is the
-// device-resident form for many steps.  The thermal double distribution
-// (sys.use_thermal) is not part of this step and is rejected.
+// device-resident form for many steps.  With sys.use_thermal the thermal double
+// distribution sys.g_thermal moves with sys.f and steps on the GPU with it (omega_k,
+// beta, g_buoyancy, T_ref from sys); the drivers' own loops on g between steps keep
+// working, or lbm_gpu::set_thermal_conditions() runs them on the device in run_steps.
 //
 // Link against liblbm2d.so (lbm-2d_amd/Makefile).  lbm_gpu::config().storage selects
 // the population storage of the GPU lattice (FP64 by default here, for results equal
@@ -27,6 +29,7 @@
 #include "../../include/lbm_gpu.h"
 #include "geometry.hpp"
 #include "lbm_types.hpp"
+#include "thermal.hpp"
 
 namespace lbm_gpu {
 
@@ -64,6 +67,10 @@ struct Context {
     // fluid nodes with solid neighbours, for the per-node forces: (node, direction)
     std::vector<std::pair<int, int>> links;
     bool have_geometry = false;
+    // thermal populations on the handle; device-side conditions for run_steps
+    bool thermal = false;
+    std::vector<uint8_t> thermal_code;
+    std::vector<lbm_thermal_rule> thermal_rules;
 
     ~Context()
     {
@@ -121,7 +128,7 @@ inline void prepare(LBMCapabilities& sys, double tau, double u_inflow)
     const int les = g_use_les ? 1 : 0;
     if (!C.h || C.nx != NX || C.ny != NY || C.case_type != case_type || C.collision != collision || C.les != les ||
         C.cs != g_cs || C.body_force != sys.body_force_x || C.storage != cfg.storage || C.shift_u != cfg.shift_u ||
-        C.sr != cfg.stochastic_rounding || C.sr_seed != cfg.rounding_seed) {
+        C.sr != cfg.stochastic_rounding || C.sr_seed != cfg.rounding_seed || (C.thermal && !sys.use_thermal)) {
         if (C.h) {
             lbm_destroy(C.h);
             C.h = nullptr;
@@ -150,6 +157,7 @@ inline void prepare(LBMCapabilities& sys, double tau, double u_inflow)
         C.body_force = sys.body_force_x;
         C.storage = cfg.storage;
         C.have_geometry = false;
+        C.thermal = false;
     }
     if (!C.have_geometry || C.obstacle != sys.obstacle || !same_geometry(C.geom, sys.bb_geom)) {
         check(lbm_set_obstacle(C.h, 0, sys.obstacle.data()), "lbm_set_obstacle");
@@ -183,6 +191,21 @@ inline void prepare(LBMCapabilities& sys, double tau, double u_inflow)
     }
     check(lbm_set_params(C.h, 0, tau, u_inflow), "lbm_set_params");
     check(lbm_upload(C.h, 0, sys.f.data()), "lbm_upload");
+    if (sys.use_thermal) {
+        check(lbm_set_thermal(C.h, 0, sys.omega_k, sys.beta, sys.g_buoyancy, sys.T_ref), "lbm_set_thermal");
+        C.thermal = true;
+        check(lbm_upload_thermal(C.h, 0, sys.g_thermal.data()), "lbm_upload_thermal");
+        check(lbm_set_thermal_rules(C.h, 0, nullptr, nullptr, 0), "lbm_set_thermal_rules");
+    }
+}
+
+// Device-side conditions on g after every step for run_steps (lbm_set_thermal_rules):
+// code[y*NX + x], rules as in include/lbm_gpu.h.  execute_time_step leaves them to the
+// driver's own host loop.
+inline void set_thermal_conditions(std::vector<uint8_t> code, std::vector<lbm_thermal_rule> rules)
+{
+    context().thermal_code = std::move(code);
+    context().thermal_rules = std::move(rules);
 }
 
 // Per-node momentum exchange from the post-boundary state on the host, over the
@@ -202,12 +225,16 @@ inline void node_forces(LBMCapabilities& sys)
 inline void run_steps(LBMCapabilities& sys, int n, double tau, double u_inflow, std::vector<double>* fx = nullptr,
                       std::vector<double>* fy = nullptr)
 {
-    if (sys.use_thermal) throw std::runtime_error("execute_time_step: the thermal DDF is not part of the GPU step");
     prepare(sys, tau, u_inflow);
     Context& C = context();
+    if (sys.use_thermal && !C.thermal_code.empty())
+        check(lbm_set_thermal_rules(C.h, 0, C.thermal_code.data(), C.thermal_rules.data(),
+                                    static_cast<int32_t>(C.thermal_rules.size())),
+              "lbm_set_thermal_rules");
     check(lbm_set_launch(C.h, 0, 0, 0, -1, config().graph_steps), "lbm_set_launch");
     check(lbm_step(C.h, n), "lbm_step");
     check(lbm_download(C.h, 0, sys.f.data()), "lbm_download");
+    if (sys.use_thermal) check(lbm_download_thermal(C.h, 0, sys.g_thermal.data()), "lbm_download_thermal");
     std::vector<double> bx(static_cast<size_t>(n)), by(static_cast<size_t>(n));
     int32_t got = 0;
     check(lbm_forces(C.h, 0, bx.data(), by.data(), n, &got), "lbm_forces");
@@ -221,11 +248,11 @@ inline void run_steps(LBMCapabilities& sys, int n, double tau, double u_inflow, 
 // One complete time step (src/lbm.hpp:293-582) on the GPU.
 inline void execute_time_step(LBMCapabilities& sys, double tau, double u_inflow)
 {
-    if (sys.use_thermal) throw std::runtime_error("execute_time_step: the thermal DDF is not part of the GPU step");
     lbm_gpu::prepare(sys, tau, u_inflow);
     lbm_gpu::Context& C = lbm_gpu::context();
     lbm_gpu::check(lbm_step(C.h, 1), "lbm_step");
     lbm_gpu::check(lbm_download(C.h, 0, sys.f.data()), "lbm_download");
+    if (sys.use_thermal) lbm_gpu::check(lbm_download_thermal(C.h, 0, sys.g_thermal.data()), "lbm_download_thermal");
     if (g_case != CaseType::CAVITY) lbm_gpu::node_forces(sys);
 }
 
@@ -281,21 +308,28 @@ inline bool check_convergence(const std::vector<double>& values, int window = 10
 // File formats of the reference's writers (src/lbm.hpp:588-926), consumed by its
 // post-processing scripts and web viewer.
 
-// output_dir/frames/frame_<step>.json: fields sampled every max(1, NX/100) nodes:
-// |u|, u, v, rho, p = rho/3, vorticity by central differences, obstacle flag.
-inline void save_json_frame(const LBMCapabilities& sys, int step, const std::string& output_dir)
+namespace lbm_gpu {
+// The frame writers' common body; with a thermal channel when t_wall is non-null
+// (solid samples take *t_wall, fluid ones sum(g), NaN read as 1).
+inline void write_frame(const LBMCapabilities& sys, int step, const std::string& output_dir, const double* t_wall)
 {
     const int ds = std::max(1, NX / 100);
     const int mx = (NX + ds - 1) / ds, my = (NY + ds - 1) / ds;
     const int m = mx * my;
-    std::vector<double> speed(m, 0.0), uu(m, 0.0), vv(m, 0.0), rr(m, 0.0), om(m, 0.0);
+    std::vector<double> speed(m, 0.0), uu(m, 0.0), vv(m, 0.0), rr(m, 0.0), om(m, 0.0), tt(m, 0.0);
     std::vector<int> solid(m, 0);
     for (int j = 0; j < my; ++j)
         for (int i = 0; i < mx; ++i) {
             const int k = j * mx + i, n = node_index(i * ds, j * ds);
             if (sys.obstacle[n]) {
                 solid[k] = 1;
+                if (t_wall) tt[k] = *t_wall;
                 continue;
+            }
+            if (t_wall) {
+                double T;
+                compute_temperature(&sys.g_thermal[static_cast<size_t>(n) * 9], T);
+                tt[k] = std::isnan(T) ? 1.0 : T;
             }
             double rho, u, v;
             compute_macros(&sys.f[static_cast<size_t>(n) * 9], rho, u, v);
@@ -338,8 +372,26 @@ inline void save_json_frame(const LBMCapabilities& sys, int step, const std::str
     field("rho", [&](int k) { return rr[k]; });
     field("p", [&](int k) { return rr[k] / 3.0; });
     field("omega", [&](int k) { return om[k]; });
+    if (t_wall) field("temperature", [&](int k) { return tt[k]; });
     field("obstacle", [&](int k) { return solid[k]; });
     out << "]}";
+}
+}  // namespace lbm_gpu
+
+// output_dir/frames/frame_<step>.json: fields sampled every max(1, NX/100) nodes:
+// |u|, u, v, rho, p = rho/3, vorticity by central differences, obstacle flag
+// (src/lbm.hpp:588-700).
+inline void save_json_frame(const LBMCapabilities& sys, int step, const std::string& output_dir)
+{
+    lbm_gpu::write_frame(sys, step, output_dir, nullptr);
+}
+
+// The same frame with a "temperature" channel before "obstacle": sum(g) on fluid
+// samples, T_wall on solid ones (src/lbm.hpp:704-816).
+inline void save_json_frame_thermal(const LBMCapabilities& sys, int step, const std::string& output_dir,
+                                    double T_wall)
+{
+    lbm_gpu::write_frame(sys, step, output_dir, &T_wall);
 }
 
 // output_dir/forces.jsonl, one {"step","cd","cl"} line per call; truncated when the

@@ -95,7 +95,7 @@ struct LBMCapabilities {
 
     std::vector<double> g_thermal;
     std::vector<double> g_thermal_next;
-    bool use_thermal = false;  // the thermal double distribution is not part of the GPU step
+    bool use_thermal = false;  // thermal double distribution g (stepped on the GPU with f)
     double T_ref = 1.0;
     double alpha = 0.1666666667;
     double omega_k = 1.0;
@@ -115,6 +115,8 @@ struct LBMCapabilities {
         f_next.assign(n * NUM_DIRECTIONS, 0.0);
         obstacle.assign(n, 0);
         wall_dist.assign(n, 0.0);
+        g_thermal.assign(n * NUM_DIRECTIONS, 1.0);  // T = 9 until a driver sets it
+        g_thermal_next.assign(n * NUM_DIRECTIONS, 1.0);
         reset_forces();
     }
 

@@ -57,6 +57,14 @@ class LbmInfo(C.Structure):
     ]
 
 
+class LbmThermalRule(C.Structure):
+    _fields_ = [("t", C.c_double), ("kind", C.c_int32), ("pad", C.c_int32)]
+
+
+THERMAL_EQ_FLOW, THERMAL_EQ_REST = 0, 1
+THERMAL_MIRROR = 16
+
+
 class LbmError(RuntimeError):
     pass
 
@@ -114,6 +122,11 @@ def lib() -> C.CDLL:
         "lbm_get_info": (C.c_int, [_h, C.POINTER(LbmInfo)]),
         "lbm_sync": (C.c_int, [_h]),
         "lbm_wall_distance": (C.c_int, [_h, i32, _dp]),
+        "lbm_set_thermal": (C.c_int, [_h, i32, dbl, dbl, dbl, dbl]),
+        "lbm_set_thermal_rules": (C.c_int, [_h, i32, _u8p, C.POINTER(LbmThermalRule), i32]),
+        "lbm_upload_thermal": (C.c_int, [_h, i32, _dp]),
+        "lbm_download_thermal": (C.c_int, [_h, i32, _dp]),
+        "lbm_temperature": (C.c_int, [_h, i32, i32, i32, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -235,6 +248,43 @@ class Ensemble:
         ny, nx = self.shape(l)
         out = np.empty((ny, nx, 9), dtype=np.float64)
         _check(lib().lbm_download(self._h, l, out.ctypes.data_as(_dp)))
+        return out
+
+    # ---- thermal double distribution (one-lattice handles, f64 / f32)
+    def set_thermal(self, l: int, omega_k: float, beta: float = 0.0, g_buoyancy: float = 0.0,
+                    t_ref: float = 1.0):
+        _check(lib().lbm_set_thermal(self._h, l, omega_k, beta, g_buoyancy, t_ref))
+
+    def set_thermal_rules(self, l: int, code: Optional[np.ndarray], rules: Sequence = ()):
+        """code: (ny, nx) uint8 (low 4 bits: rule index + 1, THERMAL_MIRROR bit);
+        rules: (t, kind) pairs, kind THERMAL_EQ_FLOW or THERMAL_EQ_REST."""
+        arr = (LbmThermalRule * max(1, len(rules)))()
+        for k, (t, kind) in enumerate(rules):
+            arr[k].t, arr[k].kind = t, kind
+        if code is None:
+            cp = None
+        else:
+            c = np.ascontiguousarray(code, dtype=np.uint8)
+            assert c.shape == self.shape(l), (c.shape, self.shape(l))
+            cp = c.ctypes.data_as(_u8p)
+        _check(lib().lbm_set_thermal_rules(self._h, l, cp, arr, len(rules)))
+
+    def upload_thermal(self, l: int, g: np.ndarray):
+        ny, nx = self.shape(l)
+        a = np.ascontiguousarray(g, dtype=np.float64).reshape(ny, nx, 9)
+        _check(lib().lbm_upload_thermal(self._h, l, a.ctypes.data_as(_dp)))
+
+    def download_thermal(self, l: int = 0) -> np.ndarray:
+        ny, nx = self.shape(l)
+        out = np.empty((ny, nx, 9), dtype=np.float64)
+        _check(lib().lbm_download_thermal(self._h, l, out.ctypes.data_as(_dp)))
+        return out
+
+    def temperature(self, l: int = 0, y0: int = 0, y1: Optional[int] = None) -> np.ndarray:
+        ny, nx = self.shape(l)
+        y1 = ny if y1 is None else y1
+        out = np.empty((max(0, y1 - y0), nx))
+        _check(lib().lbm_temperature(self._h, l, y0, y1, out.ctypes.data_as(_dp)))
         return out
 
     def step(self, n: int = 1):

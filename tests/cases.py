@@ -284,3 +284,31 @@ def thermal_walls(o, tc: ThermalCase):
 def thermal_step(o, tc: ThermalCase):
     o.step(tc.base.tau, tc.base.u)
     thermal_walls(o, tc)
+
+
+def gpu_thermal_rules(tc: ThermalCase, obstacle: np.ndarray):
+    """The driver's wall loop as per-node conditions of the GPU library: (code, rules)."""
+    if not tc.walls:
+        return None, []
+    ny, nx = obstacle.shape
+    code = np.zeros((ny, nx), np.uint8)
+    if tc.walls[0] == "solids":
+        code[obstacle != 0] = 1
+        return code, [(tc.walls[1], G.THERMAL_EQ_REST)]
+    code[1:ny - 1, 1] = 1
+    code[1:ny - 1, nx - 2] = 2
+    code[1, 1:nx - 1] |= G.THERMAL_MIRROR
+    code[ny - 2, 1:nx - 1] |= G.THERMAL_MIRROR
+    return code, [(tc.walls[1], G.THERMAL_EQ_FLOW), (tc.walls[2], G.THERMAL_EQ_FLOW)]
+
+
+def build_gpu_thermal(tc: ThermalCase, storage: str):
+    """GPU lattice with the thermal populations, state and conditions of the oracle's."""
+    o = build_thermal_oracle(tc)
+    ens, _ = build_gpu(tc.base, storage, o=o)
+    ens.set_thermal(0, tc.omega_k, tc.beta, tc.g_buoyancy, tc.t_ref)
+    ens.upload_thermal(0, o.g.copy())
+    code, rules = gpu_thermal_rules(tc, o.obstacle)
+    if code is not None:
+        ens.set_thermal_rules(0, code, rules)
+    return ens, o

@@ -4,8 +4,8 @@ lbm_types.hpp, geometry.hpp) accept the reference's own case drivers unchanged.
 Each in-scope driver source (src/*.cpp of the reference) is compiled from stdin, so
 its #include "lbm.hpp" resolves to the build's headers, and linked against
 liblbm2d.so.  Runs where the reference sources are present; nothing is copied.
-The thermal and immersed-boundary drivers are outside the GPU step and are not
-compiled here.
+The immersed-boundary driver (src/airfoil_ibm.cpp) is outside the GPU step and is
+not compiled here.
 """
 import os
 import subprocess
@@ -17,7 +17,7 @@ REF = "/root/reference/src"
 PKG = os.path.join(ROOT, "lbm-2d_amd")
 DRIVERS = ["main", "cavity", "flat_plate", "step", "rotating_cylinder", "square_cylinder",
            "cylinder_near_wall", "side_by_side_cylinders", "orifice_plate", "periodic_hills",
-           "urban_canyon", "downwash"]
+           "urban_canyon", "downwash", "heated_cylinder", "thermal_cavity"]
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference sources not present")
