@@ -1373,8 +1373,9 @@ __device__ __forceinline__ void thermal_collide(const ThermalArgs& t, R d[9], R 
 }
 
 // One lattice with thermal populations, one node per thread over 64 x 4 tiles of rows
-// [t.y0, t.y1): the momentum state from the general path (every node), the thermal
-// state from thermal_gather and the node conditions; then, by t.mode, the collisions
+// [t.y0, t.y1): the state of f and g from the plain pulls (interior nodes that are not
+// special) or from the general path and thermal_gather, then the node conditions;
+// then, by t.mode, the collisions
 // and stores of a step (forces into the launch's record terms) or of the current state,
 // or the thermal state / temperature as FP64.  With t.from_state the sources hold the
 // state itself (uploaded, no step taken): no gather and no conditions.
@@ -1388,10 +1389,23 @@ __global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
     const int y = t.y0 + static_cast<int>(blockIdx.y) * 4 + (static_cast<int>(threadIdx.x) >> 6);
     if (x >= L.nx || y >= t.y1) return;
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
-    const uint8_t fl = a.flags[L.cell_off + idx];
-    const bool solid = (fl & CELL_SOLID) != 0;
     const S* fsrc = static_cast<const S*>(a.src) + L.f_off;
     const S* gsrc = static_cast<const S*>(t.gsrc) + L.f_off;
+    // The flag and both bulk pulls in one round trip (as cell_update: every pull address
+    // lies inside the population pool and its guard); an interior node that is not
+    // special has nine in-lattice fluid neighbours, so both pulls are its state.
+    const uint8_t fl = a.flags[L.cell_off + idx];
+    S fr[9], gr[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int64_t o = j * L.plane + idx - (CY[j] * L.pitch + CX[j]);
+        fr[j] = fsrc[o];
+        gr[j] = gsrc[o];
+    }
+    __asm__ volatile("" ::"v"(fr[0]), "v"(fr[1]), "v"(fr[2]), "v"(fr[3]), "v"(fr[4]), "v"(fr[5]), "v"(fr[6]),
+                     "v"(fr[7]), "v"(fr[8]), "v"(gr[0]), "v"(gr[1]), "v"(gr[2]), "v"(gr[3]), "v"(gr[4]),
+                     "v"(gr[5]), "v"(gr[6]), "v"(gr[7]), "v"(gr[8]));
+    const bool solid = (fl & CELL_SOLID) != 0;
     R d[9], gd[9];
     if (t.from_state) {
 #pragma unroll
@@ -1399,6 +1413,13 @@ __global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
             d[j] = Form<ST>::dec(fsrc[j * L.plane + idx], 1.f, 0.f);
             gd[j] = Form<ST>::dec(gsrc[j * L.plane + idx], 1.f, 0.f);
         }
+    } else if (!(fl & CELL_SPECIAL)) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            d[j] = Form<ST>::dec(fr[j], 1.f, 0.f);
+            gd[j] = Form<ST>::dec(gr[j], 1.f, 0.f);
+        }
+        if (t.code) thermal_rules<R>(t, t.code[L.cell_off + idx], d, gd);
     } else {
         const Ctx<ST> c{L, fsrc, a.flags, a.link_slot, a.links, a.inv_scale};
         const int gx = gathered_x<ST>(L, x);
@@ -1443,15 +1464,16 @@ __global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
     }
     // as the one-step kernels: solid nodes keep a stored value only where it is read
     // (written solids); the post-upload collision stores every node
+    const bool nt = (a.options & LAUNCH_STREAM_STORES) != 0;
     if (!solid || t.from_state || (fl & CELL_WRITE)) {
         S* o = static_cast<S*>(a.dst) + L.f_off + idx;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) o[j * L.plane] = Form<ST>::enc(d[j], 1.f, 0.f);
+        for (int j = 0; j < 9; ++j) store_val(o + j * L.plane, Form<ST>::enc(d[j], 1.f, 0.f), nt);
     }
     if (!solid || t.from_state) {
         S* o = static_cast<S*>(t.gdst) + L.f_off + idx;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) o[j * L.plane] = Form<ST>::enc(gd[j], 1.f, 0.f);
+        for (int j = 0; j < 9; ++j) store_val(o + j * L.plane, Form<ST>::enc(gd[j], 1.f, 0.f), nt);
     }
 }
 
