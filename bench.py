@@ -118,7 +118,7 @@ def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False,
     return out
 
 
-def bench_thermal(nx, ny, storage, steps, warmup, torch, stream):
+def bench_thermal(nx, ny, storage, steps, warmup, torch, stream, launch=""):
     """Heated cylinder with the thermal double distribution (src/heated_cylinder.cpp at
     Re = 100, Pr = 0.71, T_wall = 1.5, buoyancy on: beta = 0.001, g = -1e-5): circle
     r = ny/10 at (nx/4, ny/2), f and g at equilibrium with u = 0.1, T = 1; the driver's
@@ -138,6 +138,8 @@ def bench_thermal(nx, ny, storage, steps, warmup, torch, stream):
     ens.upload_thermal(0, g)
     del g
     ens.set_thermal_rules(0, mask, [(1.5, G.THERMAL_EQ_REST)])
+    if launch:
+        ens.set_launch(*(int(v) for v in launch.split(",")))
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
     kms = graph_step_ms(ens, torch, replays=5)
@@ -152,6 +154,7 @@ def bench_thermal(nx, ny, storage, steps, warmup, torch, stream):
         "bytes_per_step": b,
         "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "T_centre_row_max": float(T.max()),
+        "launch_options": info.launch_options,
     }
     ens.close()
     return out
@@ -483,7 +486,7 @@ def main():
             elif args.form.startswith("thermal_"):
                 storage, size = args.form.split("_")[1:3]
                 nx, ny = (int(v) for v in size.split("x"))
-                out = bench_thermal(nx, ny, storage, 100, 10, torch, stream.cuda_stream)
+                out = bench_thermal(nx, ny, storage, 100, 10, torch, stream.cuda_stream, launch=args.launch)
             else:
                 storage = args.form.split("_")[0]
                 out = bench_form(6400, 2400, 240, 0.644, storage, 100, 10, torch, stream.cuda_stream, les=True,

@@ -1271,6 +1271,9 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
 
 // ------------------------------------------------------------------ thermal populations
 
+// Thermal kernel tiles: kThermalTileW nodes wide, 256 / kThermalTileW rows, one node per thread.
+constexpr int kThermalTileW = 256;
+
 // Post-stream thermal populations (as deviations g - w) of node (x,y), pulled from the
 // post-collision planes gs (src/lbm.hpp:489-528): y always periodic, x periodic on a
 // periodic x edge; a source off the lattice or solid means the population bounced back
@@ -1372,7 +1375,7 @@ __device__ __forceinline__ void thermal_collide(const ThermalArgs& t, R d[9], R 
     }
 }
 
-// One lattice with thermal populations, one node per thread over 64 x 4 tiles of rows
+// One lattice with thermal populations, one node per thread over kThermalTileW-wide tiles of rows
 // [t.y0, t.y1): the state of f and g from the plain pulls (interior nodes that are not
 // special) or from the general path and thermal_gather, then the node conditions;
 // then, by t.mode, the collisions
@@ -1385,8 +1388,16 @@ __global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
     const LatticeDev& L = a.L0;
-    const int x = static_cast<int>(blockIdx.x) * 64 + (static_cast<int>(threadIdx.x) & 63);
-    const int y = t.y0 + static_cast<int>(blockIdx.y) * 4 + (static_cast<int>(threadIdx.x) >> 6);
+    // tiles row-major, each XCD taking one contiguous range of them (the blocks of a
+    // launch go round-robin over the 8 XCDs): the rows a tile pulls from its neighbours
+    // above and below are mostly in the same L2
+    int b = static_cast<int>(blockIdx.x);
+    const int nb = static_cast<int>(gridDim.x);
+    const int q = nb >> 3, r = nb & 7, xcd = b & 7;
+    b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+    const int tiles_x = (L.nx + kThermalTileW - 1) / kThermalTileW;
+    const int x = (b % tiles_x) * kThermalTileW + (static_cast<int>(threadIdx.x) % kThermalTileW);
+    const int y = t.y0 + (b / tiles_x) * (256 / kThermalTileW) + (static_cast<int>(threadIdx.x) / kThermalTileW);
     if (x >= L.nx || y >= t.y1) return;
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
     const S* fsrc = static_cast<const S*>(a.src) + L.f_off;
@@ -1431,8 +1442,7 @@ __global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
         if (t.mode == THERMAL_STEP && a.nf && (fl & CELL_LINKS) && L.forces) {
             LinkVals<R> ls_ = lg;
             if (gx != x) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
-            double* accp = a.acc + static_cast<int64_t>((blockIdx.y * gridDim.x + blockIdx.x) % kAccCopies) *
-                                       a.total_slots * 2;
+            double* accp = a.acc + static_cast<int64_t>(blockIdx.x % kAccCopies) * a.total_slots * 2;
             node_forces<ST>(L, c, ForceSink{a.nf, accp}, ls_, d);
         }
         thermal_gather<ST>(L, gsrc, a.flags, x, y, solid, gd);
@@ -1692,7 +1702,8 @@ hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream
 
 hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s)
 {
-    dim3 grid((a.L0.nx + 63) / 64, (t.y1 - t.y0 + 3) / 4);
+    const int rows = 256 / kThermalTileW;
+    dim3 grid(((a.L0.nx + kThermalTileW - 1) / kThermalTileW) * ((t.y1 - t.y0 + rows - 1) / rows));
     switch (storage) {
     case ST_F64: hipLaunchKernelGGL(thermal_kernel<ST_F64>, grid, dim3(256), 0, s, a, t); break;
     case ST_F32: hipLaunchKernelGGL(thermal_kernel<ST_F32>, grid, dim3(256), 0, s, a, t); break;
