@@ -8,6 +8,8 @@
 //   wdist        uint16 wall distance per node (LES lattices)
 //   forces       per launch of a batch: FP64 force terms per link record and fallback
 //                accumulators per slot; a per-slot history ring filled by the flush
+//   thermal      (one-lattice handles) two buffers of g planes with the populations'
+//                layout and parity, one condition byte per node
 // and builds them from the reference's obstacle/geometry description
 // (src/lbm_types.hpp:90-182, src/lbm.hpp:79-87, 931-964).
 #include <hip/hip_runtime.h>

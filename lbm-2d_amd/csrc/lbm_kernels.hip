@@ -18,7 +18,9 @@
 // coalesced row loads, 9 aligned stores and one flag byte.  Every other node takes
 // the general path, which handles links, edges and boundary conditions exactly as
 // the reference's branches do.  step2_kernel advances one lattice two steps per launch
-// through LDS.
+// through LDS.  thermal_kernel steps a lattice that carries the thermal populations g
+// as well (src/lbm.hpp:356-397, 489-528): the same pull form for f, the adiabatic
+// streaming of g, the thermal collision and the Boussinesq forcing of f.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
