@@ -5,7 +5,7 @@
 //   flags        one byte per node (solid / special / links / write)
 //   link_slot    int32 per node: index of the node's boundary-link record
 //   links        one LinkRec per fluid node with links (q, scheme, wall velocity, slot)
-//   wdist        uint16 wall distance per node (LES lattices)
+//   wdist        uint16 wall distance per node, computed on the device from the flags
 //   forces       per launch of a batch: FP64 force terms per link record and fallback
 //                accumulators per slot; a per-slot history ring filled by the flush
 //   thermal      (one-lattice handles) two buffers of g planes with the populations'
@@ -111,8 +111,6 @@ struct HostLattice {
     std::vector<LinkRec> recs;
     std::vector<uint8_t> flags;      // pitch layout
     std::vector<int32_t> lslot;      // pitch layout
-    std::vector<uint16_t> wdist;     // pitch layout (LES only)
-    std::vector<double> wall_dist;   // nx*ny (LES only)
     std::vector<int32_t> special;    // y*pitch+x of the general-path nodes
     int64_t special_off = 0;
     int64_t fluid = 0;
@@ -228,36 +226,6 @@ int d2h(Ensemble* E, void* dst, const void* src, size_t bytes)
     return LBM_OK;
 }
 
-// Multi-source BFS over the 4-neighbour graph from every solid node, non-periodic
-// (the fixpoint of the relaxation in src/lbm_types.hpp:231-264); 1e9 = unreachable.
-void wall_distance(const std::vector<uint8_t>& obs, int nx, int ny, std::vector<double>& d)
-{
-    const int64_t n = static_cast<int64_t>(nx) * ny;
-    d.assign(n, 1e9);
-    std::vector<int64_t> q;
-    q.reserve(n);
-    for (int64_t c = 0; c < n; ++c)
-        if (obs[c]) {
-            d[c] = 0.0;
-            q.push_back(c);
-        }
-    for (size_t h = 0; h < q.size(); ++h) {
-        const int64_t c = q[h];
-        const int x = static_cast<int>(c % nx), y = static_cast<int>(c / nx);
-        const int nbx[4] = {x - 1, x + 1, x, x}, nby[4] = {y, y, y - 1, y + 1};
-        for (int k = 0; k < 4; ++k) {
-            if (nbx[k] < 0 || nbx[k] >= nx || nby[k] < 0 || nby[k] >= ny) continue;
-            const int64_t m = static_cast<int64_t>(nby[k]) * nx + nbx[k];
-            if (d[m] > d[c] + 1.0) {
-                d[m] = d[c] + 1.0;
-                q.push_back(m);
-            }
-        }
-    }
-}
-
-// Neighbour of (x,y) along e_i under the lattice's edge rules.  Returns false when
-// the population leaves through an OPEN or WALL edge (no boundary link then).
 bool neighbour(const lbm_desc& d, int x, int y, int i, int& tx, int& ty)
 {
     tx = x + CX[i];
@@ -389,19 +357,6 @@ int build_geometry(HostLattice& H)
             if ((f & CELL_SOLID) && (f & CELL_WRITE))
                 H.special.push_back(static_cast<int32_t>(static_cast<int64_t>(y) * H.pitch + x));
         }
-    if (d.les) {
-        wall_distance(H.obstacle, nx, ny, H.wall_dist);
-        H.wdist.assign(static_cast<size_t>(ny) * H.pitch, 0xffff);
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) {
-                const double v = H.wall_dist[static_cast<int64_t>(y) * nx + x];
-                H.wdist[static_cast<int64_t>(y) * H.pitch + x] =
-                    v >= 65535.0 ? 0xffff : static_cast<uint16_t>(v);
-            }
-    } else {
-        H.wdist.clear();
-        H.wall_dist.clear();
-    }
     H.geom_dirty = false;
     return LBM_OK;
 }
@@ -482,12 +437,10 @@ int commit(Ensemble* E)
             all.reserve(nrec);
             std::vector<uint8_t> flags(E->cell_elems, 0);
             std::vector<int32_t> lslot(E->cell_elems, -1);
-            std::vector<uint16_t> wd(E->cell_elems, 0xffff);
             for (auto& H : E->lat) {
                 all.insert(all.end(), H.recs.begin(), H.recs.end());
                 std::copy(H.flags.begin(), H.flags.end(), flags.begin() + H.cell_off);
                 std::copy(H.lslot.begin(), H.lslot.end(), lslot.begin() + H.cell_off);
-                if (!H.wdist.empty()) std::copy(H.wdist.begin(), H.wdist.end(), wd.begin() + H.cell_off);
             }
             if ((rc = h2d(E, E->d_links, all.data(), all.size() * sizeof(LinkRec))) != LBM_OK) return rc;
             // each record's force terms go to its primary slot: that of its first counted link
@@ -528,7 +481,11 @@ int commit(Ensemble* E)
             E->graph_dirty = true;
             if ((rc = h2d(E, E->d_flags, flags.data(), flags.size())) != LBM_OK) return rc;
             if ((rc = h2d(E, E->d_lslot, lslot.data(), lslot.size() * 4)) != LBM_OK) return rc;
-            if ((rc = h2d(E, E->d_wdist, wd.data(), wd.size() * 2)) != LBM_OK) return rc;
+            // wall distance of every lattice from its flags, on the device (read by the
+            // LES lattices' collisions and by lbm_wall_distance)
+            for (auto& H : E->lat)
+                LBM_HIP(launch_wall_distance(E->d_flags + H.cell_off, E->d_wdist + H.cell_off, H.d.nx, H.d.ny,
+                                             H.pitch, E->stream));
             if (slots != E->total_slots) {
                 E->total_slots = slots;
                 const size_t dummy_n = static_cast<size_t>(slots) * kAccCopies * 2;
@@ -1842,11 +1799,15 @@ int lbm_wall_distance(lbm_handle h, int32_t l, double* out)
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!out) return fail(LBM_ERR_INVALID, "null output");
+    if ((rc = lbm::commit(h)) != LBM_OK) return rc;
     auto& H = h->lat[l];
-    if (H.obstacle.empty()) H.obstacle.assign(static_cast<size_t>(H.d.nx) * H.d.ny, 0);
-    std::vector<double> d;
-    lbm::wall_distance(H.obstacle, H.d.nx, H.d.ny, d);
-    std::memcpy(out, d.data(), d.size() * 8);
+    std::vector<uint16_t> w(static_cast<size_t>(H.d.ny) * H.pitch);
+    if ((rc = lbm::d2h(h, w.data(), h->d_wdist + H.cell_off, w.size() * 2)) != LBM_OK) return rc;
+    for (int y = 0; y < H.d.ny; ++y)
+        for (int x = 0; x < H.d.nx; ++x) {
+            const uint16_t v = w[static_cast<int64_t>(y) * H.pitch + x];
+            out[static_cast<int64_t>(y) * H.d.nx + x] = v == 0xffff ? 1e9 : static_cast<double>(v);
+        }
     return LBM_OK;
 }
 

@@ -142,6 +142,9 @@ struct ThermalLoadArgs {
     void* state_dst;
 };
 
+// Wall distance (uint16, 0xffff = none within 65534) of one lattice's nodes from its
+// flag bytes; both pointers at the lattice's first cell.
+hipError_t launch_wall_distance(const uint8_t* flags, uint16_t* wdist, int nx, int ny, int pitch, hipStream_t s);
 hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s);
 hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s);
 

@@ -1507,6 +1507,89 @@ __global__ __launch_bounds__(256) void thermal_load_kernel(ThermalLoadArgs a)
     for (int j = 0; j < 9; ++j) o[j * L.plane] = Form<ST>::enc(static_cast<R>((in ? in[j] : 1.0) - WD[j]), 1.f, 0.f);
 }
 
+// ------------------------------------------------------------------ wall distance
+
+// The wall distance of src/lbm_types.hpp:231-264 -- the fixpoint of d = min(d, d_nbr + 1)
+// over the 4 neighbours, solids 0, non-periodic -- is the 4-neighbour path length to the
+// nearest solid node; a shortest path to the L1-nearest solid never needs to pass
+// another solid first, so it equals the L1 distance transform
+// d(x, y) = min over y' of (|y - y'| + h(x, y')), h the distance to the nearest solid in
+// row y'.  Two passes, stored as uint16 saturating at 0xffff (no solid reachable, or
+// 65535 and more).
+
+constexpr uint32_t kFarWall = 0xffffu;
+
+__device__ __forceinline__ uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// h of every node of row blockIdx.x: each of the 256 threads scans a contiguous chunk
+// for its last and first solid, a scan over the chunks in LDS gives the nearest solid
+// left and right of each chunk, and a second pass over the chunk writes min(x - left,
+// right - x).
+__global__ __launch_bounds__(256) void wall_rows_kernel(const uint8_t* flags, uint16_t* wd, int nx, int pitch)
+{
+    __shared__ int last[256], first[256];
+    const int y = static_cast<int>(blockIdx.x);
+    const int t = static_cast<int>(threadIdx.x);
+    const int chunk = (nx + 255) / 256;
+    const int x0 = min(nx, t * chunk), x1 = min(nx, x0 + chunk);
+    const uint8_t* f = flags + static_cast<int64_t>(y) * pitch;
+    int lo = -1, hi = -1;
+    for (int x = x0; x < x1; ++x)
+        if (f[x] & CELL_SOLID) {
+            if (lo < 0) lo = x;
+            hi = x;
+        }
+    last[t] = hi;                  // -1: none
+    first[t] = lo < 0 ? nx : lo;   // nx: none
+    __syncthreads();
+    if (t == 0) {
+        // exclusive scans: nearest solid left of chunk k (max), right of chunk k (min)
+        int run = -1;
+        for (int k = 0; k < 256; ++k) {
+            const int v = last[k];
+            last[k] = run;
+            run = max(run, v);
+        }
+        run = nx;
+        for (int k = 255; k >= 0; --k) {
+            const int v = first[k];
+            first[k] = run;
+            run = min(run, v);
+        }
+    }
+    __syncthreads();
+    int left = last[t], right = first[t];
+    uint16_t* o = wd + static_cast<int64_t>(y) * pitch;
+    for (int x = x0; x < x1; ++x) {
+        if (f[x] & CELL_SOLID) left = x;
+        o[x] = static_cast<uint16_t>(left < 0 ? kFarWall : umin32(kFarWall, static_cast<uint32_t>(x - left)));
+    }
+    for (int x = x1 - 1; x >= x0; --x) {
+        if (f[x] & CELL_SOLID) right = x;
+        if (right < nx) o[x] = static_cast<uint16_t>(umin32(o[x], umin32(kFarWall, static_cast<uint32_t>(right - x))));
+    }
+}
+
+// d(x, y) = min over y' of h(x, y') + |y - y'|: a forward and a backward min-plus scan down
+// column x (one thread per column, so the rows are read coalesced), in place.
+__global__ __launch_bounds__(256) void wall_cols_kernel(uint16_t* wd, int nx, int ny, int pitch)
+{
+    const int x = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (x >= nx) return;
+    uint16_t* c = wd + x;
+    uint32_t d = c[0];
+    for (int y = 1; y < ny; ++y) {
+        uint16_t* p = c + static_cast<int64_t>(y) * pitch;
+        d = umin32(*p, umin32(kFarWall, d + 1));
+        *p = static_cast<uint16_t>(d);
+    }
+    for (int y = ny - 2; y >= 0; --y) {
+        uint16_t* p = c + static_cast<int64_t>(y) * pitch;
+        d = umin32(*p, umin32(kFarWall, d + 1));
+        *p = static_cast<uint16_t>(d);
+    }
+}
+
 // Advances the per-lattice step counters after a batch of launches; an odd number of
 // launches (a two-step launch swaps the buffers once) leaves the state in the other buffer.
 __global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
@@ -1699,6 +1782,15 @@ hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream
     case ST_BF16S: hipLaunchKernelGGL(prologue_kernel<ST_BF16S>, grid, dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_wall_distance(const uint8_t* flags, uint16_t* wdist, int nx, int ny, int pitch, hipStream_t s)
+{
+    hipLaunchKernelGGL(wall_rows_kernel, dim3(ny), dim3(256), 0, s, flags, wdist, nx, pitch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(wall_cols_kernel, dim3((nx + 255) / 256), dim3(256), 0, s, wdist, nx, ny, pitch);
     return hipGetLastError();
 }
 

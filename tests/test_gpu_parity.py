@@ -171,6 +171,35 @@ def test_wall_distance_matches_oracle():
     np.testing.assert_array_equal(ens.wall_distance(0), O.wall_distance(o.obstacle))
 
 
+@pytest.mark.parametrize("kind", ["random", "none", "corner", "walls", "wide"])
+def test_device_wall_distance_equals_bfs(kind):
+    """The device distance transform against the oracle's BFS (the fixpoint of the
+    reference's sweeps) on lattices of several shapes -- no solid, one far corner, walls,
+    rows longer than one chunk per thread -- in an ensemble, so lattice offsets count."""
+    import oracle as O
+    rng = np.random.default_rng({"random": 1, "none": 2, "corner": 3, "walls": 4, "wide": 5}[kind])
+    shapes = {"random": (70, 45), "none": (33, 20), "corner": (300, 7), "walls": (64, 64), "wide": (1100, 9)}
+    nx, ny = shapes[kind]
+    m = np.zeros((ny, nx), np.uint8)
+    if kind == "random":
+        m[rng.random((ny, nx)) < 0.04] = 1
+    elif kind == "corner":
+        m[ny - 1, 0] = 1
+    elif kind == "walls":
+        m[0, :] = m[:, 0] = m[:, nx - 1] = 1
+    elif kind == "wide":
+        m[3, 517] = m[0, 1099] = 1
+    other = G.desc_for_case(G.CYLINDER, 40, 30, tau=0.6)
+    d = G.desc_for_case(G.RIBS, nx, ny, tau=0.6, les=True, cs=0.1)
+    ens = G.Ensemble([other, d], storage="f64")
+    om = np.zeros((30, 40), np.uint8)
+    om[10:14, 10:14] = 1
+    ens.set_obstacle(0, om)
+    ens.set_obstacle(1, m)
+    np.testing.assert_array_equal(ens.wall_distance(1), O.wall_distance(m))
+    np.testing.assert_array_equal(ens.wall_distance(0), O.wall_distance(om))
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16s"])
 def test_every_launch_shape_and_option_is_bitwise_identical(storage):
     """Each compiled (block, rows, nodes per thread) shape, with and without the XCD
