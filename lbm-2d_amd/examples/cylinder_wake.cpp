@@ -8,13 +8,15 @@
 //     MODE:    step (execute_time_step per step) | run (lbm_gpu::run_steps)
 //
 // Prints one JSON line per step ({"step","fx","fy"}) and a final line with the
-// population checksums.
+// population checksums and the force analysis of analysis.hpp (mean Cd and Cl amplitude
+// over the latter half, Strouhal number from Cl's crossings; -1 where undefined).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
 
+#include "analysis.hpp"
 #include "lbm.hpp"
 
 int main(int argc, char** argv)
@@ -53,19 +55,21 @@ int main(int argc, char** argv)
             for (int i = 0; i < 9; ++i) fn[i] = compute_equilibrium(i, rho, u, v + dv);
         }
 
+    std::vector<double> fx, fy;
     try {
         if (mode == "step") {
             for (int s = 0; s < steps; ++s) {
                 execute_time_step(sys, tau, u_in);
-                double fx = 0.0, fy = 0.0;
+                double sx = 0.0, sy = 0.0;
                 for (int n = 0; n < NX * NY; ++n) {
-                    fx += sys.fx_body[n];
-                    fy += sys.fy_body[n];
+                    sx += sys.fx_body[n];
+                    sy += sys.fy_body[n];
                 }
-                std::printf("{\"step\":%d,\"fx\":%.17g,\"fy\":%.17g}\n", s, fx, fy);
+                fx.push_back(sx);
+                fy.push_back(sy);
+                std::printf("{\"step\":%d,\"fx\":%.17g,\"fy\":%.17g}\n", s, sx, sy);
             }
         } else {
-            std::vector<double> fx, fy;
             lbm_gpu::run_steps(sys, steps, tau, u_in, &fx, &fy);
             for (size_t s = 0; s < fx.size(); ++s) std::printf("{\"step\":%zu,\"fx\":%.17g,\"fy\":%.17g}\n", s, fx[s], fy[s]);
         }
@@ -78,6 +82,12 @@ int main(int argc, char** argv)
         sum += v;
         sq += v * v;
     }
-    std::printf("{\"sum\":%.17g,\"sumsq\":%.17g}\n", sum, sq);
+    std::vector<double> cd, cl;
+    lbm_analysis::coefficients(fx, fy, 2.0 * r, u_in, cd, cl);
+    double cd_mean = 0.0, cl_amp = 0.0;
+    lbm_analysis::late_stats(cd, cl, cd_mean, cl_amp);
+    const auto st = lbm_analysis::strouhal_crossings(cl, 2.0 * r, u_in);
+    std::printf("{\"sum\":%.17g,\"sumsq\":%.17g,\"cd_mean\":%.17g,\"cl_amp\":%.17g,\"st\":%.17g}\n", sum, sq,
+                cd_mean, cl_amp, st ? *st : -1.0);
     return 0;
 }

@@ -7,6 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import lbm2d as G
 import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -45,6 +46,11 @@ def test_cylinder_driver_f64_matches_oracle(mode):
     got = np.array([[p["fx"], p["fy"]] for p in per_step])
     np.testing.assert_allclose(got, fx, rtol=0, atol=1e-10)
     assert abs(chk["sum"] - s) < 1e-9 and abs(chk["sumsq"] - sq) < 1e-9
+    # the driver's C++ force analysis (analysis.hpp) on its own history = lbm2d's on the oracle's
+    cd, cl = G.coefficients(fx[:, 0], fx[:, 1], 12.0, 0.1)
+    # (forces agree to 1e-10, i.e. coefficients to 2e-9)
+    assert chk["cd_mean"] == pytest.approx(G.late_stats(cd, cl)[0], rel=0, abs=2e-9)
+    assert chk["cl_amp"] == pytest.approx(G.late_stats(cd, cl)[1], rel=0, abs=2e-9)
 
 
 def test_cylinder_driver_f32_close_to_oracle():
