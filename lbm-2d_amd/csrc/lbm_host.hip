@@ -385,6 +385,12 @@ LatticeDev make_dev(const HostLattice& H, int index, float scale)
     L.outlet = d.outlet;
     L.lid = d.lid;
     L.forces = d.forces;
+    L.mei_static = 1;
+    for (const LinkRec& r : H.recs) {
+        if (r.scheme & LINK_MOVING) L.mei_static = 0;
+        for (int i = 1; i < 9; ++i)
+            if ((r.mask >> i) & 1u && ((r.scheme >> (2 * i)) & 3u) != static_cast<uint32_t>(LBM_BB_MEI)) L.mei_static = 0;
+    }
     // MRTParams::from_tau (src/lbm_types.hpp:72-76)
     double s = 1.0 / d.tau;
     L.s_shear = s < 0.5 ? 0.5 : (s > 1.99 ? 1.99 : s);

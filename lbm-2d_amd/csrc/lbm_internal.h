@@ -32,7 +32,7 @@ struct alignas(16) LatticeDev {
     int32_t slot_base;    // first force slot of this lattice
     int32_t n_slots;
     int32_t collision, les, x_edge, y_edge, inlet, inlet_y0, inlet_h, outlet, lid, forces;
-    int32_t pad0;
+    int32_t mei_static;  // every boundary link is Mei with a resting wall
     double s_shear;                    // MRT shear rate; s_bulk = 1.2, s_normal = 1 are fixed
                                        // (MRTParams::from_tau, src/lbm_types.hpp:72-76)
     double tau, inv_tau, u_in, cs, cs2, nu, inv_nu, fscale;

@@ -389,6 +389,26 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
         rho_own += s;
     }
 
+    if (L.mei_static && mask && px >= 1 && px <= L.nx - 2 && py >= 1 && py <= L.ny - 2) {
+        // every link Mei with a resting wall and the node off the edges: each slot is its
+        // link's Mei value (the general expression below with u_w = 0) or the plain pull,
+        // selected without branches
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int i = OPP[j];
+            const R q = lk.q[i];
+            const R uwx = R(0), uwy = R(0);
+            const R eu = R(CX[i]) * uwx + R(CY[i]) * uwy;
+            const R uu = uwx * uwx + uwy * uwy;
+            const R feq_dev =
+                wgt<R>(i) * (rho_own * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * uu) - R(1));
+            const R mei = q * feq_dev + (R(1) - q) * own[i] + (R(2) * q - R(1)) * wgt<R>(i) * rho_own * eu * R(3);
+            const R pull = Form<ST>::dec(p.pull[j], c.inv_scale, L.shift[j]);
+            d[j] = ((mask >> i) & 1u) ? mei : pull;
+        }
+        return;
+    }
+
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         const int i = OPP[j];  // push direction whose target is px - e_j
