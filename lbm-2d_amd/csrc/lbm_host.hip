@@ -348,6 +348,20 @@ int build_geometry(HostLattice& H)
         for (size_t p = 0; p < H.flags.size(); ++p)
             if (H.flags[p] & CELL_SOLID) H.flags[p] |= CELL_WRITE;
     }
+    // inlet and outlet nodes the one-lattice kernel's bulk tiles take: off the corners,
+    // fluid, without links; at the outlet the copied node (nx-2) as well.  Their state is
+    // the plain pulls plus the inlet closure (the closure overwrites exactly the slots
+    // whose sources lie off the lattice), or the plain pulls of the copied node.
+    if (d.x_edge == LBM_EDGE_OPEN && nx >= 4)
+        for (int y = 1; y + 1 < ny; ++y) {
+            uint8_t& in = H.flags[static_cast<int64_t>(y) * H.pitch];
+            const bool closed = d.inlet == LBM_INLET_ZOU_HE || (d.inlet == LBM_INLET_PARABOLIC && y >= d.inlet_y0);
+            if (closed && !(in & (CELL_SOLID | CELL_LINKS))) in |= CELL_COLUMN;
+            uint8_t& out = H.flags[static_cast<int64_t>(y) * H.pitch + nx - 1];
+            const uint8_t src = H.flags[static_cast<int64_t>(y) * H.pitch + nx - 2];
+            if (d.outlet && !(out & (CELL_SOLID | CELL_LINKS)) && !(src & (CELL_SOLID | CELL_LINKS)))
+                out |= CELL_COLUMN;
+        }
     // written solids off the edges, for the one-lattice kernel's list blocks (its link
     // and edge blocks cover the other general-path nodes)
     H.special.clear();

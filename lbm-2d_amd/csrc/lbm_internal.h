@@ -17,6 +17,8 @@ enum : uint8_t {
     CELL_SPECIAL = 1u << 1,  // needs the general path (edge, boundary link, solid)
     CELL_LINKS = 1u << 2,    // fluid node with >= 1 boundary link (link record exists)
     CELL_WRITE = 1u << 3,    // solid node whose stored value must be kept current
+    CELL_COLUMN = 1u << 4,   // inlet / outlet node off the corners whose general path reduces
+                             // to plain pulls (+ the inlet closure, or shifted one column)
 };
 
 // Population storage forms (match enum lbm_storage).

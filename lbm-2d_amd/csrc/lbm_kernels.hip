@@ -485,6 +485,19 @@ template <int ST> __device__ __forceinline__ int gathered_x(const LatticeDev& L,
     return L.outlet && x == L.nx - 1 ? L.nx - 2 : x;  // enforce_outflow copies x = nx-2
 }
 
+// The velocity inlet at row y of a fluid node on x = 0: Zou/He with u_in, or the
+// parabolic profile on rows y >= inlet_y0 (src/lbm.hpp:21-58).
+template <typename R> __device__ __forceinline__ void inlet_closure(const LatticeDev& L, int y, R d[9])
+{
+    if (L.inlet == INLET_ZOU_HE) {
+        zou_he<R>(d, static_cast<R>(L.u_in));
+    } else if (y >= L.inlet_y0) {
+        const double yy = static_cast<double>(y - L.inlet_y0);
+        const double h = static_cast<double>(L.inlet_h);
+        zou_he<R>(d, static_cast<R>(L.u_in * 4.0 * yy * (h - yy) / (h * h)));
+    }
+}
+
 template <int ST>
 __device__ __forceinline__ void state_general(const Ctx<ST>& c, int x, int y, bool solid, const Pre<ST>& p,
                                               const LinkVals<typename Form<ST>::real>& lk,
@@ -496,15 +509,7 @@ __device__ __forceinline__ void state_general(const Ctx<ST>& c, int x, int y, bo
     gather_general<ST>(c, gx, y, p, lk, d);
     if (gx == 0 && L.inlet != INLET_NONE) {
         const bool gsolid = (p.fl & CELL_SOLID) != 0;
-        if (!gsolid) {
-            if (L.inlet == INLET_ZOU_HE) {
-                zou_he<R>(d, static_cast<R>(L.u_in));
-            } else if (y >= L.inlet_y0) {
-                const double yy = static_cast<double>(y - L.inlet_y0);
-                const double h = static_cast<double>(L.inlet_h);
-                zou_he<R>(d, static_cast<R>(L.u_in * 4.0 * yy * (h - yy) / (h * h)));
-            }
-        }
+        if (!gsolid) inlet_closure<R>(L, y, d);
     }
     if (L.lid && y == L.ny - 1 && !solid) {
         // enforce_lid: feq(rho, u_lid, 0) (src/lbm.hpp:110-120)
@@ -686,7 +691,7 @@ __device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a
     Pre<ST> p;
     preload<ST>(c, gx, y, p);
     const uint16_t wraw = L.les ? a.wdist[L.cell_off + idx] : uint16_t(0xffffu);
-    if (!is_link && (fl & CELL_LINKS)) return;
+    if (!is_link && (fl & (CELL_LINKS | CELL_COLUMN))) return;
     if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return;
     LinkVals<R> lg = own_links;
     if (!is_link || gx != x) links_of<ST>(c, p, lg);
@@ -707,14 +712,16 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     // more than one row), and the asm below keeps the loads ahead of the branch.
     const uint8_t fl = a.flags[L.cell_off + idx];
     S raw[9];
-    const S* b = c.base + idx;
+    // on the outlet column the pulls of the copied node x - 1 (enforce_outflow)
+    const S* b = c.base + idx - ((SKIP_SPECIAL && L.outlet && x == L.nx - 1) ? 1 : 0);
 #pragma unroll
     for (int j = 0; j < 9; ++j) raw[j] = b[j * L.plane - (CY[j] * L.pitch + CX[j])];
     uint16_t wraw = 0xffffu;
     if (L.les) wraw = a.wdist[L.cell_off + idx];
     __asm__ volatile("" ::"v"(raw[0]), "v"(raw[1]), "v"(raw[2]), "v"(raw[3]), "v"(raw[4]), "v"(raw[5]),
                      "v"(raw[6]), "v"(raw[7]), "v"(raw[8]), "v"(wraw));
-    if (fl & CELL_SPECIAL) {
+    const bool column = SKIP_SPECIAL && (fl & CELL_COLUMN);
+    if ((fl & CELL_SPECIAL) && !column) {
         if (SKIP_SPECIAL) return;
         if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return;
         const int gx = gathered_x<ST>(L, x);
@@ -730,6 +737,8 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     R d[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(raw[j], a.inv_scale, L.shift[j]);
+    // an inlet node: the closure replaces the three slots whose sources lie off the lattice
+    if (column && x == 0) inlet_closure<R>(L, y, d);
     R wd = R(0);
     if (L.les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
     collide<R>(d, L, wd, R(1));
