@@ -47,6 +47,8 @@ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 
 constexpr int kHistoryRows = 4096;
+// Device time per candidate launch shape in lbm_autotune (ms).
+constexpr float kTuneMs = 1.5f;
 constexpr int kMinBatch = 32;  // accumulator blocks when graphs are shorter or off
 constexpr int64_t kStagingBytes = 256ll << 20;
 
@@ -1546,21 +1548,29 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         a.acc = E->d_dummy_acc;
         a.nf = E->d_dummy_nf;
         a.nan_flags = E->d_dummy_nan;
-        const int reps = 10;
         hipError_t le = hipSuccess;
         for (int w = 0; w < 2 && le == hipSuccess; ++w)
             le = launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream);
-        (void)hipEventRecord(e0, E->stream);
-        for (int r = 0; r < reps && le == hipSuccess; ++r)
-            le = launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream);
-        (void)hipEventRecord(e1, E->stream);
-        if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        // ten launches for an estimate, then enough for ~1.5 ms of device time, so that
+        // shapes a few percent apart on small lattices are told apart
+        float ms = 0.f;
+        bool ok = le == hipSuccess;
+        for (int pass = 0, reps = 10; pass < 2 && ok; ++pass) {
+            (void)hipEventRecord(e0, E->stream);
+            for (int r = 0; r < reps && le == hipSuccess; ++r)
+                le = launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream);
+            (void)hipEventRecord(e1, E->stream);
+            ok = le == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            if (!ok) break;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= reps;
+            if (ms * reps >= kTuneMs) break;
+            reps = std::min(2000, static_cast<int>(kTuneMs / std::max(ms, 1e-4f)) + 1);
+        }
+        if (!ok) {
             rc = fail(LBM_ERR_DEVICE, "autotune launch failed");
             break;
         }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= reps;
         if (ms < best) {
             best = ms;
             bs[0] = E->bx;
@@ -1589,19 +1599,26 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         a.acc = a.acc2 = E->d_dummy_acc;
         a.nf = a.nf2 = E->d_dummy_nf;
         a.nan_flags = E->d_dummy_nan;
-        const int reps = 5;
         hipError_t le = hipSuccess;
         for (int w = 0; w < 2 && le == hipSuccess; ++w) le = launch_step2(E->storage, rows, blocks, a, E->stream);
-        (void)hipEventRecord(e0, E->stream);
-        for (int r = 0; r < reps && le == hipSuccess; ++r) le = launch_step2(E->storage, rows, blocks, a, E->stream);
-        (void)hipEventRecord(e1, E->stream);
-        if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        float ms = 0.f;
+        bool ok = le == hipSuccess;
+        for (int pass = 0, reps = 5; pass < 2 && ok; ++pass) {
+            (void)hipEventRecord(e0, E->stream);
+            for (int r = 0; r < reps && le == hipSuccess; ++r) le = launch_step2(E->storage, rows, blocks, a, E->stream);
+            (void)hipEventRecord(e1, E->stream);
+            ok = le == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            if (!ok) break;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= reps;
+            if (ms * reps >= kTuneMs) break;
+            reps = std::min(1000, static_cast<int>(kTuneMs / std::max(ms, 1e-4f)) + 1);
+        }
+        if (!ok) {
             rc = fail(LBM_ERR_DEVICE, "autotune launch failed");
             break;
         }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= 2 * reps;
+        ms /= 2;  // per step
         if (ms < best) {
             best = ms;
             bs[0] = 64;
