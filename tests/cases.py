@@ -239,6 +239,9 @@ THERMAL_CASES = {
     "lid": ThermalCase("lid", CASES["cavity"], 1.4, 0.005, -2e-3, 1.0, _blob(16, 12, 0.4, 20.0), steps=80),
     # periodic hills: the thermal populations wrap in x too
     "hills": ThermalCase("hills", CASES["hills"], 1.2, 0.01, -1e-3, 1.0, _blob(30, 26, 0.5, 30.0)),
+    # rotating cylinder with Bouzidi links and the Ladd wall term
+    "rotating": ThermalCase("rotating", CASES["bouzidi"], 1.0 / (0.5 + 3.0 * _alpha(0.6)), 0.01, -1e-3, 1.0,
+                            _blob(36, 30, 0.4, 50.0), walls=("solids", 1.3), steps=40),
 }
 
 

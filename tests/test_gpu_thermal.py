@@ -148,3 +148,17 @@ def test_thermal_errors():
     with pytest.raises(G.LbmError):
         ens.upload_thermal(0, np.ones((c.ny, c.nx, 9)))  # only right after f
     ens.close()
+
+
+def test_thermal_ignores_two_steps_per_launch():
+    # two steps per launch is a momentum-only kernel: a thermal handle keeps one step per
+    # launch whatever the launch options say
+    tc = THERMAL_CASES["heated"]
+    a, _ = build_gpu_thermal(tc, "f32")
+    b, _ = build_gpu_thermal(tc, "f32")
+    a.set_launch(64, 4, 1, 4, 10)
+    a.step(25)
+    b.step(25)
+    np.testing.assert_array_equal(a.download(0), b.download(0))
+    np.testing.assert_array_equal(a.download_thermal(0), b.download_thermal(0))
+    assert a.steps_done(0) == 25
