@@ -350,7 +350,7 @@ int build_geometry(HostLattice& H)
         for (size_t p = 0; p < H.flags.size(); ++p)
             if (H.flags[p] & CELL_SOLID) H.flags[p] |= CELL_WRITE;
     }
-    // inlet and outlet nodes the one-lattice kernel's bulk tiles take: off the corners,
+    // inlet and outlet nodes the step kernel's bulk tiles take: off the corners,
     // fluid, without links; at the outlet the copied node (nx-2) as well.  Their state is
     // the plain pulls plus the inlet closure (the closure overwrites exactly the slots
     // whose sources lie off the lattice), or the plain pulls of the copied node.

@@ -713,14 +713,14 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     const uint8_t fl = a.flags[L.cell_off + idx];
     S raw[9];
     // on the outlet column the pulls of the copied node x - 1 (enforce_outflow)
-    const S* b = c.base + idx - ((SKIP_SPECIAL && L.outlet && x == L.nx - 1) ? 1 : 0);
+    const S* b = c.base + idx - ((L.outlet && x == L.nx - 1) ? 1 : 0);
 #pragma unroll
     for (int j = 0; j < 9; ++j) raw[j] = b[j * L.plane - (CY[j] * L.pitch + CX[j])];
     uint16_t wraw = 0xffffu;
     if (L.les) wraw = a.wdist[L.cell_off + idx];
     __asm__ volatile("" ::"v"(raw[0]), "v"(raw[1]), "v"(raw[2]), "v"(raw[3]), "v"(raw[4]), "v"(raw[5]),
                      "v"(raw[6]), "v"(raw[7]), "v"(raw[8]), "v"(wraw));
-    const bool column = SKIP_SPECIAL && (fl & CELL_COLUMN);
+    const bool column = (fl & CELL_COLUMN) != 0;
     if ((fl & CELL_SPECIAL) && !column) {
         if (SKIP_SPECIAL) return;
         if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return;
