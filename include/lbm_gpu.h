@@ -158,6 +158,13 @@ int lbm_set_storage_shift(lbm_handle h, int32_t lattice, double u, double v);
  * exactly; rounding errors have zero mean, which removes the drift that identical
  * nodes rounding identically cause.  No counterpart in the reference (FP64 state). */
 int lbm_set_stochastic_rounding(lbm_handle h, int32_t on, uint32_t seed);
+/* The cylinder drivers' shedding trigger (src/main.cpp:67-83) on the device: every fluid
+ * node of columns [x0, x1) becomes feq(rho, u, v + dv[(x - x0)*ny + y]) of its own moments
+ * (solid nodes' entries are ignored; the drivers draw dv from std::mt19937(42) with
+ * uniform_real_distribution(-1e-4, 1e-4), x outer, y inner, fluid nodes only).  Only
+ * right after lbm_upload / lbm_init_equilibrium, before stepping: with the device-built
+ * initial state, the driver's setup without a 9*N FP64 host array. */
+int lbm_perturb_velocity(lbm_handle h, int32_t lattice, int32_t x0, int32_t x1, const double* dv);
 /* Density and velocity of rows [y0, y1) of the current state, row-major, nx*(y1-y0)
  * FP64 each (any output may be null): compute_macros (src/lbm_types.hpp:278-288) on
  * what lbm_download would return, computed on the device -- the fields the frame

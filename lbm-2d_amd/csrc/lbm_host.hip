@@ -1396,6 +1396,43 @@ int lbm_init_equilibrium(lbm_handle h, int32_t l, double rho, double u, double v
     return lbm::load_state(h, l, nullptr, rho, u, v);
 }
 
+int lbm_perturb_velocity(lbm_handle h, int32_t l, int32_t x0, int32_t x1, const double* dv)
+{
+    using namespace lbm;
+    int rc = lattice_ok(h, l);
+    if (rc) return rc;
+    Ensemble* E = h;
+    HostLattice& H = E->lat[l];
+    if (!dv) return fail(LBM_ERR_INVALID, "null perturbation array");
+    if (!H.uploaded || H.steps > 0)
+        return fail(LBM_ERR_INVALID, "perturb right after lbm_upload / lbm_init_equilibrium, before stepping");
+    if (x0 < 0 || x1 > H.d.nx || x0 >= x1) return fail(LBM_ERR_INVALID, "column range out of bounds");
+    if ((rc = commit(E)) != LBM_OK) return rc;
+    const size_t n = static_cast<size_t>(x1 - x0) * H.d.ny;
+    double* d_dv = nullptr;
+    if (hipMalloc(&d_dv, n * 8) != hipSuccess) return fail(LBM_ERR_NOMEM, "perturbation staging");
+    rc = h2d(E, d_dv, dv, n * 8);
+    if (rc == LBM_OK) {
+        PerturbArgs a{};
+        a.lat = E->d_lat;
+        a.lattice = l;
+        a.x0 = x0;
+        a.x1 = x1;
+        a.dv = d_dv;
+        a.state = E->buf[1 - H.par];
+        a.flags = E->d_flags;
+        a.scale = E->scale;
+        a.inv_scale = E->inv_scale;
+        hipError_t e = launch_perturb(E->storage, a, H.d.ny, E->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+        if (e != hipSuccess) rc = fail(LBM_ERR_DEVICE, std::string("perturbation: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(d_dv);
+    if (rc != LBM_OK) return rc;
+    // the stored post-collision state follows the new state
+    return E->thermal ? thermal_recollide(E) : materialize(E, l, nullptr, nullptr, true);
+}
+
 int lbm_macros(lbm_handle h, int32_t l, int32_t y0, int32_t y1, double* rho, double* u, double* v)
 {
     int rc = lbm::lattice_ok(h, l);

@@ -145,6 +145,19 @@ struct ThermalLoadArgs {
 // Wall distance (uint16, 0xffff = none within 65534) of one lattice's nodes from its
 // flag bytes; both pointers at the lattice's first cell.
 hipError_t launch_wall_distance(const uint8_t* flags, uint16_t* wdist, int nx, int ny, int pitch, hipStream_t s);
+// Transverse velocity perturbation of the stored state (columns [x0, x1) of one lattice):
+// fluid node (x, y) becomes feq(rho, u, v + dv[(x - x0) * ny + y]) of its own moments.
+struct PerturbArgs {
+    const LatticeDev* lat;
+    int32_t lattice;
+    int32_t x0, x1;
+    const double* dv;
+    void* state;
+    const uint8_t* flags;
+    float scale, inv_scale;
+};
+
+hipError_t launch_perturb(int storage, const PerturbArgs& a, int ny, hipStream_t s);
 hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s);
 hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s);
 

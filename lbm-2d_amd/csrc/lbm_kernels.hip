@@ -1300,6 +1300,41 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
     for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
 }
 
+// ------------------------------------------------------------------ initial perturbation
+
+// The cylinder drivers' shedding trigger (src/main.cpp:67-83) on the stored state: fluid
+// nodes of columns [x0, x1) take feq(rho, u, v + dv) of their own moments, computed in
+// FP64 in the reference's order (compute_macros, compute_equilibrium).
+template <int ST>
+__global__ __launch_bounds__(256) void perturb_kernel(PerturbArgs a)
+{
+#pragma clang fp contract(off)
+    using S = typename Form<ST>::store;
+    const LatticeDev L = a.lat[a.lattice];
+    const int y = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int x = a.x0 + static_cast<int>(blockIdx.y);
+    if (y >= L.ny || x >= a.x1) return;
+    const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+    if (a.flags[L.cell_off + idx] & CELL_SOLID) return;
+    S* st = static_cast<S*>(a.state) + L.f_off + idx;
+    double f[9], rho = 0.0, mx = 0.0, my = 0.0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        f[j] = WD[j] + static_cast<double>(Form<ST>::dec(st[j * L.plane], a.inv_scale, L.shift[j]));
+        rho += f[j];
+        mx += f[j] * CX[j];
+        my += f[j] * CY[j];
+    }
+    const double u = mx / rho, v = my / rho + a.dv[static_cast<int64_t>(x - a.x0) * L.ny + y];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const double eu = CX[j] * u + CY[j] * v;
+        const double uu = u * u + v * v;
+        const double feq = WD[j] * rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uu);
+        st[j * L.plane] = Form<ST>::enc(static_cast<typename Form<ST>::real>(feq - WD[j]), a.scale, L.shift[j]);
+    }
+}
+
 // ------------------------------------------------------------------ thermal populations
 
 // Thermal kernel tiles: kThermalTileW nodes wide, 256 / kThermalTileW rows, one node per thread.
@@ -1820,6 +1855,19 @@ hipError_t launch_wall_distance(const uint8_t* flags, uint16_t* wdist, int nx, i
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(wall_cols_kernel, dim3((nx + 255) / 256), dim3(256), 0, s, wdist, nx, ny, pitch);
+    return hipGetLastError();
+}
+
+hipError_t launch_perturb(int storage, const PerturbArgs& a, int ny, hipStream_t s)
+{
+    dim3 grid((ny + 255) / 256, a.x1 - a.x0);
+    switch (storage) {
+    case ST_F64: hipLaunchKernelGGL(perturb_kernel<ST_F64>, grid, dim3(256), 0, s, a); break;
+    case ST_F32: hipLaunchKernelGGL(perturb_kernel<ST_F32>, grid, dim3(256), 0, s, a); break;
+    case ST_F16S: hipLaunchKernelGGL(perturb_kernel<ST_F16S>, grid, dim3(256), 0, s, a); break;
+    case ST_BF16S: hipLaunchKernelGGL(perturb_kernel<ST_BF16S>, grid, dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

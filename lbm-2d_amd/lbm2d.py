@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
         "lbm_get_info": (C.c_int, [_h, C.POINTER(LbmInfo)]),
         "lbm_sync": (C.c_int, [_h]),
         "lbm_wall_distance": (C.c_int, [_h, i32, _dp]),
+        "lbm_perturb_velocity": (C.c_int, [_h, i32, i32, i32, _dp]),
         "lbm_set_thermal": (C.c_int, [_h, i32, dbl, dbl, dbl, dbl]),
         "lbm_set_thermal_rules": (C.c_int, [_h, i32, _u8p, C.POINTER(LbmThermalRule), i32]),
         "lbm_upload_thermal": (C.c_int, [_h, i32, _dp]),
@@ -235,6 +236,13 @@ class Ensemble:
 
     def init_equilibrium(self, l: int, rho: float = 1.0, u: float = 0.0, v: float = 0.0):
         _check(lib().lbm_init_equilibrium(self._h, l, rho, u, v))
+
+    def perturb_velocity(self, l: int, x0: int, x1: int, dv: np.ndarray):
+        """dv: (x1 - x0, ny), added to the transverse velocity of the fluid nodes of
+        columns [x0, x1) (see seed42_draws for the drivers' values)."""
+        ny, nx = self.shape(l)
+        a = np.ascontiguousarray(dv, dtype=np.float64).reshape(x1 - x0, ny)
+        _check(lib().lbm_perturb_velocity(self._h, l, x0, x1, a.ctypes.data_as(_dp)))
 
     def macros(self, l: int = 0, y0: int = 0, y1: Optional[int] = None):
         """(rho, u, v), each (y1-y0, nx) FP64, of rows [y0, y1) of the current state."""
@@ -405,6 +413,21 @@ class Mt19937:
         if r >= 1.0:
             r = 1.0 - 2.0 ** -53
         return r * (b - a) + a
+
+
+def seed42_draws(obstacle: np.ndarray, x0: int, x1: int, amp: float = 1e-4) -> np.ndarray:
+    """The drivers' perturbation values (src/main.cpp:67-83) as a (x1 - x0, ny) array for
+    Ensemble.perturb_velocity: std::mt19937(42) uniform(-amp, amp) draws, x outer, y inner,
+    one per fluid node; 0 on solid nodes."""
+    g = Mt19937(42)
+    ny, nx = obstacle.shape
+    x1 = min(nx, x1)
+    dv = np.zeros((max(0, x1 - x0), ny))
+    for x in range(x0, x1):
+        for y in range(ny):
+            if not obstacle[y, x]:
+                dv[x - x0, y] = g.uniform(-amp, amp)
+    return dv
 
 
 def perturb_seed42(f: np.ndarray, obstacle: np.ndarray, x0: int, x1: int, amp: float = 1e-4) -> None:

@@ -249,6 +249,34 @@ def test_device_equilibrium_init_equals_upload(name, storage):
     assert a.steps_done(0) == 7
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s"])
+@pytest.mark.parametrize("name", ["cylinder", "near_wall"])
+def test_device_perturbation_matches_oracle(name, storage):
+    """lbm_perturb_velocity applies the drivers' seed-42 trigger (src/main.cpp:67-83) to
+    the device-built equilibrium: the state, and the run after it, match the oracle's
+    host-side perturbation to the storage's tolerance (F64 to rounding of f - w)."""
+    from cases import apply_gpu_geometry, gpu_desc
+    c = CASES[name]
+    o = build_oracle(c)
+    x0, x1 = c.pert
+    ens = G.Ensemble([gpu_desc(c)], storage=storage)
+    apply_gpu_geometry(ens, 0, c, o.obstacle.copy())
+    ens.init_equilibrium(0, *c.init)
+    dv = G.seed42_draws(o.obstacle, x0, x1)
+    ens.perturb_velocity(0, x0, min(x1, o.nx), dv)
+    tol = {"f64": 1e-15, "f32": 2e-7, "f16s": 2e-3}[storage]
+    err0 = np.abs(ens.download(0) - o.f).max()
+    assert err0 <= tol, err0
+    ens.step(c.steps)
+    for _ in range(c.steps):
+        o.step(c.tau, c.u)
+    err = np.abs(ens.download(0) - o.f).max()
+    print(f"[{name} {storage}] perturbed start {err0:.3e}, after {c.steps} steps {err:.3e}")
+    assert err <= {"f64": 1e-12, "f32": TOL["f32"], "f16s": TOL["f16s"]}[storage]
+    with pytest.raises(G.LbmError):
+        ens.perturb_velocity(0, x0, min(x1, o.nx), dv)
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32", "bf16s"])
 def test_macros_equal_compute_macros_of_download(storage):
     """lbm_macros = compute_macros (src/lbm_types.hpp:278-288) of the downloaded state, bit

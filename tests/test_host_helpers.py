@@ -41,6 +41,38 @@ def test_perturbation_matches_oracle_bitwise():
     np.testing.assert_array_equal(f, o.f)
 
 
+def test_seed42_draws_are_the_perturbation_values():
+    """seed42_draws holds exactly the values the driver adds to v, in its (x, y) order:
+    feq(rho, u, v + dv) with them reproduces the oracle's perturbation bit for bit, and a
+    window past the edge or with solids draws nothing for those nodes."""
+    nx, ny, cx, cy, r = 120, 40, 30, 21, 4
+    o = O.Oracle(nx, ny, O.CYLINDER)
+    o.place_cylinder(cx, cy, r)
+    o.init_equilibrium(1.0, 0.1, 0.0)
+    o.perturb_seed42(cx - 8, nx)
+    mask = G.circle_mask(nx, ny, cx, cy, r)
+    dv = G.seed42_draws(mask, cx - 8, nx + 30)
+    assert dv.shape == (nx - cx + 8, ny)
+    solid = mask[:, cx - 8:].T.astype(bool)
+    assert np.all(dv[solid] == 0.0)
+    assert np.count_nonzero(dv) == np.count_nonzero(~solid)
+    f = G.equilibrium(np.ones((ny, nx)), np.full((ny, nx), 0.1), np.zeros((ny, nx)))
+    for x in range(cx - 8, nx):
+        for y in range(ny):
+            if mask[y, x]:
+                continue
+            c = f[y, x]
+            rr = 0.0
+            mx = 0.0
+            my = 0.0
+            for i in range(9):
+                rr += c[i]
+                mx += c[i] * G.CXV[i]
+                my += c[i] * G.CYV[i]
+            f[y, x] = G.equilibrium(np.array(rr), np.array(mx / rr), np.array(my / rr + dv[x - cx + 8, y]))
+    np.testing.assert_array_equal(f, o.f)
+
+
 def test_welch_matches_scipy():
     signal = pytest.importorskip("scipy.signal")
     rng = np.random.default_rng(3)
