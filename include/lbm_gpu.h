@@ -118,6 +118,18 @@ int lbm_destroy(lbm_handle h);
  * LBMCapabilities::obstacle array; it resets every link to simple bounce-back
  * counted in slot 0, the reference's behaviour without bb_geom. */
 int lbm_set_obstacle(lbm_handle h, int32_t lattice, const uint8_t* mask);
+/* The drivers' obstacle placement on the device: place_cylinder (src/lbm.hpp:931-944,
+ * solid where sqrt(dx^2+dy^2) < r) for each of the n_circles (cx, cy, r) triples, then
+ * place_polygon (src/lbm.hpp:951-964, even-odd rule of src/geometry.hpp:120-134) for each
+ * polygon: poly_n[k] vertices, consecutive (x, y) pairs in poly_xy.  Cells only become
+ * solid; each obstacle costs its bounding box, not the lattice.  The mask changes as with
+ * lbm_set_obstacle (bounce-back geometry, moving wall, force radius and explicit links
+ * are cleared: set them afterwards).  The reference's place_* also set bb_geom: here that
+ * is lbm_set_bb_circle / lbm_set_bb_polygon. */
+int lbm_place_obstacles(lbm_handle h, int32_t lattice, const double* circles, int32_t n_circles,
+                        const double* poly_xy, const int32_t* poly_n, int32_t n_polys);
+/* The lattice's obstacle mask, nx*ny bytes (1 = solid). */
+int lbm_get_obstacle(lbm_handle h, int32_t lattice, uint8_t* mask);
 /* The reference's single interpolated-BB geometry (BounceBackGeometry with a circle
  * or polygon, src/lbm_types.hpp:90-176): q of every link from that one shape, 1.0
  * where the link misses it; scheme LBM_BB_MEI (the reference default,

@@ -1193,6 +1193,106 @@ int lbm_set_obstacle(lbm_handle h, int32_t l, const uint8_t* mask)
     return LBM_OK;
 }
 
+int lbm_place_obstacles(lbm_handle h, int32_t l, const double* circles, int32_t n_circles, const double* poly_xy,
+                        const int32_t* poly_n, int32_t n_polys)
+{
+    using namespace lbm;
+    int rc = lattice_ok(h, l);
+    if (rc) return rc;
+    if (n_circles < 0 || n_polys < 0) return fail(LBM_ERR_INVALID, "negative obstacle count");
+    if ((n_circles && !circles) || (n_polys && (!poly_xy || !poly_n))) return fail(LBM_ERR_INVALID, "null obstacle array");
+    int64_t n_vert = 0;
+    for (int k = 0; k < n_polys; ++k) {
+        if (poly_n[k] < 1) return fail(LBM_ERR_INVALID, "a polygon needs at least one vertex");
+        n_vert += poly_n[k];
+    }
+    for (int64_t k = 0; k < 2 * n_vert; ++k)
+        if (!std::isfinite(poly_xy[k])) return fail(LBM_ERR_INVALID, "non-finite polygon vertex");
+    for (int k = 0; k < 3 * n_circles; ++k)
+        if (!std::isfinite(circles[k])) return fail(LBM_ERR_INVALID, "non-finite circle");
+    Ensemble* E = h;
+    HostLattice& H = E->lat[l];
+    const int nx = H.d.nx, ny = H.d.ny;
+    const size_t n = static_cast<size_t>(nx) * ny;
+    if (H.obstacle.size() != n) H.obstacle.assign(n, 0);
+    uint8_t* d_mask = nullptr;
+    double* d_xy = nullptr;
+    if (hipMalloc(&d_mask, n) != hipSuccess) return fail(LBM_ERR_NOMEM, "obstacle mask staging");
+    if (n_vert && hipMalloc(&d_xy, static_cast<size_t>(n_vert) * 16) != hipSuccess) {
+        (void)hipFree(d_mask);
+        return fail(LBM_ERR_NOMEM, "polygon staging");
+    }
+    rc = h2d(E, d_mask, H.obstacle.data(), n);
+    if (rc == LBM_OK && n_vert) rc = h2d(E, d_xy, poly_xy, static_cast<size_t>(n_vert) * 16);
+    // each obstacle over its bounding box, widened by a cell against rounding and clipped
+    auto clip = [](double lo, double hi, int size, int32_t& a, int32_t& b) {
+        lo = std::floor(lo) - 1.0;
+        hi = std::floor(hi) + 2.0;
+        a = static_cast<int32_t>(std::max(0.0, std::min(lo, static_cast<double>(size))));
+        b = static_cast<int32_t>(std::max(0.0, std::min(hi, static_cast<double>(size))));
+    };
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < n_circles && rc == LBM_OK && e == hipSuccess; ++k) {
+        PlaceArgs a{};
+        a.mask = d_mask;
+        a.nx = nx;
+        a.kind = 0;
+        a.cx = circles[3 * k];
+        a.cy = circles[3 * k + 1];
+        a.r = circles[3 * k + 2];
+        const double ra = std::fabs(a.r);
+        clip(a.cx - ra, a.cx + ra, nx, a.x0, a.x1);
+        clip(a.cy - ra, a.cy + ra, ny, a.y0, a.y1);
+        e = launch_place(a, E->stream);
+    }
+    int64_t v0 = 0;
+    for (int k = 0; k < n_polys && rc == LBM_OK && e == hipSuccess; ++k) {
+        PlaceArgs a{};
+        a.mask = d_mask;
+        a.nx = nx;
+        a.kind = 1;
+        a.xy = d_xy + 2 * v0;
+        a.n = poly_n[k];
+        double x_lo = poly_xy[2 * v0], x_hi = x_lo, y_lo = poly_xy[2 * v0 + 1], y_hi = y_lo;
+        for (int v = 1; v < poly_n[k]; ++v) {
+            x_lo = std::min(x_lo, poly_xy[2 * (v0 + v)]);
+            x_hi = std::max(x_hi, poly_xy[2 * (v0 + v)]);
+            y_lo = std::min(y_lo, poly_xy[2 * (v0 + v) + 1]);
+            y_hi = std::max(y_hi, poly_xy[2 * (v0 + v) + 1]);
+        }
+        clip(x_lo, x_hi, nx, a.x0, a.x1);
+        clip(y_lo, y_hi, ny, a.y0, a.y1);
+        e = launch_place(a, E->stream);
+        v0 += poly_n[k];
+    }
+    if (rc == LBM_OK && e == hipSuccess) e = hipStreamSynchronize(E->stream);
+    if (rc == LBM_OK && e != hipSuccess) rc = fail(LBM_ERR_DEVICE, std::string("rasterisation: ") + hipGetErrorString(e));
+    if (rc == LBM_OK) rc = d2h(E, H.obstacle.data(), d_mask, n);
+    (void)hipFree(d_mask);
+    if (d_xy) (void)hipFree(d_xy);
+    if (rc != LBM_OK) return rc;
+    // the mask changed: as lbm_set_obstacle
+    H.bb_kind = 0;
+    H.moving = false;
+    H.frad = false;
+    H.explicit_links.clear();
+    H.geom_dirty = true;
+    H.uploaded = false;
+    E->geom_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_get_obstacle(lbm_handle h, int32_t l, uint8_t* mask)
+{
+    int rc = lbm::lattice_ok(h, l);
+    if (rc) return rc;
+    if (!mask) return fail(LBM_ERR_INVALID, "null mask");
+    const auto& H = h->lat[l];
+    const size_t n = static_cast<size_t>(H.d.nx) * H.d.ny;
+    for (size_t c = 0; c < n; ++c) mask[c] = c < H.obstacle.size() ? H.obstacle[c] : 0;
+    return LBM_OK;
+}
+
 int lbm_set_storage_shift(lbm_handle h, int32_t l, double u, double v)
 {
     int rc = lbm::lattice_ok(h, l);

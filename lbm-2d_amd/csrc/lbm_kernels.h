@@ -158,6 +158,20 @@ struct PerturbArgs {
 };
 
 hipError_t launch_perturb(int storage, const PerturbArgs& a, int ny, hipStream_t s);
+
+// One obstacle rasterised into an nx-wide mask over its bounding box [x0,x1) x [y0,y1):
+// a circle (kind 0: cx, cy, r) or a polygon (kind 1: n vertices xy[2n]).
+struct PlaceArgs {
+    uint8_t* mask;
+    int32_t nx;
+    int32_t x0, y0, x1, y1;
+    int32_t kind;
+    double cx, cy, r;
+    const double* xy;
+    int32_t n;
+};
+
+hipError_t launch_place(const PlaceArgs& a, hipStream_t s);
 hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s);
 hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s);
 

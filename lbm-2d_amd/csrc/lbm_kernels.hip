@@ -1335,6 +1335,33 @@ __global__ __launch_bounds__(256) void perturb_kernel(PerturbArgs a)
     }
 }
 
+// ------------------------------------------------------------------ obstacle rasterisation
+
+// place_cylinder (src/lbm.hpp:931-944): solid where sqrt(dx^2 + dy^2) < r; place_polygon
+// (src/lbm.hpp:951-964) with point_in_polygon's even-odd crossing rule
+// (src/geometry.hpp:120-134) at the node's integer coordinates.  FP64, unfused, in the
+// reference's operation order; cells only become solid (the reference never clears).
+__global__ __launch_bounds__(256) void place_kernel(PlaceArgs a)
+{
+#pragma clang fp contract(off)
+    const int x = a.x0 + static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int y = a.y0 + static_cast<int>(blockIdx.y);
+    if (x >= a.x1 || y >= a.y1) return;
+    const double px = static_cast<double>(x), py = static_cast<double>(y);
+    bool in = false;
+    if (a.kind == 0) {
+        const double dx = px - a.cx, dy = py - a.cy;
+        in = __builtin_sqrt(dx * dx + dy * dy) < a.r;
+    } else {
+        for (int i = 0, j = a.n - 1; i < a.n; j = i++) {
+            const double xi = a.xy[2 * i], yi = a.xy[2 * i + 1];
+            const double xj = a.xy[2 * j], yj = a.xy[2 * j + 1];
+            if (((yi > py) != (yj > py)) && (px < (xj - xi) * (py - yi) / (yj - yi) + xi)) in = !in;
+        }
+    }
+    if (in) a.mask[static_cast<int64_t>(y) * a.nx + x] = 1;
+}
+
 // ------------------------------------------------------------------ thermal populations
 
 // Thermal kernel tiles: kThermalTileW nodes wide, 256 / kThermalTileW rows, one node per thread.
@@ -1855,6 +1882,13 @@ hipError_t launch_wall_distance(const uint8_t* flags, uint16_t* wdist, int nx, i
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(wall_cols_kernel, dim3((nx + 255) / 256), dim3(256), 0, s, wdist, nx, ny, pitch);
+    return hipGetLastError();
+}
+
+hipError_t launch_place(const PlaceArgs& a, hipStream_t s)
+{
+    if (a.x1 <= a.x0 || a.y1 <= a.y0) return hipSuccess;
+    hipLaunchKernelGGL(place_kernel, dim3((a.x1 - a.x0 + 255) / 256, a.y1 - a.y0), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

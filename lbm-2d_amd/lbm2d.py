@@ -72,6 +72,7 @@ class LbmError(RuntimeError):
 _lib = None
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
 _h = C.c_void_p
 
 
@@ -123,6 +124,8 @@ def lib() -> C.CDLL:
         "lbm_sync": (C.c_int, [_h]),
         "lbm_wall_distance": (C.c_int, [_h, i32, _dp]),
         "lbm_perturb_velocity": (C.c_int, [_h, i32, i32, i32, _dp]),
+        "lbm_place_obstacles": (C.c_int, [_h, i32, _dp, i32, _dp, _i32p, i32]),
+        "lbm_get_obstacle": (C.c_int, [_h, i32, _u8p]),
         "lbm_set_thermal": (C.c_int, [_h, i32, dbl, dbl, dbl, dbl]),
         "lbm_set_thermal_rules": (C.c_int, [_h, i32, _u8p, C.POINTER(LbmThermalRule), i32]),
         "lbm_upload_thermal": (C.c_int, [_h, i32, _dp]),
@@ -192,6 +195,21 @@ class Ensemble:
         m = np.ascontiguousarray(mask, dtype=np.uint8)
         assert m.shape == self.shape(l), (m.shape, self.shape(l))
         _check(lib().lbm_set_obstacle(self._h, l, m.ctypes.data_as(_u8p)))
+
+    def place_obstacles(self, l: int, circles=(), polygons=()):
+        """place_cylinder / place_polygon on the device: circles as (cx, cy, r), polygons
+        as (n, 2) vertex arrays; cells only become solid."""
+        c = np.ascontiguousarray(np.asarray(circles, np.float64).reshape(-1, 3))
+        polys = [np.asarray(p, np.float64).reshape(-1, 2) for p in polygons]
+        xy = np.ascontiguousarray(np.concatenate(polys) if polys else np.zeros((0, 2)))
+        counts = np.array([len(p) for p in polys], np.int32)
+        _check(lib().lbm_place_obstacles(self._h, l, c.ctypes.data_as(_dp), len(c), xy.ctypes.data_as(_dp),
+                                         counts.ctypes.data_as(_i32p), len(polys)))
+
+    def obstacle(self, l: int = 0) -> np.ndarray:
+        m = np.zeros(self.shape(l), np.uint8)
+        _check(lib().lbm_get_obstacle(self._h, l, m.ctypes.data_as(_u8p)))
+        return m
 
     def set_bb_circle(self, l, cx, cy, r, scheme=BB_MEI):
         _check(lib().lbm_set_bb_circle(self._h, l, cx, cy, r, scheme))
