@@ -243,61 +243,98 @@ bool neighbour(const lbm_desc& d, int x, int y, int i, int& tx, int& ty)
     return true;
 }
 
-// Flags, link records and wall distance of one lattice from its description.
-int build_geometry(HostLattice& H)
+// Device scratch freed on every return path of the geometry build.
+struct Scratch {
+    void* p = nullptr;
+    ~Scratch()
+    {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T> int alloc(T*& out, size_t count)
+    {
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return fail(LBM_ERR_NOMEM, "geometry build scratch");
+        }
+        out = static_cast<T*>(p);
+        return LBM_OK;
+    }
+};
+
+// Flags, link records and wall distance of one lattice from its description.  The
+// per-node pass (flags, link masks, the records' order) runs on the device; the
+// per-link values (q, schemes, slots, wall velocities) are set here on the records.
+int build_geometry(Ensemble* E, HostLattice& H)
 {
     const lbm_desc& d = H.d;
     const int nx = d.nx, ny = d.ny;
-    if (H.obstacle.empty()) H.obstacle.assign(static_cast<size_t>(nx) * ny, 0);
-    H.flags.assign(static_cast<size_t>(ny) * H.pitch, 0);
-    H.lslot.assign(static_cast<size_t>(ny) * H.pitch, -1);
-    H.recs.clear();
-    H.fluid = 0;
-    std::vector<int32_t> rec_of(static_cast<size_t>(nx) * ny, -1);
-    for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-            const int64_t c = static_cast<int64_t>(y) * nx + x;
-            const int64_t p = static_cast<int64_t>(y) * H.pitch + x;
-            uint8_t fl = 0;
-            const bool edge = x == 0 || x == nx - 1 || y == 0 || y == ny - 1;
-            if (H.obstacle[c]) {
-                fl = CELL_SOLID | CELL_SPECIAL;
-            } else {
-                ++H.fluid;
-                uint32_t mask = 0;
-                for (int i = 1; i < 9; ++i) {
-                    int tx, ty;
-                    if (neighbour(d, x, y, i, tx, ty) && H.obstacle[static_cast<int64_t>(ty) * nx + tx])
-                        mask |= 1u << i;
-                }
-                if (edge) fl |= CELL_SPECIAL;
-                if (mask) {
-                    fl |= CELL_SPECIAL | CELL_LINKS;
-                    LinkRec r{};
-                    r.mask = mask;
-                    r.scheme = 0;
-                    for (int i = 0; i < 9; ++i) {
-                        r.slot[i] = 0;
-                        r.q[i] = 0.5;
-                        r.uwx[i] = r.uwy[i] = 0.0;
-                    }
-                    r.cell = static_cast<int32_t>(p);
-                    rec_of[c] = static_cast<int32_t>(H.recs.size());
-                    H.lslot[p] = rec_of[c];
-                    H.recs.push_back(r);
-                }
-            }
-            H.flags[p] = fl;
+    const size_t ncell = static_cast<size_t>(nx) * ny, npitch = static_cast<size_t>(ny) * H.pitch;
+    if (H.obstacle.size() != ncell) H.obstacle.assign(ncell, 0);
+    Scratch s_obs, s_flags, s_lmask, s_rows, s_off, s_fluid, s_lslot, s_cell, s_mask;
+    GeomArgs g{};
+    int rc;
+    uint8_t* obs = nullptr;
+    int32_t* rows = nullptr;
+    if ((rc = s_obs.alloc(obs, ncell)) || (rc = s_flags.alloc(g.flags, npitch)) || (rc = s_lmask.alloc(g.lmask, ncell)) ||
+        (rc = s_rows.alloc(rows, ny)) || (rc = s_fluid.alloc(g.fluid, 1)) || (rc = s_lslot.alloc(g.lslot, npitch)))
+        return rc;
+    g.obs = obs;
+    g.row_links = rows;
+    g.nx = nx;
+    g.ny = ny;
+    g.pitch = H.pitch;
+    g.x_edge = d.x_edge;
+    g.y_edge = d.y_edge;
+    if ((rc = h2d(E, obs, H.obstacle.data(), ncell)) != LBM_OK) return rc;
+    LBM_HIP(hipMemsetAsync(g.flags, 0, npitch, E->stream));
+    LBM_HIP(hipMemsetAsync(g.lslot, 0xff, npitch * 4, E->stream));
+    LBM_HIP(hipMemsetAsync(rows, 0, static_cast<size_t>(ny) * 4, E->stream));
+    LBM_HIP(hipMemsetAsync(g.fluid, 0, 8, E->stream));
+    LBM_HIP(launch_geom_cells(g, E->stream));
+    std::vector<int32_t> row_off(static_cast<size_t>(ny) + 1, 0);
+    unsigned long long fluid = 0;
+    if ((rc = d2h(E, row_off.data() + 1, rows, static_cast<size_t>(ny) * 4)) != LBM_OK) return rc;
+    if ((rc = d2h(E, &fluid, g.fluid, 8)) != LBM_OK) return rc;
+    for (int y = 0; y < ny; ++y) row_off[y + 1] += row_off[y];
+    const int32_t n_links = row_off[ny];
+    std::vector<int32_t> rec_cell(n_links);
+    std::vector<uint8_t> rec_mask(n_links);
+    if (n_links > 0) {
+        int32_t* off = nullptr;
+        if ((rc = s_off.alloc(off, ny)) || (rc = s_cell.alloc(g.rec_cell, n_links)) ||
+            (rc = s_mask.alloc(g.rec_mask, n_links)))
+            return rc;
+        g.row_off = off;
+        if ((rc = h2d(E, off, row_off.data(), static_cast<size_t>(ny) * 4)) != LBM_OK) return rc;
+        LBM_HIP(launch_geom_links(g, E->stream));
+        if ((rc = d2h(E, rec_cell.data(), g.rec_cell, static_cast<size_t>(n_links) * 4)) != LBM_OK) return rc;
+        if ((rc = d2h(E, rec_mask.data(), g.rec_mask, static_cast<size_t>(n_links))) != LBM_OK) return rc;
+    }
+    H.flags.resize(npitch);
+    H.lslot.resize(npitch);
+    if ((rc = d2h(E, H.flags.data(), g.flags, npitch)) != LBM_OK) return rc;
+    if ((rc = d2h(E, H.lslot.data(), g.lslot, npitch * 4)) != LBM_OK) return rc;
+    H.fluid = static_cast<int64_t>(fluid);
+    // records in row-major order of their nodes: simple bounce-back counted in slot 0
+    // until the geometry below says otherwise
+    H.recs.assign(static_cast<size_t>(n_links), LinkRec{});
+    for (int32_t r = 0; r < n_links; ++r) {
+        LinkRec& k = H.recs[r];
+        k.mask = static_cast<uint32_t>(rec_mask[r]) << 1;
+        k.scheme = 0;
+        for (int i = 0; i < 9; ++i) {
+            k.slot[i] = 0;
+            k.q[i] = 0.5;
+            k.uwx[i] = k.uwy[i] = 0.0;
         }
+        k.cell = rec_cell[r];
+    }
     auto each_link = [&](auto&& fn) {
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) {
-                const int32_t ri = rec_of[static_cast<int64_t>(y) * nx + x];
-                if (ri < 0) continue;
-                LinkRec& r = H.recs[ri];
-                for (int i = 1; i < 9; ++i)
-                    if (r.mask & (1u << i)) fn(x, y, i, r);
-            }
+        for (LinkRec& r : H.recs) {
+            const int y = r.cell / H.pitch, x = r.cell - y * H.pitch;
+            for (int i = 1; i < 9; ++i)
+                if (r.mask & (1u << i)) fn(x, y, i, r);
+        }
     };
     if (H.bb_kind != 0) {
         each_link([&](int x, int y, int i, LinkRec& r) {
@@ -325,7 +362,7 @@ int build_geometry(HostLattice& H)
     for (const lbm_link& k : H.explicit_links) {
         if (k.cell < 0 || k.cell >= static_cast<int64_t>(nx) * ny || k.dir < 1 || k.dir > 8)
             return fail(LBM_ERR_INVALID, "link cell/dir out of range");
-        const int32_t ri = rec_of[k.cell];
+        const int32_t ri = H.lslot[static_cast<int64_t>(k.cell / nx) * H.pitch + k.cell % nx];
         if (ri < 0 || !(H.recs[ri].mask & (1u << k.dir)))
             return fail(LBM_ERR_INVALID, "link " + std::to_string(k.cell) + "/" + std::to_string(k.dir) +
                                              " is not a fluid-to-solid link of the obstacle mask");
@@ -438,7 +475,7 @@ int commit(Ensemble* E)
         bool any = false;
         for (auto& H : E->lat)
             if (H.geom_dirty) {
-                if ((rc = build_geometry(H)) != LBM_OK) return rc;
+                if ((rc = build_geometry(E, H)) != LBM_OK) return rc;
                 any = true;
             }
         if (any) {

@@ -172,6 +172,28 @@ struct PlaceArgs {
 };
 
 hipError_t launch_place(const PlaceArgs& a, hipStream_t s);
+
+// Cell flags and boundary-link records of one lattice from its obstacle mask.  Pass 1
+// (launch_geom_cells): flags (pitch layout, zeroed beforehand), each cell's link mask
+// bits 1..8 shifted down by one, link cells per row (zeroed) and the fluid count (zeroed).
+// Pass 2 (launch_geom_links, after the host's scan of the row counts into row_off): the
+// link cells in row-major order -> rec_cell (y*pitch+x), rec_mask (bits 1..8 shifted down
+// by one, as in lmask) and lslot (pitch layout, -1 beforehand).
+struct GeomArgs {
+    const uint8_t* obs;
+    uint8_t* flags;
+    uint8_t* lmask;
+    int32_t* row_links;
+    unsigned long long* fluid;
+    const int32_t* row_off;
+    int32_t* lslot;
+    int32_t* rec_cell;
+    uint8_t* rec_mask;
+    int32_t nx, ny, pitch, x_edge, y_edge;
+};
+
+hipError_t launch_geom_cells(const GeomArgs& g, hipStream_t s);
+hipError_t launch_geom_links(const GeomArgs& g, hipStream_t s);
 hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s);
 hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s);
 

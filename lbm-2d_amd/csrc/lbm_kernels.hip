@@ -1362,6 +1362,89 @@ __global__ __launch_bounds__(256) void place_kernel(PlaceArgs a)
     if (in) a.mask[static_cast<int64_t>(y) * a.nx + x] = 1;
 }
 
+// ------------------------------------------------------------------ cell flags and link records
+
+// Flags and link mask of node (x, y), as the lattice set-up loop of the step's streaming
+// rules needs them: a solid node is SOLID | SPECIAL; a fluid node is SPECIAL on the
+// lattice's edge, and LINKS | SPECIAL when a neighbour along e_i (i = 1..8, wrapped on
+// periodic edges, absent off the others) is solid (src/lbm.hpp:457-484, 542-581).
+__global__ __launch_bounds__(256) void geom_cells_kernel(GeomArgs g)
+{
+    const int x = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int y = static_cast<int>(blockIdx.y);
+    bool fluid = false, link = false;
+    if (x < g.nx) {
+        const int64_t c = static_cast<int64_t>(y) * g.nx + x;
+        uint8_t fl;
+        uint32_t mask = 0;
+        if (g.obs[c]) {
+            fl = CELL_SOLID | CELL_SPECIAL;
+        } else {
+            fluid = true;
+#pragma unroll
+            for (int i = 1; i < 9; ++i) {
+                int tx = x + CX[i], ty = y + CY[i];
+                bool ok = true;
+                if (tx < 0 || tx >= g.nx) {
+                    ok = g.x_edge == EDGE_PERIODIC;
+                    tx = tx < 0 ? tx + g.nx : tx - g.nx;
+                }
+                if (ty < 0 || ty >= g.ny) {
+                    ok = ok && g.y_edge == EDGE_PERIODIC;
+                    ty = ty < 0 ? ty + g.ny : ty - g.ny;
+                }
+                if (ok && g.obs[static_cast<int64_t>(ty) * g.nx + tx]) mask |= 1u << i;
+            }
+            const bool edge = x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1;
+            fl = edge ? CELL_SPECIAL : 0;
+            if (mask) {
+                fl |= CELL_SPECIAL | CELL_LINKS;
+                link = true;
+            }
+        }
+        g.flags[static_cast<int64_t>(y) * g.pitch + x] = fl;
+        g.lmask[c] = static_cast<uint8_t>(mask >> 1);
+    }
+    const uint64_t fb = __ballot(fluid), lb = __ballot(link);
+    if ((threadIdx.x & 63) == 0) {
+        if (fb) atomicAdd(g.fluid, static_cast<unsigned long long>(__popcll(fb)));
+        if (lb) atomicAdd(g.row_links + y, __popcll(lb));
+    }
+}
+
+// One block per row: the row's link cells in x order, numbered from row_off[y] with a
+// ballot scan per wave and the four waves' totals in LDS.
+__global__ __launch_bounds__(256) void geom_links_kernel(GeomArgs g)
+{
+    __shared__ int wave_links[4];
+    const int y = static_cast<int>(blockIdx.x);
+    const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
+    int base = g.row_off[y];
+    for (int x0 = 0; x0 < g.nx; x0 += 256) {
+        const int x = x0 + t;
+        const uint8_t m = x < g.nx ? g.lmask[static_cast<int64_t>(y) * g.nx + x] : uint8_t(0);
+        const uint64_t b = __ballot(m != 0);
+        const int below = __popcll(b & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_links[w] = __popcll(b);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            off += k < w ? wave_links[k] : 0;
+            total += wave_links[k];
+        }
+        if (m) {
+            const int r = base + off + below;
+            const int64_t p = static_cast<int64_t>(y) * g.pitch + x;
+            g.rec_cell[r] = static_cast<int32_t>(p);
+            g.rec_mask[r] = m;
+            g.lslot[p] = r;
+        }
+        base += total;
+        __syncthreads();  // wave_links is rewritten by the next chunk
+    }
+}
+
 // ------------------------------------------------------------------ thermal populations
 
 // Thermal kernel tiles: kThermalTileW nodes wide, 256 / kThermalTileW rows, one node per thread.
@@ -1882,6 +1965,18 @@ hipError_t launch_wall_distance(const uint8_t* flags, uint16_t* wdist, int nx, i
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(wall_cols_kernel, dim3((nx + 255) / 256), dim3(256), 0, s, wdist, nx, ny, pitch);
+    return hipGetLastError();
+}
+
+hipError_t launch_geom_cells(const GeomArgs& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(geom_cells_kernel, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_geom_links(const GeomArgs& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(geom_links_kernel, dim3(g.ny), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
