@@ -11,7 +11,8 @@
 //   thermal      (one-lattice handles) two buffers of g planes with the populations'
 //                layout and parity, one condition byte per node
 // and builds them from the reference's obstacle/geometry description
-// (src/lbm_types.hpp:90-182, src/lbm.hpp:79-87, 931-964).
+// (src/lbm_types.hpp:90-182, src/lbm.hpp:79-87, 931-964): the per-node pass on the
+// device, the per-link values here.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -20,7 +20,9 @@
 // the reference's branches do.  step2_kernel advances one lattice two steps per launch
 // through LDS.  thermal_kernel steps a lattice that carries the thermal populations g
 // as well (src/lbm.hpp:356-397, 489-528): the same pull form for f, the adiabatic
-// streaming of g, the thermal collision and the Boussinesq forcing of f.
+// streaming of g, the thermal collision and the Boussinesq forcing of f.  Set-up
+// kernels: obstacle rasterisation, the per-node geometry pass (flags, link records),
+// wall distance, initial state and perturbation.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
