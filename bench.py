@@ -62,9 +62,9 @@ def algorithmic_bytes(info, les_bytes):
     return info.fluid_cells * 18 * s + info.cells * 1 + info.les_cells * les_bytes
 
 
-def time_steps(ens, steps, warmup, torch):
-    # the first call of 100 or more steps captures the 100-step hipGraph: keep it untimed
-    ens.step(max(warmup, 100))
+def time_steps(ens, steps, warmup, torch, graph_steps=100):
+    # the first call of graph_steps or more steps captures the hipGraph: keep it untimed
+    ens.step(max(warmup, graph_steps))
     ens.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -456,6 +456,8 @@ def main():
                                                    "(default: autotune)")
     ap.add_argument("--large", action="store_true", help="also time the 32768x16384 domain")
     ap.add_argument("--no-forms", action="store_true", help="headline workload only")
+    ap.add_argument("--graph-steps", type=int, default=500,
+                    help="launches per hipGraph of the headline run (forces still recorded every step)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="", help="run one extra form only and print it (profiling): "
                                                 "f32_6400x2400_les, f16s_6400x2400_les, f16s_sr_6400x2400_les, "
@@ -497,8 +499,9 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(the lattice update is measured on a single MI355X).", file=sys.stderr)
         sys.exit(2)
-    if args.steps % 100 != 0:
-        args.steps = max(100, args.steps // 100 * 100)
+    gs = args.graph_steps
+    if args.steps % gs != 0:
+        args.steps = max(gs, args.steps // gs * gs)
 
     import torch
     import lbm2d as G
@@ -511,12 +514,12 @@ def main():
             ens.set_launch(*shape)
         else:
             shape, _ = ens.autotune()
-        ens.set_launch(graph_steps=100)
+        ens.set_launch(graph_steps=gs)
         info = ens.info()
         print(f"# headline lattice ready, launch {shape}", file=sys.stderr, flush=True)
-        t = time_steps(ens, args.steps, args.warmup, torch)
+        t = time_steps(ens, args.steps, args.warmup, torch, gs)
         glups = info.cells * args.steps / t / 1e9
-        kms = graph_step_ms(ens, torch)
+        kms = graph_step_ms(ens, torch, replays=max(2, 2000 // gs), graph_steps=gs)
         bytes_launch = algorithmic_bytes(info, 0)
         fx, fy = ens.forces(0)
         cd_last = 2.0 * fx[-1, 0] / (60 * 0.01)
@@ -565,7 +568,7 @@ def main():
             "workload": "cylinder wake 800x300, D2Q9 MRT, FP32 storage (f-w) and compute, "
                         "r=30 at (200,150), Re=100 tau=0.68 u=0.1, Mei interpolated bounce-back, "
                         "Zou/He inlet, outflow copy, periodic y, seed-42 perturbation, "
-                        "hipGraph of 100 steps",
+                        f"hipGraph of {gs} steps",
             "cells": info.cells, "fluid_cells": info.fluid_cells, "launch": list(shape),
             "cd_last_step": cd_last,
         },
