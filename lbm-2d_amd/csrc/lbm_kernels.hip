@@ -896,11 +896,15 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
         src = static_cast<const S*>(a.src);
         dst = static_cast<S*>(a.dst);
         // every argument the first memory round needs, fetched from the kernel argument
-        // segment in one group instead of between branches
+        // segment in one group instead of between branches.  L.lattice and nan_flags keep
+        // their registers live: a register of a group load the kernel does not read is
+        // otherwise reused for a later argument, which must then wait for the whole group
+        // (scalar loads return out of order) and takes a second round.
         __asm__ volatile("" ::"s"(src), "s"(dst), "s"(a.flags), "s"(a.wdist), "s"(a.acc), "s"(a.total_slots),
                          "s"(a.tiles_x), "s"(a.options), "s"(nb), "s"(L.nx), "s"(L.ny), "s"(L.pitch),
                          "s"(L.f_off), "s"(L.plane), "s"(L.cell_off), "s"(L.les), "s"(a.special),
-                         "s"(a.n_special), "s"(a.special_blocks), "s"(a.n_links), "s"(a.n_edges));
+                         "s"(a.n_special), "s"(a.special_blocks), "s"(a.n_links), "s"(a.n_edges), "s"(L.lattice),
+                         "s"(a.nan_flags));
     }
     Rounding rd{a.sr != 0, 0u};
     if constexpr (SINGLE) {
