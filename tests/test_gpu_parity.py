@@ -320,8 +320,21 @@ def test_device_rasterisation_matches_place_functions():
         e.step(9)
     np.testing.assert_array_equal(ens.download(0), ref.download(0))
     np.testing.assert_array_equal(ens.forces(0)[0], ref.forces(0)[0])
+    # rejected on the host, before any device work: a vertex-less polygon, non-finite
+    # values, an unknown lattice; the mask is unchanged by the failed calls
+    before = ens.obstacle(0)
     with pytest.raises(G.LbmError):
         ens.place_obstacles(0, [], [np.zeros((0, 2))])
+    with pytest.raises(G.LbmError):
+        ens.place_obstacles(0, [(10.0, 10.0, np.nan)])
+    with pytest.raises(G.LbmError):
+        ens.place_obstacles(0, [], [np.array([(0.0, 0.0), (np.inf, 5.0), (3.0, 9.0)])])
+    with pytest.raises(G.LbmError):
+        ens.place_obstacles(1, [(10.0, 10.0, 3.0)])
+    np.testing.assert_array_equal(ens.obstacle(0), before)
+    # an empty call changes nothing but still resets the link set-up, as lbm_set_obstacle
+    ens.place_obstacles(0)
+    np.testing.assert_array_equal(ens.obstacle(0), before)
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "bf16s"])
