@@ -1112,7 +1112,9 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
     for (int l = 0; l < n; ++l) {
         const lbm_desc& d = descs[l];
         if (d.nx < 3 || d.ny < 3) return fail(LBM_ERR_INVALID, "lattice must be at least 3x3");
-        if (static_cast<int64_t>(d.nx) * d.ny > (1ll << 31)) return fail(LBM_ERR_INVALID, "lattice too large");
+        // node indices y * pitch + x (pitch: nx rounded up to 64) are int32
+        if (static_cast<int64_t>((d.nx + 63) / 64 * 64) * d.ny > (1ll << 31))
+            return fail(LBM_ERR_INVALID, "lattice too large: its rows padded to 64 nodes exceed 2^31 nodes");
         if (d.collision < 0 || d.collision > 1 || d.x_edge < 0 || d.x_edge > 2 || d.y_edge < 0 || d.y_edge > 2 ||
             d.inlet < 0 || d.inlet > 2)
             return fail(LBM_ERR_INVALID, "bad enum in lbm_desc");
