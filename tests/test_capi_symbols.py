@@ -49,7 +49,7 @@ def test_errors_are_reported_not_raised_across_the_abi():
     assert L.lbm_create(bad, 1, 1, 0, None, ctypes.byref(h)) != 0
     assert b"3x3" in L.lbm_last_error()
     # within 2^31 nodes, but the rows padded to 64 nodes (int32 node indices) are not
-    bad[0].nx, bad[0].ny = 32769, 65535
-    assert 32769 * 65535 <= 2 ** 31 < 32832 * 65535
+    bad[0].nx, bad[0].ny = 65, 33038209
+    assert 65 * 33038209 <= 2 ** 31 < 128 * 33038209
     assert L.lbm_create(bad, 1, 1, 0, None, ctypes.byref(h)) != 0
     assert b"too large" in L.lbm_last_error()
