@@ -1453,8 +1453,9 @@ __global__ __launch_bounds__(256) void geom_links_kernel(GeomArgs g)
 
 // ------------------------------------------------------------------ thermal populations
 
-// Thermal kernel tiles: kThermalTileW nodes wide, 256 / kThermalTileW rows, one node per thread.
-constexpr int kThermalTileW = 256;
+// Thermal kernel tiles: TW nodes wide (64 or 256), 256 / TW rows, one node per thread.
+// The launch takes the width that leaves fewer idle lanes at the end of a row (256 on a
+// tie): 800-wide rows idle 4% of the lanes in 64-wide tiles against 22% in 256-wide ones.
 
 // Post-stream thermal populations (as deviations g - w) of node (x,y), pulled from the
 // post-collision planes gs (src/lbm.hpp:489-528): y always periodic, x periodic on a
@@ -1557,14 +1558,14 @@ __device__ __forceinline__ void thermal_collide(const ThermalArgs& t, R d[9], R 
     }
 }
 
-// One lattice with thermal populations, one node per thread over kThermalTileW-wide tiles of rows
+// One lattice with thermal populations, one node per thread over TW-wide tiles of rows
 // [t.y0, t.y1): the state of f and g from the plain pulls (interior nodes that are not
 // special) or from the general path and thermal_gather, then the node conditions;
 // then, by t.mode, the collisions
 // and stores of a step (forces into the launch's record terms) or of the current state,
 // or the thermal state / temperature as FP64.  With t.from_state the sources hold the
 // state itself (uploaded, no step taken): no gather and no conditions.
-template <int ST>
+template <int ST, int TW>
 __global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
 {
     using S = typename Form<ST>::store;
@@ -1577,9 +1578,9 @@ __global__ __launch_bounds__(256) void thermal_kernel(StepArgs a, ThermalArgs t)
     const int nb = static_cast<int>(gridDim.x);
     const int q = nb >> 3, r = nb & 7, xcd = b & 7;
     b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
-    const int tiles_x = (L.nx + kThermalTileW - 1) / kThermalTileW;
-    const int x = (b % tiles_x) * kThermalTileW + (static_cast<int>(threadIdx.x) % kThermalTileW);
-    const int y = t.y0 + (b / tiles_x) * (256 / kThermalTileW) + (static_cast<int>(threadIdx.x) / kThermalTileW);
+    const int tiles_x = (L.nx + TW - 1) / TW;
+    const int x = (b % tiles_x) * TW + (static_cast<int>(threadIdx.x) % TW);
+    const int y = t.y0 + (b / tiles_x) * (256 / TW) + (static_cast<int>(threadIdx.x) / TW);
     if (x >= L.nx || y >= t.y1) return;
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
     const S* fsrc = static_cast<const S*>(a.src) + L.f_off;
@@ -2006,16 +2007,23 @@ hipError_t launch_perturb(int storage, const PerturbArgs& a, int ny, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s)
+template <int TW> hipError_t launch_thermal_w(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s)
 {
-    const int rows = 256 / kThermalTileW;
-    dim3 grid(((a.L0.nx + kThermalTileW - 1) / kThermalTileW) * ((t.y1 - t.y0 + rows - 1) / rows));
+    constexpr int rows = 256 / TW;
+    dim3 grid(((a.L0.nx + TW - 1) / TW) * ((t.y1 - t.y0 + rows - 1) / rows));
     switch (storage) {
-    case ST_F64: hipLaunchKernelGGL(thermal_kernel<ST_F64>, grid, dim3(256), 0, s, a, t); break;
-    case ST_F32: hipLaunchKernelGGL(thermal_kernel<ST_F32>, grid, dim3(256), 0, s, a, t); break;
+    case ST_F64: hipLaunchKernelGGL((thermal_kernel<ST_F64, TW>), grid, dim3(256), 0, s, a, t); break;
+    case ST_F32: hipLaunchKernelGGL((thermal_kernel<ST_F32, TW>), grid, dim3(256), 0, s, a, t); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s)
+{
+    const int nx = a.L0.nx;
+    const int idle64 = (nx + 63) / 64 * 64 - nx, idle256 = (nx + 255) / 256 * 256 - nx;
+    return idle64 < idle256 ? launch_thermal_w<64>(storage, a, t, s) : launch_thermal_w<256>(storage, a, t, s);
 }
 
 hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s)
