@@ -337,6 +337,65 @@ def test_device_rasterisation_matches_place_functions():
     np.testing.assert_array_equal(ens.obstacle(0), before)
 
 
+@pytest.mark.parametrize("case", ["cylinder", "hills", "cavity"])
+def test_geometry_pass_on_a_porous_mask(case):
+    """The device geometry pass on a random porous mask (25% solid nodes, runs across the
+    periodic seam and along the edges, rows padded to 192): fluid and link-node counts as
+    numpy counts them under the case's edge rules, and ten FP64 steps with simple
+    bounce-back on every link against the oracle."""
+    import oracle as O
+    kind = {"cylinder": (G.CYLINDER, O.CYLINDER, 0.05), "hills": (G.PERIODIC_HILLS, O.PERIODIC_HILLS, 0.0),
+            "cavity": (G.CAVITY, O.CAVITY, 0.05)}[case]
+    nx, ny, tau = 131, 77, 0.8
+    rng = np.random.default_rng(5)
+    mask = (rng.random((ny, nx)) < 0.25).astype(np.uint8)
+    mask[10:14, :3] = 1
+    mask[10:14, -3:] = 1
+    mask[0, 40:50] = 1
+    mask[-1, 40:50] = 1
+    # the outflow copy (src/lbm.hpp:63-73) of a solid node would give a fluid outlet node
+    # zero density (u = 0/0, in the reference too): the outlet column follows its source
+    mask[:, -1] = mask[:, -2]
+    desc = G.desc_for_case(kind[0], nx, ny, tau=tau, u_in=kind[2])
+    ens = G.Ensemble([desc], storage="f64")
+    ens.set_obstacle(0, mask)
+    o = O.Oracle(nx, ny, kind[1])
+    o.set_obstacle(mask)
+    o.init_equilibrium(1.0, kind[2], 0.0)
+    ens.upload(0, o.f.copy())
+    # fluid nodes with a solid neighbour along some e_i, neighbours wrapped on periodic edges
+    solid = mask.astype(bool)
+    near = np.zeros_like(solid)
+    for i in range(1, 9):
+        cx, cy = int(G.CXV[i]), int(G.CYV[i])
+        sh = np.roll(solid, (-cy, -cx), axis=(0, 1))
+        if desc.x_edge != G.EDGE_PERIODIC:
+            if cx == 1:
+                sh[:, -1] = False
+            elif cx == -1:
+                sh[:, 0] = False
+        if desc.y_edge != G.EDGE_PERIODIC:
+            if cy == 1:
+                sh[-1, :] = False
+            elif cy == -1:
+                sh[0, :] = False
+        near |= sh
+    info = ens.info()
+    assert info.fluid_cells == int((~solid).sum())
+    assert info.link_cells == int((near & ~solid).sum())
+    fx_o = []
+    for _ in range(10):
+        o.step(tau, kind[2])
+        fx_o.append(o.total_force()[0])
+    ens.step(10)
+    err = np.abs(ens.download(0) - o.f).max()
+    print(f"[{case}] link nodes {info.link_cells}, max|df| = {err:.3e}")
+    assert err <= TOL["f64"]
+    if kind[0] != G.CAVITY:
+        fx = ens.forces(0)[0][:, 0]
+        np.testing.assert_allclose(fx, fx_o, rtol=0, atol=FTOL["f64"] * (np.abs(fx_o).max() + 1e-3))
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32", "bf16s"])
 def test_macros_equal_compute_macros_of_download(storage):
     """lbm_macros = compute_macros (src/lbm_types.hpp:278-288) of the downloaded state, bit
