@@ -1697,9 +1697,39 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     if (E->thermal) return fail(LBM_ERR_INVALID, "a thermal lattice has one launch shape: nothing to tune");
     if ((rc = require_uploaded(E)) != LBM_OK) return rc;
     if ((rc = commit(E)) != LBM_OK) return rc;
-    void* scratch = nullptr;
-    if (hipMalloc(&scratch, static_cast<size_t>(E->pool_elems) * E->bpp) != hipSuccess)
-        return fail(LBM_ERR_NOMEM, "autotune scratch buffer");
+    // Candidates are timed as stepping runs them: each launch reads what the previous one
+    // wrote, ping-ponging between two copies of the population buffers (a launch that
+    // always reads the same source would hide what streaming stores cost the next step's
+    // reads).  The state, the accumulators and the history stay untouched.
+    const size_t pool_bytes = static_cast<size_t>(E->pool_elems) * E->bpp;
+    void* scratch[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; ++k)
+        if (hipMalloc(&scratch[k], pool_bytes) != hipSuccess) {
+            if (scratch[0]) (void)hipFree(scratch[0]);
+            return fail(LBM_ERR_NOMEM, "autotune scratch buffers");
+        }
+    for (int k = 0; k < 2 && rc == LBM_OK; ++k)
+        if (hipMemcpyAsync(scratch[k], E->buf[k], pool_bytes, hipMemcpyDeviceToDevice, E->stream) != hipSuccess)
+            rc = fail(LBM_ERR_DEVICE, "autotune scratch copy");
+    if (rc != LBM_OK) {
+        (void)hipFree(scratch[0]);
+        (void)hipFree(scratch[1]);
+        return rc;
+    }
+    // launch n of a candidate: one lattice reads scratch[(par + n) & 1] and writes the
+    // other; an ensemble takes both from the per-lattice parity flipped by step_arg
+    const int par0 = E->lat[0].par;
+    auto ping = [&](StepArgs& a, bool single, int n) {
+        if (single) {
+            a.src = scratch[(par0 + n) & 1];
+            a.dst = scratch[(par0 + n + 1) & 1];
+        } else {
+            a.buf0 = scratch[0];
+            a.buf1 = scratch[1];
+            a.dst = nullptr;
+            a.step_arg = n;
+        }
+    };
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -1719,23 +1749,26 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         StepArgs a = base_args(E);
         int blocks;
         const bool single = grid_for(E, -1, a, blocks);
-        // a scratch destination and dummy accumulators keep state and history intact
-        a.src = E->buf[E->lat[0].par];
-        a.dst = scratch;
+        // dummy accumulators keep the force history intact
         a.acc = E->d_dummy_acc;
         a.nf = E->d_dummy_nf;
         a.nan_flags = E->d_dummy_nan;
+        int n = 0;
         hipError_t le = hipSuccess;
-        for (int w = 0; w < 2 && le == hipSuccess; ++w)
+        for (int w = 0; w < 2 && le == hipSuccess; ++w, ++n) {
+            ping(a, single, n);
             le = launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream);
+        }
         // ten launches for an estimate, then enough for ~1.5 ms of device time, so that
         // shapes a few percent apart on small lattices are told apart
         float ms = 0.f;
         bool ok = le == hipSuccess;
         for (int pass = 0, reps = 10; pass < 2 && ok; ++pass) {
             (void)hipEventRecord(e0, E->stream);
-            for (int r = 0; r < reps && le == hipSuccess; ++r)
+            for (int r = 0; r < reps && le == hipSuccess; ++r, ++n) {
+                ping(a, single, n);
                 le = launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream);
+            }
             (void)hipEventRecord(e1, E->stream);
             ok = le == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
             if (!ok) break;
@@ -1771,18 +1804,23 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         a.tiles_x = (d.nx + 63) / 64;
         a.tiles_y = (d.ny + rows - 1) / rows;
         blocks = a.tiles_x * a.tiles_y;
-        a.src = E->buf[E->lat[0].par];
-        a.dst = scratch;
         a.acc = a.acc2 = E->d_dummy_acc;
         a.nf = a.nf2 = E->d_dummy_nf;
         a.nan_flags = E->d_dummy_nan;
+        int n = 0;
         hipError_t le = hipSuccess;
-        for (int w = 0; w < 2 && le == hipSuccess; ++w) le = launch_step2(E->storage, rows, blocks, a, E->stream);
+        for (int w = 0; w < 2 && le == hipSuccess; ++w, ++n) {
+            ping(a, true, n);
+            le = launch_step2(E->storage, rows, blocks, a, E->stream);
+        }
         float ms = 0.f;
         bool ok = le == hipSuccess;
         for (int pass = 0, reps = 5; pass < 2 && ok; ++pass) {
             (void)hipEventRecord(e0, E->stream);
-            for (int r = 0; r < reps && le == hipSuccess; ++r) le = launch_step2(E->storage, rows, blocks, a, E->stream);
+            for (int r = 0; r < reps && le == hipSuccess; ++r, ++n) {
+                ping(a, true, n);
+                le = launch_step2(E->storage, rows, blocks, a, E->stream);
+            }
             (void)hipEventRecord(e1, E->stream);
             ok = le == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
             if (!ok) break;
@@ -1806,7 +1844,8 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(scratch);
+    (void)hipFree(scratch[0]);
+    (void)hipFree(scratch[1]);
     const int* pick = rc == LBM_OK ? bs : saved;
     E->bx = pick[0];
     E->rows = pick[1];
