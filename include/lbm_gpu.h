@@ -199,7 +199,9 @@ int lbm_step_lattice(lbm_handle h, int32_t lattice, int32_t n);
 int lbm_set_launch(lbm_handle h, int32_t block_x, int32_t rows, int32_t nodes_per_thread,
                    int32_t options, int32_t graph_steps);
 /* Times every compiled launch shape with every option set on the current state and
- * keeps the fastest.  The lattice state and force history are unchanged. */
+ * keeps the fastest.  Each candidate steps two device copies of the population buffers
+ * in turn (as stepping does), so the call needs twice the population memory free for
+ * its duration.  The lattice state and force history are unchanged. */
 int lbm_autotune(lbm_handle h, int32_t* best_block_x, int32_t* best_rows, int32_t* best_nodes,
                  int32_t* best_options, float* best_ms);
 
