@@ -666,7 +666,7 @@ bool grid_for(Ensemble* E, int single, StepArgs& a, int& blocks, int bx = 0, int
         a.n_edges = edge_nodes(d.nx, d.ny);
         a.special = E->d_special + H.special_off;
         a.n_special = static_cast<int32_t>(H.special.size());
-        a.special_blocks = list_blocks(bx, a.n_links + a.n_edges) + (a.n_special + bx - 1) / bx;
+        a.special_blocks = list_blocks(bx, a.n_links, a.n_edges) + (a.n_special + bx - 1) / bx;
         blocks = a.special_blocks + a.tiles_x * a.tiles_y;
         return true;
     }

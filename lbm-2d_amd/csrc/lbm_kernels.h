@@ -203,8 +203,9 @@ hipError_t launch_step(int storage, int block_x, int rows, int vec, bool single,
 hipError_t launch_flush(const FlushArgs& f, hipStream_t s);
 // Two steps of one lattice per launch: tiles of 64 x tile_rows (4 or 8), 256 threads.
 hipError_t launch_step2(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s);
-// Blocks of block_x threads that take n general-path list nodes.
-int list_blocks(int block_x, int n);
+// Blocks of block_x threads that take the general-path list nodes: n_links link
+// records (kListLanes per wave), then n_edges edge nodes (every lane).
+int list_blocks(int block_x, int n_links, int n_edges);
 hipError_t launch_state(int storage, const StateArgs& a, int nx, hipStream_t s);
 hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream_t s);
 // steps += inc, buffer parity ^= flips & 1 for lattices [begin, begin+count)

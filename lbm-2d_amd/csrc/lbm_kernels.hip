@@ -917,17 +917,27 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
             const Ctx<ST> c{L, src + L.f_off, a.flags, a.link_slot, a.links, a.inv_scale};
             const ForceSink fs{a.nf, accp};
             bool bad = false;
-            // link and edge nodes: kListLanes per wave, since their loads scatter and a
-            // wave's memory requests issue one address at a time
+            // link nodes: kListLanes per wave, since their loads scatter and a wave's
+            // memory requests issue one address at a time; edge nodes: every lane, since
+            // consecutive edge nodes are consecutive along a row or a column
             constexpr int per_block = BX / 64 * kListLanes;
-            const int gb = (a.n_links + a.n_edges + per_block - 1) / per_block;
-            if (b < gb) {
-                const int lane = static_cast<int>(threadIdx.x) & 63;
-                const int t = b * per_block + (static_cast<int>(threadIdx.x) >> 6) * kListLanes + lane;
-                if (lane < kListLanes && t < a.n_links + a.n_edges) list_node<ST>(L, a, c, dst, fs, t, rd, bad);
+            const int gl = (a.n_links + per_block - 1) / per_block;
+            const int ge = gl + (a.n_edges + BX - 1) / BX;
+            if (b < ge) {
+                // one call site: one copy of the general path in the kernel
+                int t = -1;
+                if (b < gl) {
+                    const int lane = static_cast<int>(threadIdx.x) & 63;
+                    const int k = b * per_block + (static_cast<int>(threadIdx.x) >> 6) * kListLanes + lane;
+                    if (lane < kListLanes && k < a.n_links) t = k;
+                } else {
+                    const int k = a.n_links + (b - gl) * BX + static_cast<int>(threadIdx.x);
+                    if (k < a.n_links + a.n_edges) t = k;
+                }
+                if (t >= 0) list_node<ST>(L, a, c, dst, fs, t, rd, bad);
             } else {
                 // written solids (Bouzidi sources): zero populations, f - w = -w
-                const int t = (b - gb) * BX + static_cast<int>(threadIdx.x);
+                const int t = (b - ge) * BX + static_cast<int>(threadIdx.x);
                 if (t < a.n_special) {
                     S* o = dst + L.f_off + a.special[t];
 #pragma unroll
@@ -1902,10 +1912,10 @@ hipError_t launch_step(int storage, int bx, int rows, int v, bool single, int n_
     return hipErrorInvalidValue;
 }
 
-int list_blocks(int block_x, int n)
+int list_blocks(int block_x, int n_links, int n_edges)
 {
     const int per_block = block_x / 64 * kListLanes;
-    return (n + per_block - 1) / per_block;
+    return (n_links + per_block - 1) / per_block + (n_edges + block_x - 1) / block_x;
 }
 
 template <int ST>
