@@ -396,6 +396,36 @@ def test_geometry_pass_on_a_porous_mask(case):
         np.testing.assert_allclose(fx, fx_o, rtol=0, atol=FTOL["f64"] * (np.abs(fx_o).max() + 1e-3))
 
 
+@pytest.mark.parametrize("shape", [(3, 3), (5, 200), (200, 3), (65, 4), (130, 7)])
+@pytest.mark.parametrize("case", ["cylinder", "hills"])
+def test_tiny_and_thin_lattices_match_oracle(case, shape):
+    """Lattices made mostly of edge nodes (the list blocks' edge lanes, rows padded past
+    nx, columns of two nodes): twelve FP64 steps against the oracle, with a solid node
+    where the lattice has an interior."""
+    import oracle as O
+    nx, ny = shape
+    kind = {"cylinder": (G.CYLINDER, O.CYLINDER, 0.05), "hills": (G.PERIODIC_HILLS, O.PERIODIC_HILLS, 0.0)}[case]
+    mask = np.zeros((ny, nx), np.uint8)
+    if nx >= 5 and ny >= 5:
+        mask[ny // 2, nx // 2] = 1
+    desc = G.desc_for_case(kind[0], nx, ny, tau=0.7, u_in=kind[2])
+    o = O.Oracle(nx, ny, kind[1])
+    o.set_obstacle(mask)
+    o.init_equilibrium(1.0, kind[2], 0.0)
+    rng = np.random.default_rng(nx * 1000 + ny)
+    f0 = o.f * (1.0 + 0.01 * rng.standard_normal(o.f.shape)) * (mask == 0)[..., None]
+    o.f[...] = f0
+    ens = G.Ensemble([desc], storage="f64")
+    ens.set_obstacle(0, mask)
+    ens.upload(0, f0.copy())
+    for _ in range(12):
+        o.step(0.7, kind[2])
+    ens.step(12)
+    err = np.abs(ens.download(0) - o.f).max()
+    print(f"[{case} {nx}x{ny}] max|df| = {err:.3e}")
+    assert err <= TOL["f64"]
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32", "bf16s"])
 def test_macros_equal_compute_macros_of_download(storage):
     """lbm_macros = compute_macros (src/lbm_types.hpp:278-288) of the downloaded state, bit
