@@ -128,7 +128,8 @@ int lbm_set_obstacle(lbm_handle h, int32_t lattice, const uint8_t* mask);
  * is lbm_set_bb_circle / lbm_set_bb_polygon. */
 int lbm_place_obstacles(lbm_handle h, int32_t lattice, const double* circles, int32_t n_circles,
                         const double* poly_xy, const int32_t* poly_n, int32_t n_polys);
-/* The lattice's obstacle mask, nx*ny bytes (1 = solid). */
+/* The lattice's obstacle mask, nx*ny bytes (1 = solid): LBMCapabilities::obstacle
+ * (src/lbm_types.hpp:190), which the drivers read for their output frames. */
 int lbm_get_obstacle(lbm_handle h, int32_t lattice, uint8_t* mask);
 /* The reference's single interpolated-BB geometry (BounceBackGeometry with a circle
  * or polygon, src/lbm_types.hpp:90-176): q of every link from that one shape, 1.0
