@@ -426,6 +426,85 @@ def test_tiny_and_thin_lattices_match_oracle(case, shape):
     assert err <= TOL["f64"]
 
 
+def _q_circle(ccx, ccy, r, xf, yf, i):
+    """q of a link to a circle (q_cylinder, src/lbm_types.hpp:118-138), FP64."""
+    dx, dy = xf - ccx, yf - ccy
+    ex, ey = float(G.CXV[i]), float(G.CYV[i])
+    a = ex * ex + ey * ey
+    b = 2.0 * (dx * ex + dy * ey)
+    c = dx * dx + dy * dy - r * r
+    disc = b * b - 4.0 * a * c
+    if disc < 0.0:
+        return 1.0
+    sd = float(np.sqrt(disc))
+    t1, t2 = (-b - sd) / (2.0 * a), (-b + sd) / (2.0 * a)
+    if 0.0 < t1 <= 1.0:
+        return t1
+    if 0.0 < t2 <= 1.0:
+        return t2
+    return 1.0
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_explicit_links_equal_the_geometry_they_restate(storage):
+    """lbm_set_links with the q of the circle and the rotating wall's velocity at each
+    solid node, link by link, steps bit for bit as lbm_set_bb_circle + lbm_set_moving_wall
+    (which match the oracle); splitting the links over two force slots leaves the state
+    unchanged and the slots sum to the one-slot totals; links the mask does not have, bad
+    schemes and slots beyond the lattice's are rejected."""
+    nx, ny, cx, cy, r, omega, u = 96, 48, 24, 25, 6, 0.01, 0.05
+    mask = G.circle_mask(nx, ny, cx, cy, r)
+    desc = G.desc_for_case(G.CYLINDER, nx, ny, tau=0.6, u_in=u)
+    f0 = G.equilibrium(np.ones((ny, nx)), np.full((ny, nx), u), np.zeros((ny, nx)))
+    G.perturb_seed42(f0, mask, cx + 5, cx + 60)
+    links, split = [], []
+    for y in range(ny):
+        for x in range(nx):
+            if mask[y, x]:
+                continue
+            for i in range(1, 9):
+                tx, ty = x + int(G.CXV[i]), (y + int(G.CYV[i])) % ny
+                if not (0 <= tx < nx) or not mask[ty, tx]:
+                    continue
+                px, py = float(x + G.CXV[i]), float(y + G.CYV[i])
+                k = (y * nx + x, i, G.BB_MEI, 0, _q_circle(cx, cy, r, x, y, i), -omega * (py - cy), omega * (px - cx))
+                links.append(k)
+                split.append(k[:3] + (1 if y < cy else 0,) + k[4:])
+    runs = {}
+    for name in ("geometry", "links", "split"):
+        e = G.Ensemble([desc], storage=storage)
+        e.set_obstacle(0, mask)
+        if name == "geometry":
+            e.set_bb_circle(0, cx, cy, r, G.BB_MEI)
+            e.set_moving_wall(0, omega, cx, cy)
+        elif name == "links":
+            e.set_links(0, links)
+        else:
+            e.set_force_slots(0, 2)
+            e.set_links(0, split)
+        e.upload(0, f0.copy())
+        e.step(30)
+        runs[name] = (e, e.download(0), e.forces(0, n_slots=2 if name == "split" else 1))
+    g, l, sp = runs["geometry"], runs["links"], runs["split"]
+    np.testing.assert_array_equal(l[1], g[1])
+    np.testing.assert_array_equal(l[2][0], g[2][0])
+    np.testing.assert_array_equal(l[2][1], g[2][1])
+    np.testing.assert_array_equal(sp[1], g[1])
+    fx, fy = sp[2]
+    scale = np.abs(g[2][0]).max() + 1e-3
+    np.testing.assert_allclose(fx.sum(axis=1), g[2][0][:, 0], rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(fy.sum(axis=1), g[2][1][:, 0], rtol=0, atol=1e-12 * scale)
+    assert np.abs(fx[:, 0]).max() > 0 and np.abs(fx[:, 1]).max() > 0
+    # the links are checked when the geometry is committed (here by the upload)
+    e = runs["links"][0]
+    fluid_without_links = (cy * nx + 2, 1, G.BB_MEI, 0, 0.5, 0.0, 0.0)
+    for bad, why in ((fluid_without_links, "fluid-to-solid"), (links[0][:2] + (3,) + links[0][3:], "scheme"),
+                     (links[0][:3] + (5,) + links[0][4:], "slot")):
+        e.set_links(0, [bad])
+        with pytest.raises(G.LbmError, match=why):
+            e.upload(0, f0.copy())
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32", "bf16s"])
 def test_macros_equal_compute_macros_of_download(storage):
     """lbm_macros = compute_macros (src/lbm_types.hpp:278-288) of the downloaded state, bit
