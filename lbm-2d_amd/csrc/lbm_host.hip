@@ -113,7 +113,8 @@ struct HostLattice {
     // built geometry
     std::vector<LinkRec> recs;
     std::vector<uint8_t> flags;      // pitch layout
-    std::vector<int32_t> lslot;      // pitch layout
+    int32_t* d_lslot_new = nullptr;  // link slots of the last build (device, pitch layout),
+                                     // moved into the pool by commit
     std::vector<int32_t> special;    // y*pitch+x of the general-path nodes
     int64_t special_off = 0;
     int64_t fluid = 0;
@@ -271,14 +272,24 @@ int build_geometry(Ensemble* E, HostLattice& H)
     const int nx = d.nx, ny = d.ny;
     const size_t ncell = static_cast<size_t>(nx) * ny, npitch = static_cast<size_t>(ny) * H.pitch;
     if (H.obstacle.size() != ncell) H.obstacle.assign(ncell, 0);
-    Scratch s_obs, s_flags, s_lmask, s_rows, s_off, s_fluid, s_lslot, s_cell, s_mask;
+    Scratch s_obs, s_flags, s_lmask, s_rows, s_off, s_fluid, s_cell, s_mask;
     GeomArgs g{};
     int rc;
     uint8_t* obs = nullptr;
     int32_t* rows = nullptr;
     if ((rc = s_obs.alloc(obs, ncell)) || (rc = s_flags.alloc(g.flags, npitch)) || (rc = s_lmask.alloc(g.lmask, ncell)) ||
-        (rc = s_rows.alloc(rows, ny)) || (rc = s_fluid.alloc(g.fluid, 1)) || (rc = s_lslot.alloc(g.lslot, npitch)))
+        (rc = s_rows.alloc(rows, ny)) || (rc = s_fluid.alloc(g.fluid, 1)))
         return rc;
+    // the link slots stay on the device: commit copies them into the pool
+    if (H.d_lslot_new) {
+        (void)hipFree(H.d_lslot_new);
+        H.d_lslot_new = nullptr;
+    }
+    if (hipMalloc(&H.d_lslot_new, std::max<size_t>(npitch, 1) * 4) != hipSuccess) {
+        H.d_lslot_new = nullptr;
+        return fail(LBM_ERR_NOMEM, "geometry build scratch");
+    }
+    g.lslot = H.d_lslot_new;
     g.obs = obs;
     g.row_links = rows;
     g.nx = nx;
@@ -312,9 +323,7 @@ int build_geometry(Ensemble* E, HostLattice& H)
         if ((rc = d2h(E, rec_mask.data(), g.rec_mask, static_cast<size_t>(n_links))) != LBM_OK) return rc;
     }
     H.flags.resize(npitch);
-    H.lslot.resize(npitch);
     if ((rc = d2h(E, H.flags.data(), g.flags, npitch)) != LBM_OK) return rc;
-    if ((rc = d2h(E, H.lslot.data(), g.lslot, npitch * 4)) != LBM_OK) return rc;
     H.fluid = static_cast<int64_t>(fluid);
     // records in row-major order of their nodes: simple bounce-back counted in slot 0
     // until the geometry below says otherwise
@@ -363,7 +372,11 @@ int build_geometry(Ensemble* E, HostLattice& H)
     for (const lbm_link& k : H.explicit_links) {
         if (k.cell < 0 || k.cell >= static_cast<int64_t>(nx) * ny || k.dir < 1 || k.dir > 8)
             return fail(LBM_ERR_INVALID, "link cell/dir out of range");
-        const int32_t ri = H.lslot[static_cast<int64_t>(k.cell / nx) * H.pitch + k.cell % nx];
+        // the node's record: records are in row-major order of their nodes
+        const int32_t node = static_cast<int32_t>(static_cast<int64_t>(k.cell / nx) * H.pitch + k.cell % nx);
+        const auto it = std::lower_bound(H.recs.begin(), H.recs.end(), node,
+                                         [](const LinkRec& r, int32_t v) { return r.cell < v; });
+        const int32_t ri = (it != H.recs.end() && it->cell == node) ? static_cast<int32_t>(it - H.recs.begin()) : -1;
         if (ri < 0 || !(H.recs[ri].mask & (1u << k.dir)))
             return fail(LBM_ERR_INVALID, "link " + std::to_string(k.cell) + "/" + std::to_string(k.dir) +
                                              " is not a fluid-to-solid link of the obstacle mask");
@@ -496,11 +509,9 @@ int commit(Ensemble* E)
             std::vector<LinkRec> all;
             all.reserve(nrec);
             std::vector<uint8_t> flags(E->cell_elems, 0);
-            std::vector<int32_t> lslot(E->cell_elems, -1);
             for (auto& H : E->lat) {
                 all.insert(all.end(), H.recs.begin(), H.recs.end());
                 std::copy(H.flags.begin(), H.flags.end(), flags.begin() + H.cell_off);
-                std::copy(H.lslot.begin(), H.lslot.end(), lslot.begin() + H.cell_off);
             }
             if ((rc = h2d(E, E->d_links, all.data(), all.size() * sizeof(LinkRec))) != LBM_OK) return rc;
             // each record's force terms go to its primary slot: that of its first counted link
@@ -540,7 +551,16 @@ int commit(Ensemble* E)
             if ((rc = h2d(E, E->d_special, special.data(), special.size() * 4)) != LBM_OK) return rc;
             E->graph_dirty = true;
             if ((rc = h2d(E, E->d_flags, flags.data(), flags.size())) != LBM_OK) return rc;
-            if ((rc = h2d(E, E->d_lslot, lslot.data(), lslot.size() * 4)) != LBM_OK) return rc;
+            // link slots of the lattices just built, device to device (the others are in place)
+            for (auto& H : E->lat)
+                if (H.d_lslot_new) {
+                    LBM_HIP(hipMemcpyAsync(E->d_lslot + H.cell_off, H.d_lslot_new,
+                                           static_cast<size_t>(H.d.ny) * H.pitch * 4, hipMemcpyDeviceToDevice,
+                                           E->stream));
+                    LBM_HIP(hipStreamSynchronize(E->stream));
+                    (void)hipFree(H.d_lslot_new);
+                    H.d_lslot_new = nullptr;
+                }
             // wall distance of every lattice from its flags, on the device (read by the
             // LES lattices' collisions and by lbm_wall_distance)
             for (auto& H : E->lat)
@@ -1164,6 +1184,9 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
     }
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_flags, cells);
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_lslot, cells);
+    // no link record until a geometry build writes a lattice's slots
+    if (rc == LBM_OK && hipMemset(E->d_lslot, 0xff, static_cast<size_t>(cells) * 4) != hipSuccess)
+        rc = fail(LBM_ERR_DEVICE, "link slot pool");
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_wdist, cells);
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_lat, n);
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_steps, n);
@@ -1209,6 +1232,8 @@ int lbm_destroy(lbm_handle h)
                     E->d_dummy_nf, E->d_slot_start, E->d_slot_recs, E->gbuf[0], E->gbuf[1], E->d_tcode};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto& H : E->lat)
+        if (H.d_lslot_new) (void)hipFree(H.d_lslot_new);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
     delete h;
     return LBM_OK;
