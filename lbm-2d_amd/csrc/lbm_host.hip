@@ -112,9 +112,8 @@ struct HostLattice {
     bool geom_dirty = true;
     // built geometry
     std::vector<LinkRec> recs;
-    std::vector<uint8_t> flags;      // pitch layout
-    int32_t* d_lslot_new = nullptr;  // link slots of the last build (device, pitch layout),
-                                     // moved into the pool by commit
+    int32_t* d_lslot_new = nullptr;  // flags and link slots of the last build (device, pitch
+    uint8_t* d_flags_new = nullptr;  // layout), moved into the pools by commit
     std::vector<int32_t> special;    // y*pitch+x of the general-path nodes
     int64_t special_off = 0;
     int64_t fluid = 0;
@@ -245,6 +244,15 @@ bool neighbour(const lbm_desc& d, int x, int y, int i, int& tx, int& ty)
     return true;
 }
 
+// Frees a lattice's built-but-not-committed device geometry.
+void drop_new_geometry(HostLattice& H)
+{
+    if (H.d_lslot_new) (void)hipFree(H.d_lslot_new);
+    if (H.d_flags_new) (void)hipFree(H.d_flags_new);
+    H.d_lslot_new = nullptr;
+    H.d_flags_new = nullptr;
+}
+
 // Device scratch freed on every return path of the geometry build.
 struct Scratch {
     void* p = nullptr;
@@ -272,24 +280,23 @@ int build_geometry(Ensemble* E, HostLattice& H)
     const int nx = d.nx, ny = d.ny;
     const size_t ncell = static_cast<size_t>(nx) * ny, npitch = static_cast<size_t>(ny) * H.pitch;
     if (H.obstacle.size() != ncell) H.obstacle.assign(ncell, 0);
-    Scratch s_obs, s_flags, s_lmask, s_rows, s_off, s_fluid, s_cell, s_mask;
+    Scratch s_obs, s_lmask, s_rows, s_off, s_fluid, s_cell, s_mask;
     GeomArgs g{};
     int rc;
     uint8_t* obs = nullptr;
     int32_t* rows = nullptr;
-    if ((rc = s_obs.alloc(obs, ncell)) || (rc = s_flags.alloc(g.flags, npitch)) || (rc = s_lmask.alloc(g.lmask, ncell)) ||
+    if ((rc = s_obs.alloc(obs, ncell)) || (rc = s_lmask.alloc(g.lmask, ncell)) ||
         (rc = s_rows.alloc(rows, ny)) || (rc = s_fluid.alloc(g.fluid, 1)))
         return rc;
-    // the link slots stay on the device: commit copies them into the pool
-    if (H.d_lslot_new) {
-        (void)hipFree(H.d_lslot_new);
-        H.d_lslot_new = nullptr;
-    }
-    if (hipMalloc(&H.d_lslot_new, std::max<size_t>(npitch, 1) * 4) != hipSuccess) {
-        H.d_lslot_new = nullptr;
+    // flags and link slots stay on the device: commit copies them into the pools
+    drop_new_geometry(H);
+    if (hipMalloc(&H.d_lslot_new, std::max<size_t>(npitch, 1) * 4) != hipSuccess ||
+        hipMalloc(&H.d_flags_new, std::max<size_t>(npitch, 1)) != hipSuccess) {
+        drop_new_geometry(H);
         return fail(LBM_ERR_NOMEM, "geometry build scratch");
     }
     g.lslot = H.d_lslot_new;
+    g.flags = H.d_flags_new;
     g.obs = obs;
     g.row_links = rows;
     g.nx = nx;
@@ -322,8 +329,6 @@ int build_geometry(Ensemble* E, HostLattice& H)
         if ((rc = d2h(E, rec_cell.data(), g.rec_cell, static_cast<size_t>(n_links) * 4)) != LBM_OK) return rc;
         if ((rc = d2h(E, rec_mask.data(), g.rec_mask, static_cast<size_t>(n_links))) != LBM_OK) return rc;
     }
-    H.flags.resize(npitch);
-    if ((rc = d2h(E, H.flags.data(), g.flags, npitch)) != LBM_OK) return rc;
     H.fluid = static_cast<int64_t>(fluid);
     // records in row-major order of their nodes: simple bounce-back counted in slot 0
     // until the geometry below says otherwise
@@ -396,34 +401,48 @@ int build_geometry(Ensemble* E, HostLattice& H)
     for (const LinkRec& r : H.recs)
         for (int i = 1; i < 9; ++i)
             if ((r.mask & (1u << i)) && ((r.scheme >> (2 * i)) & 3) == LBM_BB_BOUZIDI) H.uses_bouzidi = true;
-    if (H.uses_bouzidi) {
-        // Bouzidi reads the stored value of upstream nodes, which may be solid
-        for (size_t p = 0; p < H.flags.size(); ++p)
-            if (H.flags[p] & CELL_SOLID) H.flags[p] |= CELL_WRITE;
+    H.special.clear();
+    if (!H.uses_bouzidi) {
+        // the inlet and outlet marks below, on the device; no written solids
+        if (d.x_edge == LBM_EDGE_OPEN && nx >= 4) {
+            const int inlet_from = d.inlet == LBM_INLET_ZOU_HE       ? 1
+                                   : d.inlet == LBM_INLET_PARABOLIC ? std::max(1, d.inlet_y0)
+                                                                    : ny;
+            LBM_HIP(launch_geom_columns(H.d_flags_new, nx, ny, H.pitch, inlet_from, d.outlet, E->stream));
+            LBM_HIP(hipStreamSynchronize(E->stream));
+        }
+        H.geom_dirty = false;
+        return LBM_OK;
     }
+    // Bouzidi reads the stored value of upstream nodes, which may be solid: every solid
+    // node is written, and the ones off the edges are listed (host pass over the flags)
+    std::vector<uint8_t> flags(npitch);
+    if ((rc = d2h(E, flags.data(), H.d_flags_new, npitch)) != LBM_OK) return rc;
+    for (size_t p = 0; p < flags.size(); ++p)
+        if (flags[p] & CELL_SOLID) flags[p] |= CELL_WRITE;
     // inlet and outlet nodes the step kernel's bulk tiles take: off the corners,
     // fluid, without links; at the outlet the copied node (nx-2) as well.  Their state is
     // the plain pulls plus the inlet closure (the closure overwrites exactly the slots
     // whose sources lie off the lattice), or the plain pulls of the copied node.
     if (d.x_edge == LBM_EDGE_OPEN && nx >= 4)
         for (int y = 1; y + 1 < ny; ++y) {
-            uint8_t& in = H.flags[static_cast<int64_t>(y) * H.pitch];
+            uint8_t& in = flags[static_cast<int64_t>(y) * H.pitch];
             const bool closed = d.inlet == LBM_INLET_ZOU_HE || (d.inlet == LBM_INLET_PARABOLIC && y >= d.inlet_y0);
             if (closed && !(in & (CELL_SOLID | CELL_LINKS))) in |= CELL_COLUMN;
-            uint8_t& out = H.flags[static_cast<int64_t>(y) * H.pitch + nx - 1];
-            const uint8_t src = H.flags[static_cast<int64_t>(y) * H.pitch + nx - 2];
+            uint8_t& out = flags[static_cast<int64_t>(y) * H.pitch + nx - 1];
+            const uint8_t src = flags[static_cast<int64_t>(y) * H.pitch + nx - 2];
             if (d.outlet && !(out & (CELL_SOLID | CELL_LINKS)) && !(src & (CELL_SOLID | CELL_LINKS)))
                 out |= CELL_COLUMN;
         }
     // written solids off the edges, for the one-lattice kernel's list blocks (its link
     // and edge blocks cover the other general-path nodes)
-    H.special.clear();
     for (int y = 1; y + 1 < ny; ++y)
         for (int x = 1; x + 1 < nx; ++x) {
-            const uint8_t f = H.flags[static_cast<int64_t>(y) * H.pitch + x];
+            const uint8_t f = flags[static_cast<int64_t>(y) * H.pitch + x];
             if ((f & CELL_SOLID) && (f & CELL_WRITE))
                 H.special.push_back(static_cast<int32_t>(static_cast<int64_t>(y) * H.pitch + x));
         }
+    if ((rc = h2d(E, H.d_flags_new, flags.data(), npitch)) != LBM_OK) return rc;
     H.geom_dirty = false;
     return LBM_OK;
 }
@@ -508,10 +527,8 @@ int commit(Ensemble* E)
             }
             std::vector<LinkRec> all;
             all.reserve(nrec);
-            std::vector<uint8_t> flags(E->cell_elems, 0);
             for (auto& H : E->lat) {
                 all.insert(all.end(), H.recs.begin(), H.recs.end());
-                std::copy(H.flags.begin(), H.flags.end(), flags.begin() + H.cell_off);
             }
             if ((rc = h2d(E, E->d_links, all.data(), all.size() * sizeof(LinkRec))) != LBM_OK) return rc;
             // each record's force terms go to its primary slot: that of its first counted link
@@ -550,16 +567,17 @@ int commit(Ensemble* E)
             }
             if ((rc = h2d(E, E->d_special, special.data(), special.size() * 4)) != LBM_OK) return rc;
             E->graph_dirty = true;
-            if ((rc = h2d(E, E->d_flags, flags.data(), flags.size())) != LBM_OK) return rc;
-            // link slots of the lattices just built, device to device (the others are in place)
+            // flags and link slots of the lattices just built, device to device (the others
+            // are in place)
             for (auto& H : E->lat)
-                if (H.d_lslot_new) {
-                    LBM_HIP(hipMemcpyAsync(E->d_lslot + H.cell_off, H.d_lslot_new,
-                                           static_cast<size_t>(H.d.ny) * H.pitch * 4, hipMemcpyDeviceToDevice,
+                if (H.d_lslot_new && H.d_flags_new) {
+                    const size_t np = static_cast<size_t>(H.d.ny) * H.pitch;
+                    LBM_HIP(hipMemcpyAsync(E->d_flags + H.cell_off, H.d_flags_new, np, hipMemcpyDeviceToDevice,
+                                           E->stream));
+                    LBM_HIP(hipMemcpyAsync(E->d_lslot + H.cell_off, H.d_lslot_new, np * 4, hipMemcpyDeviceToDevice,
                                            E->stream));
                     LBM_HIP(hipStreamSynchronize(E->stream));
-                    (void)hipFree(H.d_lslot_new);
-                    H.d_lslot_new = nullptr;
+                    drop_new_geometry(H);
                 }
             // wall distance of every lattice from its flags, on the device (read by the
             // LES lattices' collisions and by lbm_wall_distance)
@@ -1232,8 +1250,7 @@ int lbm_destroy(lbm_handle h)
                     E->d_dummy_nf, E->d_slot_start, E->d_slot_recs, E->gbuf[0], E->gbuf[1], E->d_tcode};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    for (auto& H : E->lat)
-        if (H.d_lslot_new) (void)hipFree(H.d_lslot_new);
+    for (auto& H : E->lat) lbm::drop_new_geometry(H);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
     delete h;
     return LBM_OK;

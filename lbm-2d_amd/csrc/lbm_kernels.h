@@ -193,6 +193,11 @@ struct GeomArgs {
 };
 
 hipError_t launch_geom_cells(const GeomArgs& g, hipStream_t s);
+// Inlet and outlet column marks (CELL_COLUMN) of rows 1..ny-2 of an open-x lattice:
+// the inlet node of rows >= inlet_from, the outlet node when outlet is set, each only
+// without SOLID or LINKS (at the outlet, on the copied node nx-2 as well).
+hipError_t launch_geom_columns(uint8_t* flags, int nx, int ny, int pitch, int inlet_from, int outlet,
+                               hipStream_t s);
 hipError_t launch_geom_links(const GeomArgs& g, hipStream_t s);
 hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s);
 hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s);

@@ -1428,6 +1428,18 @@ __global__ __launch_bounds__(256) void geom_cells_kernel(GeomArgs g)
     }
 }
 
+// The bulk tiles' inlet and outlet nodes (CELL_COLUMN): one thread per row 1..ny-2.
+__global__ __launch_bounds__(256) void geom_columns_kernel(uint8_t* flags, int nx, int ny, int pitch,
+                                                           int inlet_from, int outlet)
+{
+    const int y = 1 + static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (y > ny - 2) return;
+    uint8_t* row = flags + static_cast<int64_t>(y) * pitch;
+    constexpr uint8_t kBlocked = CELL_SOLID | CELL_LINKS;
+    if (y >= inlet_from && !(row[0] & kBlocked)) row[0] |= CELL_COLUMN;
+    if (outlet && !(row[nx - 1] & kBlocked) && !(row[nx - 2] & kBlocked)) row[nx - 1] |= CELL_COLUMN;
+}
+
 // One block per row: the row's link cells in x order, numbered from row_off[y] with a
 // ballot scan per wave and the four waves' totals in LDS.
 __global__ __launch_bounds__(256) void geom_links_kernel(GeomArgs g)
@@ -1988,6 +2000,15 @@ hipError_t launch_wall_distance(const uint8_t* flags, uint16_t* wdist, int nx, i
 hipError_t launch_geom_cells(const GeomArgs& g, hipStream_t s)
 {
     hipLaunchKernelGGL(geom_cells_kernel, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_geom_columns(uint8_t* flags, int nx, int ny, int pitch, int inlet_from, int outlet,
+                               hipStream_t s)
+{
+    if (ny < 3) return hipSuccess;
+    hipLaunchKernelGGL(geom_columns_kernel, dim3((ny - 2 + 255) / 256), dim3(256), 0, s, flags, nx, ny, pitch,
+                       inlet_from, outlet);
     return hipGetLastError();
 }
 
