@@ -137,6 +137,8 @@ struct Ensemble {
     bool own_stream = false;
     std::vector<HostLattice> lat;
     int64_t pool_elems = 0, cell_elems = 0;
+    int64_t guard_elems = 0;  // population buffers end with this many more elements: the
+                              // step kernel's pull addresses reach past the pool by < one row
     void* buf[2] = {nullptr, nullptr};
     uint8_t* d_flags = nullptr;
     int32_t* d_lslot = nullptr;
@@ -1194,6 +1196,7 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
     // end, so the buffers carry a guard of more than one row
     int64_t guard = 256;
     for (auto& H : E->lat) guard = std::max<int64_t>(guard, H.pitch + 256);
+    E->guard_elems = guard;
     int rc = LBM_OK;
     for (int b = 0; b < 2 && rc == LBM_OK; ++b) {
         hipError_t e = hipMalloc(&E->buf[b], static_cast<size_t>(pool + guard) * E->bpp);
@@ -1743,10 +1746,12 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     // wrote, ping-ponging between two copies of the population buffers (a launch that
     // always reads the same source would hide what streaming stores cost the next step's
     // reads).  The state, the accumulators and the history stay untouched.
+    // each copy carries the population buffers' guard: the kernels read from it
     const size_t pool_bytes = static_cast<size_t>(E->pool_elems) * E->bpp;
+    const size_t alloc_bytes = static_cast<size_t>(E->pool_elems + E->guard_elems) * E->bpp;
     void* scratch[2] = {nullptr, nullptr};
     for (int k = 0; k < 2; ++k)
-        if (hipMalloc(&scratch[k], pool_bytes) != hipSuccess) {
+        if (hipMalloc(&scratch[k], alloc_bytes) != hipSuccess) {
             if (scratch[0]) (void)hipFree(scratch[0]);
             return fail(LBM_ERR_NOMEM, "autotune scratch buffers");
         }
