@@ -140,6 +140,34 @@ def test_autotune_keeps_state():
     assert np.abs(ens.download(0) - o.f).max() <= TOL["f32"]
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_autotune_on_an_exactly_sized_pool(storage):
+    """The autotuner steps two copies of the population buffers; the step kernel's pulls
+    of a lattice's last rows reach past its pool, so the copies carry the buffers' guard.
+    1024 x 512 FP32 populations fill exactly nine 2 MiB pages (no row padding, no slack
+    after the pool).  Tuning leaves the state alone, and stepping afterwards matches, bit
+    for bit, a handle that never tuned."""
+    nx, ny = 1024, 512
+    assert (9 * nx * ny * 4) % (2 << 20) == 0
+    desc = G.desc_for_case(G.CYLINDER, nx, ny, tau=0.6, u_in=0.1)
+    mask = G.circle_mask(nx, ny, 256, 256, 40)
+    runs = []
+    for tune in (True, False):
+        e = G.Ensemble([desc], storage=storage)
+        e.set_obstacle(0, mask)
+        e.set_bb_circle(0, 256, 256, 40, G.BB_MEI)
+        e.init_equilibrium(0, 1.0, 0.1, 0.0)
+        e.step(4)
+        if tune:
+            before = e.download(0)
+            e.autotune()
+            np.testing.assert_array_equal(e.download(0), before)
+        e.step(20)
+        runs.append((e.download(0), e.forces(0)[0]))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
 def test_nan_flag_trips():
     c = CASES["cylinder"]
     o = build_oracle(c)
