@@ -404,47 +404,39 @@ int build_geometry(Ensemble* E, HostLattice& H)
         for (int i = 1; i < 9; ++i)
             if ((r.mask & (1u << i)) && ((r.scheme >> (2 * i)) & 3) == LBM_BB_BOUZIDI) H.uses_bouzidi = true;
     H.special.clear();
-    if (!H.uses_bouzidi) {
-        // the inlet and outlet marks below, on the device; no written solids
-        if (d.x_edge == LBM_EDGE_OPEN && nx >= 4) {
-            const int inlet_from = d.inlet == LBM_INLET_ZOU_HE       ? 1
-                                   : d.inlet == LBM_INLET_PARABOLIC ? std::max(1, d.inlet_y0)
-                                                                    : ny;
-            LBM_HIP(launch_geom_columns(H.d_flags_new, nx, ny, H.pitch, inlet_from, d.outlet, E->stream));
-            LBM_HIP(hipStreamSynchronize(E->stream));
-        }
-        H.geom_dirty = false;
-        return LBM_OK;
+    // inlet and outlet nodes the step kernel's bulk tiles take: off the corners, fluid,
+    // without links; at the outlet the copied node (nx-2) as well.  Their state is the
+    // plain pulls plus the inlet closure (the closure overwrites exactly the slots whose
+    // sources lie off the lattice), or the plain pulls of the copied node.
+    if (d.x_edge == LBM_EDGE_OPEN && nx >= 4) {
+        const int inlet_from = d.inlet == LBM_INLET_ZOU_HE       ? 1
+                               : d.inlet == LBM_INLET_PARABOLIC ? std::max(1, d.inlet_y0)
+                                                                : ny;
+        LBM_HIP(launch_geom_columns(H.d_flags_new, nx, ny, H.pitch, inlet_from, d.outlet, E->stream));
     }
-    // Bouzidi reads the stored value of upstream nodes, which may be solid: every solid
-    // node is written, and the ones off the edges are listed (host pass over the flags)
-    std::vector<uint8_t> flags(npitch);
-    if ((rc = d2h(E, flags.data(), H.d_flags_new, npitch)) != LBM_OK) return rc;
-    for (size_t p = 0; p < flags.size(); ++p)
-        if (flags[p] & CELL_SOLID) flags[p] |= CELL_WRITE;
-    // inlet and outlet nodes the step kernel's bulk tiles take: off the corners,
-    // fluid, without links; at the outlet the copied node (nx-2) as well.  Their state is
-    // the plain pulls plus the inlet closure (the closure overwrites exactly the slots
-    // whose sources lie off the lattice), or the plain pulls of the copied node.
-    if (d.x_edge == LBM_EDGE_OPEN && nx >= 4)
-        for (int y = 1; y + 1 < ny; ++y) {
-            uint8_t& in = flags[static_cast<int64_t>(y) * H.pitch];
-            const bool closed = d.inlet == LBM_INLET_ZOU_HE || (d.inlet == LBM_INLET_PARABOLIC && y >= d.inlet_y0);
-            if (closed && !(in & (CELL_SOLID | CELL_LINKS))) in |= CELL_COLUMN;
-            uint8_t& out = flags[static_cast<int64_t>(y) * H.pitch + nx - 1];
-            const uint8_t src = flags[static_cast<int64_t>(y) * H.pitch + nx - 2];
-            if (d.outlet && !(out & (CELL_SOLID | CELL_LINKS)) && !(src & (CELL_SOLID | CELL_LINKS)))
-                out |= CELL_COLUMN;
+    if (H.uses_bouzidi) {
+        // Bouzidi reads the stored value of upstream nodes, which may be solid: every
+        // solid node is written, and the ones off the edges are listed for the
+        // one-lattice kernel's list blocks (its link and edge blocks cover the other
+        // general-path nodes), in row-major order
+        LBM_HIP(hipMemsetAsync(rows, 0, static_cast<size_t>(ny) * 4, E->stream));
+        LBM_HIP(launch_geom_write(H.d_flags_new, nx, ny, H.pitch, rows, E->stream));
+        std::vector<int32_t> solid_off(static_cast<size_t>(ny) + 1, 0);
+        if ((rc = d2h(E, solid_off.data() + 1, rows, static_cast<size_t>(ny) * 4)) != LBM_OK) return rc;
+        for (int y = 0; y < ny; ++y) solid_off[y + 1] += solid_off[y];
+        const int32_t n_solid = solid_off[ny];
+        if (n_solid > 0) {
+            Scratch s_soff, s_list;
+            int32_t* soff = nullptr;
+            int32_t* list = nullptr;
+            if ((rc = s_soff.alloc(soff, ny)) || (rc = s_list.alloc(list, n_solid))) return rc;
+            if ((rc = h2d(E, soff, solid_off.data(), static_cast<size_t>(ny) * 4)) != LBM_OK) return rc;
+            LBM_HIP(launch_geom_solids(H.d_flags_new, nx, ny, H.pitch, soff, list, E->stream));
+            H.special.resize(static_cast<size_t>(n_solid));
+            if ((rc = d2h(E, H.special.data(), list, static_cast<size_t>(n_solid) * 4)) != LBM_OK) return rc;
         }
-    // written solids off the edges, for the one-lattice kernel's list blocks (its link
-    // and edge blocks cover the other general-path nodes)
-    for (int y = 1; y + 1 < ny; ++y)
-        for (int x = 1; x + 1 < nx; ++x) {
-            const uint8_t f = flags[static_cast<int64_t>(y) * H.pitch + x];
-            if ((f & CELL_SOLID) && (f & CELL_WRITE))
-                H.special.push_back(static_cast<int32_t>(static_cast<int64_t>(y) * H.pitch + x));
-        }
-    if ((rc = h2d(E, H.d_flags_new, flags.data(), npitch)) != LBM_OK) return rc;
+    }
+    LBM_HIP(hipStreamSynchronize(E->stream));
     H.geom_dirty = false;
     return LBM_OK;
 }

@@ -198,6 +198,12 @@ hipError_t launch_geom_cells(const GeomArgs& g, hipStream_t s);
 // without SOLID or LINKS (at the outlet, on the copied node nx-2 as well).
 hipError_t launch_geom_columns(uint8_t* flags, int nx, int ny, int pitch, int inlet_from, int outlet,
                                hipStream_t s);
+// Lattices with Bouzidi links: every solid node is written (CELL_WRITE); row_count[y]
+// (zeroed beforehand) counts the solid nodes off the edges, and the second pass lists
+// them, y * pitch + x in row-major order from row_off[y].
+hipError_t launch_geom_write(uint8_t* flags, int nx, int ny, int pitch, int32_t* row_count, hipStream_t s);
+hipError_t launch_geom_solids(const uint8_t* flags, int nx, int ny, int pitch, const int32_t* row_off,
+                              int32_t* out, hipStream_t s);
 hipError_t launch_geom_links(const GeomArgs& g, hipStream_t s);
 hipError_t launch_thermal(int storage, const StepArgs& a, const ThermalArgs& t, hipStream_t s);
 hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hipStream_t s);

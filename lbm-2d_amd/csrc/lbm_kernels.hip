@@ -1440,6 +1440,58 @@ __global__ __launch_bounds__(256) void geom_columns_kernel(uint8_t* flags, int n
     if (outlet && !(row[nx - 1] & kBlocked) && !(row[nx - 2] & kBlocked)) row[nx - 1] |= CELL_COLUMN;
 }
 
+// Bouzidi reads the stored value of upstream nodes, which may be solid: every solid node
+// is written, and the ones off the edges are counted per row for their list.
+__global__ __launch_bounds__(256) void geom_write_kernel(uint8_t* flags, int nx, int ny, int pitch,
+                                                         int32_t* row_count)
+{
+    const int x = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int y = static_cast<int>(blockIdx.y);
+    bool listed = false;
+    if (x < nx) {
+        uint8_t& f = flags[static_cast<int64_t>(y) * pitch + x];
+        if (f & CELL_SOLID) {
+            f |= CELL_WRITE;
+            listed = x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2;
+        }
+    }
+    const uint64_t b = __ballot(listed);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(row_count + y, __popcll(b));
+}
+
+// One block per row: the row's written solids off the edges in x order, numbered from
+// row_off[y] (ballot prefix per wave, the four waves' totals in LDS).
+__global__ __launch_bounds__(256) void geom_solids_kernel(const uint8_t* flags, int nx, int ny, int pitch,
+                                                          const int32_t* row_off, int32_t* out)
+{
+    __shared__ int wave_count[4];
+    const int y = static_cast<int>(blockIdx.x);
+    const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
+    const bool inner_row = y >= 1 && y <= ny - 2;
+    int base = row_off[y];
+    for (int x0 = 0; x0 < nx; x0 += 256) {
+        const int x = x0 + t;
+        bool listed = false;
+        if (inner_row && x >= 1 && x <= nx - 2) {
+            const uint8_t f = flags[static_cast<int64_t>(y) * pitch + x];
+            listed = (f & CELL_SOLID) && (f & CELL_WRITE);
+        }
+        const uint64_t b = __ballot(listed);
+        const int below = __popcll(b & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_count[w] = __popcll(b);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            off += k < w ? wave_count[k] : 0;
+            total += wave_count[k];
+        }
+        if (listed) out[base + off + below] = y * pitch + x;
+        base += total;
+        __syncthreads();  // wave_count is rewritten by the next chunk
+    }
+}
+
 // One block per row: the row's link cells in x order, numbered from row_off[y] with a
 // ballot scan per wave and the four waves' totals in LDS.
 __global__ __launch_bounds__(256) void geom_links_kernel(GeomArgs g)
@@ -2009,6 +2061,20 @@ hipError_t launch_geom_columns(uint8_t* flags, int nx, int ny, int pitch, int in
     if (ny < 3) return hipSuccess;
     hipLaunchKernelGGL(geom_columns_kernel, dim3((ny - 2 + 255) / 256), dim3(256), 0, s, flags, nx, ny, pitch,
                        inlet_from, outlet);
+    return hipGetLastError();
+}
+
+hipError_t launch_geom_write(uint8_t* flags, int nx, int ny, int pitch, int32_t* row_count, hipStream_t s)
+{
+    hipLaunchKernelGGL(geom_write_kernel, dim3((nx + 255) / 256, ny), dim3(256), 0, s, flags, nx, ny, pitch,
+                       row_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_geom_solids(const uint8_t* flags, int nx, int ny, int pitch, const int32_t* row_off,
+                              int32_t* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(geom_solids_kernel, dim3(ny), dim3(256), 0, s, flags, nx, ny, pitch, row_off, out);
     return hipGetLastError();
 }
 
