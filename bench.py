@@ -1,26 +1,38 @@
 """Throughput of the D2Q9 MRT lattice update on one MI355X.
 
-Headline (``value``): GLUPS of the FP32 fused MRT step on the cylinder-wake
-lattice, 800x300, circle r=30 at (200,150), Re=100 (tau=0.68), u=0.1, seed-42
-perturbation, Mei interpolated bounce-back, Zou/He inlet, outflow copy,
-periodic y -- the reference's semantics (src/main.cpp, src/lbm.hpp).  Inputs are
-resident in HBM; the timed region is K steps replayed from hipGraphs of 100.
+Headline (``value``): GLUPS of the FP32 fused MRT step on the largest single-GPU
+workload, the 32768 x 16384 domain (536.9 M cells) with Smagorinsky + Van Driest LES
+(c_s = 0.12, tau = 0.505, u = 0.1) and 128 seeded obstacles (64 circles r 20-60 with
+exact-circle Mei links, 64 plates 155 x 2 at angle of attack in [-15, 15] deg on a
+staircase with simple bounce-back), Zou/He inlet, outflow copy, periodic y -- the
+reference's semantics (src/lbm.hpp:293-582).  Inputs are resident in HBM; the timed
+region is exactly --steps steps after --warmup untimed ones, launched through the
+library's hipGraphs of --graph-steps launches (any remainder as plain launches).
+This workload as specified (impulsive start at u = 0.1, tau = 0.505, the reference's Mei
+bounce-back) is not stable: every storage form, FP64 included, produces its first
+non-finite density between steps 350 and 400, at the surface of an r = 59 circle.  The
+headline therefore runs warmup + steps + 120 steps of roofline timing, 230 by default,
+and reports the NaN flag.
 
-Extra forms on the same JSON line (``forms``): FP32 and FP16-shifted storage on
-the 6400x2400 LES cylinder, the 256-lattice batched ensemble against the same
-lattices stepped one launch each, and the 32768x16384 domain when
-``--large`` is given.
+``roofline`` is for the dominant kernel of the headline: algorithmic bytes per launch =
+(N_fluid*18*s + N_cells*1 + N_les*2) per step covered by the launch (s = bytes per stored
+population, the u16 wall distance with LES), divided by that kernel's average launch
+duration measured here with HIP events on the library's stream (torch events recorded on
+the stream the library launches on): the difference between replays of hipGraphs of 40
+and of 20 steps, which differ by exactly 10 two-step launches (or 20 one-step launches),
+against 8 TB/s.  ``traffic`` is the HBM bytes per launch read from the committed
+rocprofv3 PMC summary of the same workload and launch shape.
 
-``roofline`` is for the dominant kernel (the fused step kernel of the headline
-workload): algorithmic bytes per launch = N_fluid*18*s + N_cells*1 (s = bytes per
-stored population; + 2 B per node of the u16 wall distance with LES), divided by the
-average launch duration measured here with HIP events on the library's stream
-around hipGraph replays, against 8 TB/s.  ``traffic`` is the HBM bytes per launch
-read from the committed rocprofv3 PMC summary of the same workload.
+``forms`` (same JSON line): the 16-bit storage forms on the same domain, the 6400 x 2400
+LES cylinder in FP32 / FP16S / BF16S, the 800 x 300 cylinder (cache-resident: its two
+population buffers, 35 MB, fit in the 256 MB Infinity Cache), the 256-lattice batched
+ensemble against the same lattices stepped one launch each, and the thermal kernel.
 
-``cpu_baseline`` times the reference itself (oracle/_ref/libref_fast.so, built
-from the reference sources) or, when that is absent, the FP64 oracle, on this
-host's cores on a bounded sample of the same workload, in a child process.
+``cpu_baseline`` times the FP64 CPU restatement of the step (oracle/, with the LES wall
+distance computed once, since the reference recomputes it serially every step) on a
+labelled 8192 x 4096 sub-domain of the headline workload with the same obstacle density,
+on this host's cores (OMP_NUM_THREADS as set) and on one thread, in child processes; the
+reference itself (oracle/_ref) is timed beside it on the same sub-domain without LES.
 """
 from __future__ import annotations
 
@@ -38,6 +50,11 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "lbm-2d_amd"))
 
 HBM_PEAK_GBS = 8000.0
+LARGE_PROFILE = os.path.join(ROOT, "profiles", "step_kernel_large32768x16384_les_f32.md")
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def cylinder_lattice(nx, ny, cx, cy, r, tau, u, storage, stream, les=False, cs=0.1, perturb=True):
@@ -57,14 +74,18 @@ def cylinder_lattice(nx, ny, cx, cy, r, tau, u, storage, stream, les=False, cs=0
     return ens, mask
 
 
-def algorithmic_bytes(info, les_bytes):
+def algorithmic_bytes(info, les_bytes=2):
+    """Bytes per step: 18 population moves per fluid node, the flag byte of every node,
+    the wall distance of every node of an LES lattice."""
     s = info.bytes_per_pop
     return info.fluid_cells * 18 * s + info.cells * 1 + info.les_cells * les_bytes
 
 
-def time_steps(ens, steps, warmup, torch, graph_steps=100):
-    # the first call of graph_steps or more steps captures the hipGraph: keep it untimed
-    ens.step(max(warmup, graph_steps))
+def time_steps(ens, steps, warmup, torch):
+    """Seconds for exactly `steps` steps after `warmup` untimed ones, bracketed by
+    synchronisations of the library's stream and the device."""
+    if warmup:
+        ens.step(warmup)
     ens.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -75,9 +96,9 @@ def time_steps(ens, steps, warmup, torch, graph_steps=100):
 
 
 def graph_step_ms(ens, torch, replays=20, graph_steps=100):
-    """Average duration of one step-kernel launch: HIP events on the library's stream
-    around each replay of a hipGraph of graph_steps launches, whose kernels run back to
-    back (the per-step counter update is one 1-thread kernel per replay)."""
+    """Average device time per step: HIP events on the library's stream around each
+    replay of a hipGraph of graph_steps steps."""
+    ens.set_launch(graph_steps=graph_steps)
     stream = torch.cuda.current_stream()
     ens.step(graph_steps)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(replays + 1)]
@@ -87,6 +108,27 @@ def graph_step_ms(ens, torch, replays=20, graph_steps=100):
         evs[i + 1].record(stream)
     torch.cuda.synchronize()
     return float(np.mean([evs[i].elapsed_time(evs[i + 1]) for i in range(replays)])) / graph_steps
+
+
+def launches(n, two_step):
+    """Step-kernel launches of a batch of n steps (two-step launches end with one-step ones)."""
+    if not two_step or n < 2:
+        return n
+    pairs = (n - 1) // 2
+    return pairs + (n - 2 * pairs)
+
+
+def kernel_launch_ms(ens, torch, two_step, short=20, long=40, replays=1):
+    """Average duration of one launch of the dominant step kernel: HIP events on the
+    library's stream around replays of hipGraphs of `short` and of `long` steps; their
+    difference is (long - short) / 2 two-step launches (or long - short one-step
+    launches) and nothing else of note (the longer batch's force flush covers more
+    launches: ~1 us).  Steps taken: 2 (1 + replays) (short + long)."""
+    per = {}
+    for g in (short, long):
+        per[g] = graph_step_ms(ens, torch, replays=replays, graph_steps=g) * g
+    extra = launches(long, two_step) - launches(short, two_step)
+    return (per[long] - per[short]) / extra, per
 
 
 def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1, launch="", sr=False):
@@ -102,17 +144,16 @@ def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False,
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
     kms = graph_step_ms(ens, torch, replays=5)
-    b = algorithmic_bytes(info, 2)
-    m0 = ens.mass(0)
+    b = algorithmic_bytes(info)
     out = {
         "lattice": f"{nx}x{ny}", "storage": storage, "les": les, "rounding": "stochastic" if sr else "nearest-even",
         "GLUPS": info.cells * steps / t / 1e9,
         "ms_per_step": 1e3 * t / steps,
-        "kernel_ms": kms,
+        "kernel_ms_per_step": kms,
         "bytes_per_step": b,
         "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "launch": list(shape),
-        "mass": m0,
+        "nan": ens.nan_flag(0),
     }
     ens.close()
     return out
@@ -145,106 +186,29 @@ def bench_thermal(nx, ny, storage, steps, warmup, torch, stream, launch=""):
     kms = graph_step_ms(ens, torch, replays=5)
     s = info.bytes_per_pop
     b = info.fluid_cells * 36 * s + info.cells * 2  # f and g planes, flag and condition bytes
-    T = ens.temperature(0, ny // 2, ny // 2 + 1)
     out = {
         "lattice": f"{nx}x{ny}", "storage": storage, "tau": tau,
         "GLUPS": info.cells * steps / t / 1e9,
         "ms_per_step": 1e3 * t / steps,
-        "kernel_ms": kms,
+        "kernel_ms_per_step": kms,
         "bytes_per_step": b,
         "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "T_centre_row_max": float(T.max()),
-        "launch_options": info.launch_options,
     }
     ens.close()
     return out
 
 
-def ensemble_descs(seed=42, n=256):
-    """Batched-ensemble lattices: NX~U[256,1024], NY~U[96,384], 1-4 circles r~U[8,40],
-    centres uniform with a 2r margin, non-overlapping; Re log-uniform in [50,400];
-    tau = 0.5 + 3*0.1*D/Re with D the largest diameter.  numpy Generator(PCG64(seed))."""
-    import lbm2d as G
-    rng = np.random.default_rng(seed)
-    lats = []
-    for _ in range(n):
-        nx = int(rng.integers(256, 1025))
-        ny = int(rng.integers(96, 385))
-        k = int(rng.integers(1, 5))
-        circles = []
-        tries = 0
-        while len(circles) < k and tries < 200:
-            tries += 1
-            r = int(rng.integers(8, 41))
-            if 4 * r >= ny - 2 or 4 * r >= nx - 2:
-                continue
-            cx = int(rng.integers(2 * r, nx - 2 * r))
-            cy = int(rng.integers(2 * r, ny - 2 * r))
-            if all((cx - a) ** 2 + (cy - b) ** 2 > (r + c + 2) ** 2 for a, b, c in circles):
-                circles.append((cx, cy, r))
-        re = float(np.exp(rng.uniform(np.log(50), np.log(400))))
-        dmax = 2 * max(c[2] for c in circles)
-        tau = 0.5 + 3 * 0.1 * dmax / re
-        d = G.desc_for_case(G.CYLINDER, nx, ny, tau=tau, u_in=0.1)
-        mask = np.zeros((ny, nx), np.uint8)
-        for cx, cy, r in circles:
-            mask |= G.circle_mask(nx, ny, cx, cy, r)
-        lats.append((d, mask, circles))
-    return lats
-
-
-def ensemble_links(mask, circles, nx, ny):
-    """Per-obstacle exact-circle q for every link (nearest circle owns the link)."""
-    import lbm2d as G
-    links = []
-    ys, xs = np.nonzero(mask == 0)
-    solid = mask.astype(bool)
-    for i in range(1, 9):
-        tx = xs + G.CXV[i]
-        ty = (ys + G.CYV[i]) % ny
-        ok = (tx >= 0) & (tx < nx)
-        sel = np.nonzero(ok)[0]
-        hit = sel[solid[ty[sel], tx[sel]]]
-        for k in hit:
-            x, y = int(xs[k]), int(ys[k])
-            sx, sy = x + G.CXV[i], y + G.CYV[i]
-            c = min(circles, key=lambda c: (sx - c[0]) ** 2 + (sy - c[1]) ** 2)
-            q = _q_circle(c[0], c[1], c[2], x, y, i)
-            links.append((y * nx + x, i, G.BB_MEI, 0, q, 0.0, 0.0))
-    return links
-
-
-def _q_circle(ccx, ccy, r, xf, yf, i):
-    import lbm2d as G
-    dx, dy = xf - ccx, yf - ccy
-    ex, ey = float(G.CXV[i]), float(G.CYV[i])
-    a = ex * ex + ey * ey
-    b = 2.0 * (dx * ex + dy * ey)
-    c = dx * dx + dy * dy - r * r
-    disc = b * b - 4.0 * a * c
-    if disc < 0.0:
-        return 1.0
-    sd = np.sqrt(disc)
-    t1, t2 = (-b - sd) / (2 * a), (-b + sd) / (2 * a)
-    if 0.0 < t1 <= 1.0:
-        return float(t1)
-    if 0.0 < t2 <= 1.0:
-        return float(t2)
-    return 1.0
-
-
 def bench_ensemble(steps, warmup, torch, stream, n=256):
-    import lbm2d as G
-    lats = ensemble_descs(n=n)
-    ens = G.Ensemble([d for d, _, _ in lats], storage="f32", stream=stream)
-    for l, (d, mask, circles) in enumerate(lats):
-        ens.set_obstacle(l, mask)
-        ens.set_links(l, ensemble_links(mask, circles, d.nx, d.ny))
-        f = G.equilibrium(np.ones((d.ny, d.nx)), np.full((d.ny, d.nx), 0.1), np.zeros((d.ny, d.nx)))
-        ens.upload(l, f)
+    """The seeded ensemble (workloads.ensemble_descs): one launch per step over every
+    lattice, hipGraphs of 100 steps, against the same lattices stepped one launch each."""
+    import workloads as W
+    lats = W.ensemble_descs(n=n)
+    ens = W.build_ensemble(lats, "f32", stream)
     shape, _ = ens.autotune()
+    ens.set_launch(graph_steps=100)
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
+    kms = graph_step_ms(ens, torch, replays=3)
     # the same lattices, one launch per lattice per step, no graph
     plain_steps = max(2, steps // 20)
     for l in range(n):
@@ -258,11 +222,12 @@ def bench_ensemble(steps, warmup, torch, stream, n=256):
     ens.sync()
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
-    b = algorithmic_bytes(info, 0)
+    b = algorithmic_bytes(info)
     glups = info.cells * steps / t / 1e9
     glups_plain = info.cells * plain_steps / tp / 1e9
-    # per-lattice Cd = 2 Fx / (D u^2), D the largest diameter, mean over the last 100 steps
-    # run (the flows are still starting up: these are early-time values)
+    nans = [l for l in range(n) if ens.nan_flag(l)]
+    # per-lattice Cd = 2 Fx / (D u^2), D the largest diameter, mean over the last 100
+    # steps run: early-time values of flows started from rest about the obstacles
     cds = []
     for l, (d, _, circles) in enumerate(lats):
         fxl, _ = ens.forces(l)
@@ -270,172 +235,144 @@ def bench_ensemble(steps, warmup, torch, stream, n=256):
         cds.append(float(2.0 * fxl[-100:, 0].mean() / (dmax * 0.01)))
     out = {
         "lattices": n, "cells": info.cells, "storage": "f32",
+        "les_lattices": int(sum(1 for d, _, _ in lats if d.les)),
         "GLUPS": glups, "ms_per_step": 1e3 * t / steps,
-        "hbm_frac": b / (t / steps) / 1e9 / HBM_PEAK_GBS,
+        "kernel_ms_per_step": kms,
+        "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "bytes_per_step": b,
         "plain_per_lattice_GLUPS": glups_plain,
         "speedup_vs_plain": glups / glups_plain,
         "launch": list(shape),
+        "nan_lattices": nans,
+        "steps_run": ens.steps_done(0),
         "cd_per_lattice": [float(f"{c:.4g}") for c in cds],
     }
     ens.close()
     return out
 
 
-def large_domain(seed=42, nx=32768, ny=16384, n_circles=64, n_plates=64):
-    """Obstacles of the 32768x16384 domain: n_circles circles r~U[20,60] (exact-circle
-    Mei links, force slots 1..n_circles) and n_plates flat plates 155x2 at angle of attack
-    ~U[-15,15] deg (polygon staircase, simple bounce-back, force slot 0).  Centres uniform
-    in x in [nx/16, 15nx/16) and in y clear of the periodic seam, bounding circles at least
-    8 cells apart; numpy Generator(PCG64(seed)), circles first.  Returns the mask and the
-    explicit link list."""
-    import lbm2d as G
-    rng = np.random.default_rng(seed)
-    mask = np.zeros((ny, nx), np.uint8)
-    placed, links = [], []
-    half_diag = float(np.hypot(77.5, 1.0))
-    for k in range(n_circles + n_plates):
-        circle = k < n_circles
-        while True:
-            if circle:
-                r = int(rng.integers(20, 61))
-                br = float(r)
-            else:
-                aoa = float(rng.uniform(-15.0, 15.0))
-                br = half_diag
-            cx = int(rng.integers(nx // 16, nx - nx // 16))
-            cy = int(rng.integers(int(br) + 4, ny - int(br) - 4))
-            if all((cx - a) ** 2 + (cy - b) ** 2 > (br + c + 8) ** 2 for a, b, c in placed):
-                break
-        placed.append((cx, cy, br))
-        x0, y0 = cx - int(br) - 2, cy - int(br) - 2
-        w = h = 2 * int(br) + 5
-        if circle:
-            yy, xx = np.mgrid[y0:y0 + h, x0:x0 + w]
-            dx, dy = (xx - cx).astype(float), (yy - cy).astype(float)
-            mask[y0:y0 + h, x0:x0 + w] |= (np.sqrt(dx * dx + dy * dy) < r).astype(np.uint8)
-            sub = mask[y0:y0 + h, x0:x0 + w]
-            ys, xs = np.nonzero(sub[1:-1, 1:-1] == 0)
-            ys, xs = ys + 1, xs + 1
-            for i in range(1, 9):
-                sx, sy = (xs + G.CXV[i] + x0 - cx).astype(float), (ys + G.CYV[i] + y0 - cy).astype(float)
-                hit = (sub[ys + G.CYV[i], xs + G.CXV[i]] != 0) & (np.sqrt(sx * sx + sy * sy) < r)
-                for yl, xl in zip(ys[hit], xs[hit]):
-                    x, y = x0 + int(xl), y0 + int(yl)
-                    links.append((y * nx + x, i, G.BB_MEI, 1 + k, _q_circle(cx, cy, r, x, y, i), 0.0, 0.0))
-        else:
-            xy = G.plate_polygon(float(cx), float(cy), 155.0, aoa, 2.0)
-            mask[y0:y0 + h, x0:x0 + w] |= G.polygon_mask(w, h, xy - np.array([x0, y0], float))
-    return mask, links
-
-
-def bench_large(steps, warmup, torch, stream, horizon=100, seed=42):
-    """The 32768x16384 LES domain (tau=0.505, u=0.1, cs=0.12, periodic y): FP64 / FP32 /
-    FP16S / BF16S lattices from the same geometry and a uniform equilibrium start; after
-    `horizon` steps the velocity field of each reduced form against FP32 and against the
-    FP64 form (RMS and max of |u - u_ref| over fluid nodes, in units of u_in); then
-    autotuned throughput of each reduced form."""
-    import lbm2d as G
-    nx, ny, tau, u, cs = 32768, 16384, 0.505, 0.1, 0.12
+def bench_large(geom, form, steps, warmup, torch, stream, launch="", graph_steps=20, roofline=False):
+    """One storage form of the large LES domain: set-up, autotune (or --launch), exactly
+    `steps` timed steps after `warmup`; with roofline, the dominant kernel's launch
+    duration from graph replays."""
+    import workloads as W
+    mask, links = geom
+    storage = form.split("_")[0]
     t0 = time.perf_counter()
-    mask, links = large_domain(seed, nx, ny)
-    t_geom = time.perf_counter() - t0
-    n_slots = 1 + max(k[3] for k in links)
-    lat = {}
-    for form in ("f64", "f32", "f16s", "bf16s", "f16s_sr", "bf16s_sr"):
-        storage = form.split("_")[0]
-        t1 = time.perf_counter()
-        d = G.desc_for_case(G.CYLINDER, nx, ny, tau=tau, u_in=u, les=True, cs=cs)
-        e = G.Ensemble([d], storage=storage, stream=stream)
-        if form.endswith("_sr"):
-            e.set_stochastic_rounding(True, 1)
-        e.set_obstacle(0, mask)
-        e.set_force_slots(0, n_slots)
-        e.set_links(0, links)
-        e.init_equilibrium(0, 1.0, u, 0.0)
-        if storage == "f64":
-            m0 = e.mass(0)
-        e.step(horizon)
-        e.sync()
-        lat[form] = e
-        print(f"# 32768x16384 {form}: built and stepped {horizon} in {time.perf_counter() - t1:.1f} s",
-              file=sys.stderr, flush=True)
-    out = {"cells": nx * ny, "obstacles": "64 circles r 20-60 (Mei links) + 64 plates 155x2 (staircase)",
-           "seed": seed, "links": len(links), "host_geometry_s": t_geom, "horizon": horizon}
-    strip = 1024
-    fluid = mask == 0
-    out["fp64_mass_drift"] = lat["f64"].mass(0) / m0 - 1.0
-    pairs = [(s, r) for r in ("f64", "f32") for s in ("f32", "f16s", "bf16s", "f16s_sr", "bf16s_sr") if s != r]
-    acc = {p: [0.0, 0.0, 0] for p in pairs}
-    for y0 in range(0, ny, strip):
-        y1 = min(ny, y0 + strip)
-        fl = fluid[y0:y1]
-        vel = {k: e.macros(0, y0, y1)[1:] for k, e in lat.items()}
-        for s, r in pairs:
-            d2 = ((vel[s][0] - vel[r][0]) ** 2 + (vel[s][1] - vel[r][1]) ** 2)[fl]
-            a = acc[(s, r)]
-            a[0] += float(d2.sum())
-            a[1] = max(a[1], float(np.sqrt(d2.max())) if d2.size else 0.0)
-            a[2] += int(d2.size)
-        if y0 % 4096 == 0:
-            print(f"# 32768x16384 velocity errors: rows {y1}/{ny}", file=sys.stderr, flush=True)
-    for (s, r), (sq, mx, n) in acc.items():
-        out[f"vel_err_{s}_vs_{r}"] = {"rms": float(np.sqrt(sq / n)) / u, "max": mx / u}
-    lat.pop("f64").close()
-    for s in ("f32", "f16s", "bf16s", "f16s_sr", "bf16s_sr"):
-        e = lat.pop(s)
+    e = W.large_lattice(mask, links, storage, stream, sr=form.endswith("_sr"))
+    e.sync()
+    t_setup = time.perf_counter() - t0
+    if launch:
+        shape = tuple(int(v) for v in launch.split(","))
+        e.set_launch(*shape)
+    else:
         shape, _ = e.autotune()
-        info = e.info()
-        t = time_steps(e, steps, warmup, torch)
+    e.set_launch(graph_steps=graph_steps)
+    info = e.info()
+    m0 = e.mass(0)
+    log(f"32768x16384 {form}: set up in {t_setup:.2f} s, launch {tuple(shape)}")
+    t = time_steps(e, steps, warmup, torch)
+    b = algorithmic_bytes(info)
+    out = {"storage": storage, "rounding": "stochastic" if form.endswith("_sr") else "nearest-even",
+           "GLUPS": info.cells * steps / t / 1e9, "ms_per_step": 1e3 * t / steps, "seconds": t,
+           "bytes_per_step": b, "launch": list(shape), "cells": info.cells, "fluid_cells": info.fluid_cells,
+           "les_cells": info.les_cells, "setup_s": t_setup}
+    two = bool(shape[3] & 4)
+    if roofline:
+        lms, per = kernel_launch_ms(e, torch, two)
+        out["launch_ms"] = lms
+        out["graph_ms"] = {str(k): v for k, v in per.items()}
+        out["steps_per_launch"] = 2 if two else 1
+        out["hbm_frac"] = b * out["steps_per_launch"] / (lms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    else:
         kms = graph_step_ms(e, torch, replays=2, graph_steps=20)
-        b = algorithmic_bytes(info, 2)
-        out[s] = {"GLUPS": info.cells * steps / t / 1e9, "ms_per_step": 1e3 * t / steps, "kernel_ms": kms,
-                  "bytes_per_step": b, "hbm_frac": b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                  "launch": list(shape), "fluid_cells": info.fluid_cells, "nan": e.nan_flag(0)}
-        e.close()
-        print(f"# 32768x16384 {s}: {out[s]['GLUPS']:.1f} GLUPS", file=sys.stderr, flush=True)
+        out["kernel_ms_per_step"] = kms
+        out["hbm_frac"] = b / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    out["nan"] = e.nan_flag(0)
+    out["fp32_mass_drift"] = e.mass(0) / m0 - 1.0
+    out["steps_run"] = e.steps_done(0)
+    e.close()
+    log(f"32768x16384 {form}: {out['GLUPS']:.1f} GLUPS")
     return out
 
 
-def cpu_baseline(seconds=12.0):
-    """Child-process timing of the reference (or the oracle) on the cylinder workload."""
+CPU_SUB = dict(nx=8192, ny=4096, n_circles=4, n_plates=4)
+
+
+def cpu_baseline(seconds=10.0):
+    """The FP64 CPU step on a 8192 x 4096 sub-domain of the headline workload: 4 circles
+    and 4 plates placed as large_domain() places 64 + 64 on the full domain (the same
+    obstacle density), tau = 0.505, u = 0.1, Zou/He inlet, outflow, periodic y.  The
+    reference keeps one interpolated-bounce-back geometry, so the first circle is it.
+    Timed: the oracle with LES (wall distance computed once) on OMP_NUM_THREADS threads
+    and on one thread, and the reference itself without LES on OMP_NUM_THREADS threads,
+    each for about `seconds` of steps after one untimed step, in a child process."""
     code = f"""
 import json, os, sys, time
-sys.path.insert(0, {ROOT!r})
+sys.path.insert(0, {ROOT!r}); sys.path.insert(0, {os.path.join(ROOT, 'lbm-2d_amd')!r})
+import numpy as np
 import oracle as O
-threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count()))
-kind = 'reference' if O.ref_available(fast=True) else 'port'
-nx, ny, tau, u = 800, 300, 0.68, 0.1
+import workloads as W
+kind = sys.argv[1]
+nx, ny, nc, npl = {CPU_SUB['nx']}, {CPU_SUB['ny']}, {CPU_SUB['n_circles']}, {CPU_SUB['n_plates']}
+mask, links, shapes = W.large_domain(42, nx, ny, nc, npl, with_shapes=True)
+_, cx0, cy0, r0 = shapes[0]  # the reference's single bounce-back geometry
 if kind == 'reference':
-    s = O.RefLattice(nx, ny, O.CYLINDER, fast=True)
+    s = O.RefLattice(nx, ny, O.CYLINDER, les=False, fast=True)
 else:
-    s = O.Oracle(nx, ny, O.CYLINDER, faithful=True)
-s.place_cylinder(200, 150, 30)
-s.init_equilibrium(1.0, u, 0.0)
-s.perturb_seed42(205, 260)
-s.step(tau, u, 2)
+    s = O.Oracle(nx, ny, O.CYLINDER, les=True, cs=0.12, faithful=False)
+s.set_obstacle(mask)
+s.place_cylinder(cx0, cy0, r0)
+s.init_equilibrium(1.0, 0.1, 0.0)
+tau, u = 0.505, 0.1
+s.step(tau, u, 1)
 n, total = 0, 0.0
-chunk = 20
 while total < {seconds}:
-    t0 = time.perf_counter(); s.step(tau, u, chunk); total += time.perf_counter() - t0; n += chunk
-print(json.dumps({{'kind': kind, 'steps': n, 'seconds': total, 'threads': threads,
-                  'mlups': nx * ny * n / total / 1e6}}))
+    t0 = time.perf_counter(); s.step(tau, u, 1); total += time.perf_counter() - t0; n += 1
+print(json.dumps({{'steps': n, 'seconds': total, 'threads': int(os.environ['OMP_NUM_THREADS']),
+                  'glups': nx * ny * n / total / 1e9}}))
 """
-    env = dict(os.environ)
-    env.setdefault("OMP_NUM_THREADS", str(os.cpu_count()))
-    env["OMP_PROC_BIND"] = "close"
-    env["OMP_PLACES"] = "cores"
-    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
-                         timeout=300)
-    if out.returncode != 0:
-        return {"error": out.stderr[-400:]}
-    r = json.loads(out.stdout.strip().splitlines()[-1])
-    return {"value": r["mlups"] / 1e3, "unit": "GLUPS", "cores": r["threads"], "kind": r["kind"],
-            "sample": f"800x300 cylinder MRT, {r['steps']} steps in {r['seconds']:.1f} s, "
-                      f"FP64 AoS, OpenMP {r['threads']} threads"}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    runs = {}
+    for name, kind, nthr in (("port", "port", threads), ("port_1thread", "port", 1),
+                             ("reference_no_les", "reference", threads)):
+        import oracle as O
+        if kind == "reference" and not O.ref_available(fast=True):
+            continue
+        env = dict(os.environ)
+        env["OMP_NUM_THREADS"] = str(nthr)
+        env["OMP_PROC_BIND"] = "close"
+        env["OMP_PLACES"] = "cores"
+        out = subprocess.run([sys.executable, "-c", code, kind], capture_output=True, text=True, env=env,
+                             timeout=600)
+        if out.returncode != 0:
+            runs[name] = {"error": out.stderr[-400:]}
+            continue
+        runs[name] = json.loads(out.stdout.strip().splitlines()[-1])
+        log(f"cpu baseline {name}: {runs[name]['glups'] * 1e3:.1f} MLUPS on {nthr} threads")
+    model = ""
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    p = runs.get("port", {})
+    if "glups" not in p:
+        return {"error": p.get("error", "no run"), "runs": runs}
+    sub = f"{CPU_SUB['nx']}x{CPU_SUB['ny']}"
+    return {"value": p["glups"], "unit": "GLUPS", "cores": p["threads"], "kind": "port",
+            "sample": f"{sub} sub-domain of the 32768x16384 LES workload ({CPU_SUB['n_circles']} circles + "
+                      f"{CPU_SUB['n_plates']} plates, same obstacle density), FP64 AoS restatement of the "
+                      f"step with the LES wall distance computed once (the reference recomputes it "
+                      f"serially every step), {p['steps']} steps in {p['seconds']:.1f} s on "
+                      f"{p['threads']} OpenMP threads",
+            "one_thread_GLUPS": runs.get("port_1thread", {}).get("glups"),
+            "reference_no_les": runs.get("reference_no_les"),
+            "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "cpu_model": model}
 
 
-def profiled_traffic(path=os.path.join(ROOT, "profiles", "step_kernel_cylinder800x300_f32.md")):
+def profiled_traffic(path=LARGE_PROFILE):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration)."""
     try:
@@ -447,27 +384,54 @@ def profiled_traffic(path=os.path.join(ROOT, "profiles", "step_kernel_cylinder80
     return None
 
 
+def run_form(name, args, torch, stream, geom_cache):
+    """One extra form by name (profiling runs use these alone)."""
+    if name.startswith("large_"):
+        if "geom" not in geom_cache:
+            import workloads as W
+            geom_cache["geom"] = W.large_domain()
+        form = name[len("large_"):]
+        return bench_large(geom_cache["geom"], form, args.steps, args.warmup, torch, stream,
+                           launch=args.launch, graph_steps=args.graph_steps)
+    if name == "batched_256":
+        return bench_ensemble(max(args.steps, 200), 20, torch, stream)
+    if name.startswith("thermal_"):
+        storage, size = name.split("_")[1:3]
+        nx, ny = (int(v) for v in size.split("x"))
+        return bench_thermal(nx, ny, storage, 200, 20, torch, stream, launch=args.launch)
+    if name == "f32_800x300":
+        return bench_form(800, 300, 30, 0.68, "f32", 2000, 100, torch, stream, launch=args.launch)
+    storage = name.split("_")[0]
+    return bench_form(6400, 2400, 240, 0.644, storage, 200, 20, torch, stream, les=True,
+                      launch=args.launch, sr="_sr_" in name)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50000)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=100, help="timed steps of the headline workload")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed steps before them")
     ap.add_argument("--launch", default="", help="fix the step-kernel launch: bx,rows,nodes,options "
                                                    "(default: autotune)")
-    ap.add_argument("--large", action="store_true", help="also time the 32768x16384 domain")
+    ap.add_argument("--graph-steps", type=int, default=20, help="launches per hipGraph (even)")
     ap.add_argument("--no-forms", action="store_true", help="headline workload only")
-    ap.add_argument("--graph-steps", type=int, default=500,
-                    help="launches per hipGraph of the headline run (forces still recorded every step)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", default="", help="run one extra form only and print it (profiling): "
-                                                "f32_6400x2400_les, f16s_6400x2400_les, f16s_sr_6400x2400_les, "
-                                                "bf16s_6400x2400_les "
-                                                "(6400x2400 forms take --launch), "
-                                                "batched_256, large_32768x16384, "
-                                                "thermal_<f32|f64>_<nx>x<ny>")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--form", default="", help="run one workload only and print it (profiling): "
+                                                "large_<f32|f16s|bf16s|f16s_sr|bf16s_sr> (takes --steps, "
+                                                "--warmup, --launch), f32_6400x2400_les, f16s_6400x2400_les, "
+                                                "f16s_sr_6400x2400_les, bf16s_6400x2400_les, f32_800x300 "
+                                                "(these take --launch), batched_256, thermal_<f32|f64>_<nx>x<ny>")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the lattice update is measured on a single MI355X).", file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0 or args.graph_steps < 2 or args.graph_steps % 2:
+        print("--steps >= 1, --warmup >= 0 and an even --graph-steps >= 2 are required", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         import torch
         a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -476,101 +440,70 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
         return
-    if args.form:
-        import torch
-        torch.cuda.init()
-        stream = torch.cuda.Stream()
-        with torch.cuda.stream(stream):
-            if args.form == "batched_256":
-                out = bench_ensemble(200, 20, torch, stream.cuda_stream)
-            elif args.form == "large_32768x16384":
-                out = bench_large(100, 10, torch, stream.cuda_stream)
-            elif args.form.startswith("thermal_"):
-                storage, size = args.form.split("_")[1:3]
-                nx, ny = (int(v) for v in size.split("x"))
-                out = bench_thermal(nx, ny, storage, 100, 10, torch, stream.cuda_stream, launch=args.launch)
-            else:
-                storage = args.form.split("_")[0]
-                out = bench_form(6400, 2400, 240, 0.644, storage, 100, 10, torch, stream.cuda_stream, les=True,
-                                 launch=args.launch, sr="_sr_" in args.form)
-        print(json.dumps({args.form: out}))
-        return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the lattice update is measured on a single MI355X).", file=sys.stderr)
-        sys.exit(2)
-    gs = args.graph_steps
-    if args.steps % gs != 0:
-        args.steps = max(gs, args.steps // gs * gs)
 
+    if args.warmup + args.steps + 120 > 340 and not args.form:
+        log(f"warning: {args.warmup + args.steps + 120} steps of the 32768x16384 workload: it diverges "
+            "after about 350 steps (FP64 too); the rate is unaffected, the NaN flag will be set")
     import torch
-    import lbm2d as G
+    import workloads as W
     torch.cuda.init()
     stream = torch.cuda.Stream()
-    with torch.cuda.stream(stream):
-        ens, mask = cylinder_lattice(800, 300, 200, 150, 30, 0.68, 0.1, "f32", stream.cuda_stream)
-        if args.launch:
-            shape = tuple(int(v) for v in args.launch.split(","))
-            ens.set_launch(*shape)
-        else:
-            shape, _ = ens.autotune()
-        ens.set_launch(graph_steps=gs)
-        info = ens.info()
-        print(f"# headline lattice ready, launch {shape}", file=sys.stderr, flush=True)
-        t = time_steps(ens, args.steps, args.warmup, torch, gs)
-        glups = info.cells * args.steps / t / 1e9
-        kms = graph_step_ms(ens, torch, replays=max(2, 2000 // gs), graph_steps=gs)
-        bytes_launch = algorithmic_bytes(info, 0)
-        fx, fy = ens.forces(0)
-        cd_last = 2.0 * fx[-1, 0] / (60 * 0.01)
-        ens.close()
-        print(f"# headline done: {glups:.1f} GLUPS", file=sys.stderr, flush=True)
+    geom_cache = {}
+    if args.form:
+        with torch.cuda.stream(stream):
+            out = run_form(args.form, args, torch, stream.cuda_stream, geom_cache)
+        print(json.dumps({args.form: out}))
+        return
 
+    with torch.cuda.stream(stream):
+        t0 = time.perf_counter()
+        geom_cache["geom"] = W.large_domain()
+        log(f"32768x16384 geometry ({len(geom_cache['geom'][1])} links) in {time.perf_counter() - t0:.1f} s")
+        head = bench_large(geom_cache["geom"], "f32", args.steps, args.warmup, torch, stream.cuda_stream,
+                           launch=args.launch, graph_steps=args.graph_steps, roofline=True)
         forms = {}
         if not args.no_forms:
-            big_steps = 200
-            for storage in ("f32", "f16s", "bf16s"):
-                forms[f"{storage}_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, storage, big_steps, 20,
-                                                               torch, stream.cuda_stream, les=True)
-                print(f"# {storage} 6400x2400 done", file=sys.stderr, flush=True)
-            forms["f16s_sr_6400x2400_les"] = bench_form(6400, 2400, 240, 0.644, "f16s", big_steps, 20, torch,
-                                                        stream.cuda_stream, les=True, sr=True)
-            forms["fp16s_over_fp32"] = forms["f16s_6400x2400_les"]["GLUPS"] / forms["f32_6400x2400_les"]["GLUPS"]
-            # FP32 restricted to one step per launch (the autotuner may pick two)
-            forms["f32_6400x2400_les_one_step"] = bench_form(6400, 2400, 240, 0.644, "f32", big_steps, 20, torch,
-                                                             stream.cuda_stream, les=True, launch="256,1,1,0")
-            forms["fp16s_over_fp32_one_step"] = (forms["f16s_6400x2400_les"]["GLUPS"] /
-                                                 forms["f32_6400x2400_les_one_step"]["GLUPS"])
+            for form in ("f16s", "bf16s", "f16s_sr"):
+                forms[f"large_{form}"] = bench_large(geom_cache["geom"], form, 100, 10, torch, stream.cuda_stream)
+            forms["fp16s_over_fp32_large"] = forms["large_f16s"]["GLUPS"] / head["GLUPS"]
+            forms["fp16s_sr_over_fp32_large"] = forms["large_f16s_sr"]["GLUPS"] / head["GLUPS"]
+            geom_cache.clear()
+            for name in ("f32_6400x2400_les", "f16s_6400x2400_les", "f16s_sr_6400x2400_les", "bf16s_6400x2400_les"):
+                forms[name] = run_form(name, args, torch, stream.cuda_stream, geom_cache)
+                log(f"{name}: {forms[name]['GLUPS']:.1f} GLUPS")
+            forms["fp16s_over_fp32_6400x2400"] = forms["f16s_6400x2400_les"]["GLUPS"] / forms["f32_6400x2400_les"]["GLUPS"]
+            forms["f32_800x300_cache_resident"] = bench_form(800, 300, 30, 0.68, "f32", 2000, 100, torch,
+                                                             stream.cuda_stream)
             forms["batched_256"] = bench_ensemble(1000, 100, torch, stream.cuda_stream)
-            print("# batched ensemble done", file=sys.stderr, flush=True)
-            forms["thermal_f32_800x300"] = bench_thermal(800, 300, "f32", 2000, 100, torch, stream.cuda_stream)
+            log(f"batched ensemble: {forms['batched_256']['GLUPS']:.1f} GLUPS")
             forms["thermal_f32_6400x2400"] = bench_thermal(6400, 2400, "f32", 200, 20, torch, stream.cuda_stream)
-            print("# thermal done", file=sys.stderr, flush=True)
-            if args.large:
-                forms["large_32768x16384_les"] = bench_large(100, 10, torch, stream.cuda_stream)
+            log("thermal done")
 
-    cpu = None if args.no_cpu else cpu_baseline()
-    achieved = bytes_launch / (kms * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    two = head["steps_per_launch"] == 2
+    bytes_launch = head["bytes_per_step"] * head["steps_per_launch"]
+    achieved = bytes_launch / (head["launch_ms"] * 1e-3) / 1e9
     line = {
         "metric": "GLUPS + % HBM peak, D2Q9 MRT step: FP32 plain vs FP16-store vs batched",
-        "value": glups,
+        "value": head["GLUPS"],
         "unit": "GLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * t / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "cylinder wake 800x300, D2Q9 MRT, FP32 storage (f-w) and compute, "
-                        "r=30 at (200,150), Re=100 tau=0.68 u=0.1, Mei interpolated bounce-back, "
-                        "Zou/He inlet, outflow copy, periodic y, seed-42 perturbation, "
-                        f"hipGraph of {gs} steps",
-            "cells": info.cells, "fluid_cells": info.fluid_cells, "launch": list(shape),
-            "cd_last_step": cd_last,
+            "workload": "32768x16384 domain, D2Q9 MRT + Smagorinsky/Van Driest LES (c_s=0.12), tau=0.505, "
+                        "u=0.1, 128 seeded obstacles (64 circles r 20-60 with exact-circle Mei links, 64 plates "
+                        "155x2 at AoA in [-15,15] deg, staircase bounce-back), Zou/He inlet, outflow copy, "
+                        "periodic y, FP32 storage (f-w) and compute, uniform equilibrium start, "
+                        f"hipGraphs of {args.graph_steps} steps",
+            "cells": head["cells"], "fluid_cells": head["fluid_cells"], "launch": head["launch"],
+            "setup_s": head["setup_s"], "nan": head["nan"], "steps_run": head["steps_run"],
         },
         "roofline": {
             "bound": "hbm",
@@ -579,7 +512,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": profiled_traffic(),
-            "kernel_ms": kms,
+            "kernel": "step2_kernel<F32, tile rows>" if two else "step_kernel<F32, ...>",
+            "steps_per_launch": head["steps_per_launch"],
+            "launch_ms": head["launch_ms"],
             "bytes_per_launch": bytes_launch,
         },
         "cpu_baseline": cpu,

@@ -112,6 +112,16 @@ int lbm_desc_for_case(int case_type, int32_t nx, int32_t ny, lbm_desc* out);
 int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_exp,
                void* stream, lbm_handle* out);
 int lbm_destroy(lbm_handle h);
+/* Layout of the population buffers lbm_create would allocate for these lattices,
+ * computed without a device (memory planning): lattice l's nine planes of ny rows of
+ * pitch = nx rounded up to 64 values start at f_off[l] (n entries, may be null); each
+ * buffer holds *pool_elems values followed by a guard of *guard_elems (more than one
+ * row of the widest lattice: the step kernels form a node's pull addresses before they
+ * know whether it lies on an edge).  Bytes per buffer: (pool + guard) * bytes per
+ * population; a handle holds two, four with the thermal populations.  No counterpart in
+ * the reference, whose state is two std::vector<double> of nx*ny*9. */
+int lbm_population_layout(const lbm_desc* descs, int32_t n, int64_t* f_off, int64_t* pool_elems,
+                          int64_t* guard_elems);
 
 /* Geometry.  Set it before lbm_upload: a change afterwards invalidates the stored
  * state until the next upload.  lbm_set_obstacle takes nx*ny bytes (nonzero = solid), the

@@ -214,6 +214,51 @@ template <typename T> int dev_alloc(Ensemble* E, T** p, size_t count)
     return LBM_OK;
 }
 
+// Layout of the population buffers for lattices descs[0..n): each lattice's 9 planes of
+// ny rows of pitch (nx rounded up to 64) values from f_off[l], blocks aligned to 256
+// values; then a guard.  The step kernels form a node's nine pull addresses before they
+// know whether the node is on an edge, and the general path gathers sources one row
+// beyond the lattice on open edges: the furthest element any kernel forms for lattice l
+// is f_off + 9 ny pitch + nx - 2 (the south-east source of the last row's last node,
+// lbm_kernels.hip cell_update / preload), which lands in the next lattice's block or,
+// for the last lattice, in the guard of one row of the widest lattice plus 256 values.
+struct PoolLayout {
+    std::vector<int64_t> f_off;
+    int64_t pool = 0;   // values of every lattice's planes
+    int64_t guard = 0;  // values after them
+    int64_t elems() const { return pool + guard; }
+};
+
+PoolLayout pool_layout(const lbm_desc* descs, int n)
+{
+    PoolLayout P;
+    int64_t guard = 256;
+    for (int l = 0; l < n; ++l) {
+        const int64_t pitch = (descs[l].nx + 63) / 64 * 64;
+        const int64_t plane = static_cast<int64_t>(descs[l].ny) * pitch;
+        P.f_off.push_back(P.pool);
+        P.pool += (9 * plane + 255) / 256 * 256;
+        guard = std::max<int64_t>(guard, pitch + 256);
+    }
+    P.guard = guard;
+    return P;
+}
+
+// Every population buffer of a handle -- the two pools, the thermal populations' two,
+// the autotuner's copies -- is allocated here, with the guard.
+int alloc_population(Ensemble* E, void** p, const char* what)
+{
+    *p = nullptr;
+    const size_t bytes = static_cast<size_t>(E->pool_elems + E->guard_elems) * E->bpp;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LBM_ERR_NOMEM, std::string(what) + ": " + hipGetErrorString(e));
+    }
+    E->device_bytes += static_cast<int64_t>(bytes);
+    return LBM_OK;
+}
+
 // Host-to-device copy ordered on the handle's stream, complete on return.
 int h2d(Ensemble* E, void* dst, const void* src, size_t bytes)
 {
@@ -781,10 +826,13 @@ ThermalArgs thermal_args(Ensemble* E, int mode, int from_state, int y0, int y1, 
     return t;
 }
 
-// n <= acc_blocks steps over lattice l (l < 0: every lattice), then the force flush
-// and the counter update.  One-lattice launches take their buffers from the host's
-// parity; graph capture records them for that starting parity.
-int launch_batch(Ensemble* E, int n, int l)
+// n steps over lattice l (l < 0: every lattice), then the force flush and the counter
+// update.  The force partials of launch i go to block i of acc_base (fallback
+// accumulators, zero on entry and cleared again by the flush) and of nf_base (per-record
+// terms): the handle's blocks (n <= acc_blocks) unless the caller brings others.
+// One-lattice launches take their buffers from the host's parity; graph capture records
+// them for that starting parity.
+int launch_batch(Ensemble* E, int n, int l, double* acc_base = nullptr, double* nf_base = nullptr)
 {
     StepArgs a = base_args(E);
     int blocks;
@@ -793,8 +841,10 @@ int launch_batch(Ensemble* E, int n, int l)
     const int32_t par = E->lat[li].par;
     const bool two = !E->thermal && single && (E->options & LAUNCH_TWO_STEPS);
     const int pairs = two_step_pairs(E, n, l);
-    auto block_of = [&](int i) { return E->d_acc + static_cast<int64_t>(i) * kAccCopies * E->total_slots * 2; };
-    auto terms_of = [&](int i) { return E->d_nf + static_cast<int64_t>(i) * E->n_recs * 2; };
+    if (!acc_base) acc_base = E->d_acc;
+    if (!nf_base) nf_base = E->d_nf;
+    auto block_of = [&](int i) { return acc_base + static_cast<int64_t>(i) * kAccCopies * E->total_slots * 2; };
+    auto terms_of = [&](int i) { return nf_base + static_cast<int64_t>(i) * E->n_recs * 2; };
     int swaps = 0;
     if (E->thermal) {
         // one launch of the thermal kernel per step (one lattice; f and g share the parity)
@@ -854,8 +904,8 @@ int launch_batch(Ensemble* E, int n, int l)
         LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream));
     }
     FlushArgs f{};
-    f.acc = E->d_acc;
-    f.nf = E->d_nf;
+    f.acc = acc_base;
+    f.nf = nf_base;
     f.n_recs = E->n_recs;
     f.slot_start = E->d_slot_start;
     f.slot_recs = E->d_slot_recs;
@@ -1169,32 +1219,28 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
         }
         E->own_stream = true;
     }
-    int64_t pool = 0, cells = 0;
+    const lbm::PoolLayout P = lbm::pool_layout(descs, n);
+    int64_t cells = 0;
     E->lat.resize(n);
     for (int l = 0; l < n; ++l) {
         HostLattice& H = E->lat[l];
         H.d = descs[l];
         H.pitch = (descs[l].nx + 63) / 64 * 64;
-        H.f_off = pool;
+        H.f_off = P.f_off[l];
         const int64_t plane = static_cast<int64_t>(descs[l].ny) * H.pitch;
-        pool += (9 * plane + 255) / 256 * 256;
         H.cell_off = cells;
         cells += (plane + 255) / 256 * 256;
     }
-    E->pool_elems = pool;
+    E->pool_elems = P.pool;
     E->cell_elems = cells;
-    // the step kernel issues a node's pull loads before it knows whether the node is on
-    // an edge: the last plane of the last lattice may be read up to one row past its
-    // end, so the buffers carry a guard of more than one row
-    int64_t guard = 256;
-    for (auto& H : E->lat) guard = std::max<int64_t>(guard, H.pitch + 256);
-    E->guard_elems = guard;
+    E->guard_elems = P.guard;
     int rc = LBM_OK;
-    for (int b = 0; b < 2 && rc == LBM_OK; ++b) {
-        hipError_t e = hipMalloc(&E->buf[b], static_cast<size_t>(pool + guard) * E->bpp);
-        if (e != hipSuccess) rc = fail(LBM_ERR_NOMEM, "population buffers: " + std::string(hipGetErrorString(e)));
-        else E->device_bytes += (pool + guard) * E->bpp;
+    for (int l = 0; l < n && rc == LBM_OK; ++l) {
+        const HostLattice& H = E->lat[l];
+        const int64_t furthest = H.f_off + 9 * static_cast<int64_t>(H.d.ny) * H.pitch + H.d.nx - 2;
+        if (furthest >= P.elems()) rc = fail(LBM_ERR_INVALID, "population layout: pulls reach past the guard");
     }
+    for (int b = 0; b < 2 && rc == LBM_OK; ++b) rc = lbm::alloc_population(E, &E->buf[b], "population buffers");
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_flags, cells);
     if (rc == LBM_OK) rc = dev_alloc(E, &E->d_lslot, cells);
     // no link record until a geometry build writes a lattice's slots
@@ -1230,6 +1276,20 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
         return fail(rc, msg);
     }
     *out = E;
+    return LBM_OK;
+}
+
+int lbm_population_layout(const lbm_desc* descs, int32_t n, int64_t* f_off, int64_t* pool_elems,
+                          int64_t* guard_elems)
+{
+    if (!descs || n <= 0) return fail(LBM_ERR_INVALID, "lbm_population_layout: bad arguments");
+    for (int l = 0; l < n; ++l)
+        if (descs[l].nx < 3 || descs[l].ny < 3) return fail(LBM_ERR_INVALID, "lattice must be at least 3x3");
+    const lbm::PoolLayout P = lbm::pool_layout(descs, n);
+    if (f_off)
+        for (int l = 0; l < n; ++l) f_off[l] = P.f_off[l];
+    if (pool_elems) *pool_elems = P.pool;
+    if (guard_elems) *guard_elems = P.guard;
     return LBM_OK;
 }
 
@@ -1740,31 +1800,33 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     // reads).  The state, the accumulators and the history stay untouched.
     // each copy carries the population buffers' guard: the kernels read from it
     const size_t pool_bytes = static_cast<size_t>(E->pool_elems) * E->bpp;
-    const size_t alloc_bytes = static_cast<size_t>(E->pool_elems + E->guard_elems) * E->bpp;
-    void* scratch[2] = {nullptr, nullptr};
+    void* copies[2] = {nullptr, nullptr};
+    const int64_t held = E->device_bytes;
     for (int k = 0; k < 2; ++k)
-        if (hipMalloc(&scratch[k], alloc_bytes) != hipSuccess) {
-            if (scratch[0]) (void)hipFree(scratch[0]);
-            return fail(LBM_ERR_NOMEM, "autotune scratch buffers");
+        if ((rc = alloc_population(E, &copies[k], "autotune copies of the population buffers")) != LBM_OK) {
+            if (copies[0]) (void)hipFree(copies[0]);
+            E->device_bytes = held;
+            return rc;
         }
+    E->device_bytes = held;  // freed before returning
     for (int k = 0; k < 2 && rc == LBM_OK; ++k)
-        if (hipMemcpyAsync(scratch[k], E->buf[k], pool_bytes, hipMemcpyDeviceToDevice, E->stream) != hipSuccess)
+        if (hipMemcpyAsync(copies[k], E->buf[k], pool_bytes, hipMemcpyDeviceToDevice, E->stream) != hipSuccess)
             rc = fail(LBM_ERR_DEVICE, "autotune scratch copy");
     if (rc != LBM_OK) {
-        (void)hipFree(scratch[0]);
-        (void)hipFree(scratch[1]);
+        (void)hipFree(copies[0]);
+        (void)hipFree(copies[1]);
         return rc;
     }
-    // launch n of a candidate: one lattice reads scratch[(par + n) & 1] and writes the
+    // launch n of a candidate: one lattice reads copies[(par + n) & 1] and writes the
     // other; an ensemble takes both from the per-lattice parity flipped by step_arg
     const int par0 = E->lat[0].par;
     auto ping = [&](StepArgs& a, bool single, int n) {
         if (single) {
-            a.src = scratch[(par0 + n) & 1];
-            a.dst = scratch[(par0 + n + 1) & 1];
+            a.src = copies[(par0 + n) & 1];
+            a.dst = copies[(par0 + n + 1) & 1];
         } else {
-            a.buf0 = scratch[0];
-            a.buf1 = scratch[1];
+            a.buf0 = copies[0];
+            a.buf1 = copies[1];
             a.dst = nullptr;
             a.step_arg = n;
         }
@@ -1883,8 +1945,8 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(scratch[0]);
-    (void)hipFree(scratch[1]);
+    (void)hipFree(copies[0]);
+    (void)hipFree(copies[1]);
     const int* pick = rc == LBM_OK ? bs : saved;
     E->bx = pick[0];
     E->rows = pick[1];
@@ -1992,12 +2054,8 @@ int lbm_set_thermal(lbm_handle h, int32_t l, double omega_k, double beta, double
     if (first) {
         if (H.uploaded && H.steps > 0)
             return fail(LBM_ERR_INVALID, "turn the thermal populations on before stepping (or upload again)");
-        const int64_t guard = H.pitch + 256;
-        for (int b = 0; b < 2; ++b) {
-            hipError_t e = hipMalloc(&E->gbuf[b], static_cast<size_t>(E->pool_elems + guard) * E->bpp);
-            if (e != hipSuccess) return fail(LBM_ERR_NOMEM, "thermal buffers: " + std::string(hipGetErrorString(e)));
-            E->device_bytes += (E->pool_elems + guard) * E->bpp;
-        }
+        for (int b = 0; b < 2; ++b)
+            if ((rc = alloc_population(E, &E->gbuf[b], "thermal buffers")) != LBM_OK) return rc;
         if ((rc = dev_alloc(E, &E->d_tcode, E->cell_elems)) != LBM_OK) return rc;
         E->thermal = true;
     }

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -42,9 +43,23 @@ struct Config {
     uint32_t rounding_seed = 0;
 };
 
+// The storage form of unchanged drivers comes from the environment: LBM2D_STORAGE =
+// f64 (default), f32, f16s or bf16s; LBM2D_STOCHASTIC_ROUNDING=1 for the 16-bit forms.
 inline Config& config()
 {
-    static Config c;
+    static Config c = [] {
+        Config k;
+        if (const char* s = std::getenv("LBM2D_STORAGE")) {
+            const std::string v(s);
+            if (v == "f64") k.storage = LBM_STORE_F64;
+            else if (v == "f32") k.storage = LBM_STORE_F32;
+            else if (v == "f16s") k.storage = LBM_STORE_F16S;
+            else if (v == "bf16s") k.storage = LBM_STORE_BF16S;
+            else throw std::runtime_error("LBM2D_STORAGE must be f64, f32, f16s or bf16s, not " + v);
+        }
+        if (const char* s = std::getenv("LBM2D_STOCHASTIC_ROUNDING")) k.stochastic_rounding = std::string(s) == "1";
+        return k;
+    }();
     return c;
 }
 

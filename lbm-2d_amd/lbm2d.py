@@ -97,6 +97,8 @@ def lib() -> C.CDLL:
         "lbm_desc_for_case": (C.c_int, [C.c_int, i32, i32, C.POINTER(LbmDesc)]),
         "lbm_create": (C.c_int, [C.POINTER(LbmDesc), i32, i32, i32, C.c_void_p, C.POINTER(_h)]),
         "lbm_destroy": (C.c_int, [_h]),
+        "lbm_population_layout": (C.c_int, [C.POINTER(LbmDesc), i32, C.POINTER(i64), C.POINTER(i64),
+                                            C.POINTER(i64)]),
         "lbm_set_obstacle": (C.c_int, [_h, i32, _u8p]),
         "lbm_set_bb_circle": (C.c_int, [_h, i32, dbl, dbl, dbl, i32]),
         "lbm_set_bb_polygon": (C.c_int, [_h, i32, _dp, i32, i32]),
@@ -154,6 +156,16 @@ def desc_for_case(case: int, nx: int, ny: int, *, tau: float = 1.0, u_in: float 
     d.tau, d.u_in, d.les, d.cs = tau, u_in, int(les), cs
     d.collision, d.body_force_x = collision, body_force_x
     return d
+
+
+def population_layout(descs: Sequence[LbmDesc]):
+    """(f_off per lattice, pool values, guard values) of the population buffers a handle
+    of these lattices holds; no device needed."""
+    arr = (LbmDesc * len(descs))(*descs)
+    off = (C.c_int64 * len(descs))()
+    pool, guard = C.c_int64(), C.c_int64()
+    _check(lib().lbm_population_layout(arr, len(descs), off, C.byref(pool), C.byref(guard)))
+    return list(off), pool.value, guard.value
 
 
 class Ensemble:

@@ -142,15 +142,15 @@ def test_autotune_keeps_state():
     assert np.abs(ens.download(0) - o.f).max() <= TOL["f32"]
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16s"])
-def test_autotune_on_an_exactly_sized_pool(storage):
+@pytest.mark.parametrize("storage,nx,ny", [("f32", 1024, 512), ("f16s", 1024, 1024)])
+def test_autotune_on_an_exactly_sized_pool(storage, nx, ny):
     """The autotuner steps two copies of the population buffers; the step kernel's pulls
     of a lattice's last rows reach past its pool, so the copies carry the buffers' guard.
-    1024 x 512 FP32 populations fill exactly nine 2 MiB pages (no row padding, no slack
-    after the pool).  Tuning leaves the state alone, and stepping afterwards matches, bit
-    for bit, a handle that never tuned."""
-    nx, ny = 1024, 512
-    assert (9 * nx * ny * 4) % (2 << 20) == 0
+    The populations fill exactly nine 2 MiB pages (1024 x 512 FP32, 1024 x 1024 binary16:
+    no row padding, no slack after the pool).  Tuning leaves the state alone, and stepping
+    afterwards matches, bit for bit, a handle that never tuned."""
+    assert (9 * nx * ny * G.BYTES_PER_POP[storage]) % (2 << 20) == 0
+    assert G.population_layout([G.desc_for_case(G.CYLINDER, nx, ny)])[1] == 9 * nx * ny
     desc = G.desc_for_case(G.CYLINDER, nx, ny, tau=0.6, u_in=0.1)
     mask = G.circle_mask(nx, ny, 256, 256, 40)
     runs = []

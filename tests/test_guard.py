@@ -1,0 +1,132 @@
+"""Bounds of the step kernels' speculative pull addresses against the population buffers.
+
+The kernels form a node's nine pull addresses before they know whether the node lies
+on an edge (lbm_kernels.hip: cell_update, node_values of the two-step kernel, the
+thermal kernel's entry loads), the general path gathers sources one row off the
+lattice on open edges (preload), and the grouped bulk path moves aligned vectors of V
+nodes with the wave's outer lanes fetching one node further (group_update).  Every
+population buffer -- the two pools, the thermal populations' two, the autotuner's copies
+-- has the layout lbm_population_layout reports: planes of every lattice, then a guard.
+Here every address those access patterns can form, over every launch shape the library
+compiles (a superset: block 64..512, rows 1..8, 1..8 nodes per thread within 16 bytes
+per plane), is checked to lie inside [0, pool + guard), for exact-size pools (1024 x 512
+FP32 fills nine 2 MiB pages) and the 32768 x 16384 domain among others.  For a fixed
+direction and column an address grows with the row, so large lattices are enumerated
+on their first two and last two rows; small ones on every row.  No device is needed.
+"""
+import numpy as np
+import pytest
+
+import lbm2d as G
+
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+SHAPES = [(bx, rows, v) for bx in (64, 128, 256, 512) for rows in (1, 2, 4, 8) for v in (1, 2, 4, 8)]
+
+
+def _rows(ny):
+    return np.arange(ny) if ny <= 64 else np.array([0, 1, ny - 2, ny - 1])
+
+
+def _extremes(d, f_off, bpp):
+    """(min, max) element index over every address the kernels form for lattice d."""
+    nx, ny = d.nx, d.ny
+    pitch = (nx + 63) // 64 * 64
+    plane = ny * pitch
+    lo, hi = np.inf, -np.inf
+
+    def upd(a):
+        nonlocal lo, hi
+        lo = min(lo, int(a.min()))
+        hi = max(hi, int(a.max()))
+
+    ys = _rows(ny)
+    xs = np.arange(nx)
+    X, Y = np.meshgrid(xs, ys)
+    idx = Y * pitch + X
+    for j in range(9):
+        # cell_update / node_values / thermal entry: unwrapped pulls of every node; the
+        # outlet column pulls the copied node nx - 2's sources
+        shift = np.where((X == nx - 1) & bool(d.outlet), 1, 0)
+        upd(f_off + j * plane + idx - shift - (CY[j] * pitch + CX[j]))
+        upd(f_off + j * plane + idx)
+        # preload: sources wrapped on periodic edges only
+        sx, sy = X - CX[j], Y - CY[j]
+        if d.x_edge == G.EDGE_PERIODIC:
+            sx = np.where(sx < 0, sx + nx, np.where(sx >= nx, sx - nx, sx))
+        if d.y_edge == G.EDGE_PERIODIC:
+            sy = np.where(sy < 0, sy + ny, np.where(sy >= ny, sy - ny, sy))
+        upd(f_off + j * plane + sy * pitch + sx)
+    # group_update: interior rows, groups of V nodes at x = multiples of V below pitch
+    inner = ys[(ys >= 1) & (ys <= ny - 2)]
+    for bx, rows, v in SHAPES:
+        if v == 1 or bpp * v > 16 or len(inner) == 0:
+            continue
+        gx = np.arange(0, pitch, v)
+        GX, GY = np.meshgrid(gx, inner)
+        gidx = GY * pitch + GX
+        for j in range(9):
+            base = f_off + j * plane + gidx - CY[j] * pitch
+            upd(base)
+            upd(base + v - 1)
+            if CX[j] == 1:
+                upd(base - 1)   # lane 0's node before the wave
+            if CX[j] == -1:
+                upd(base + v)   # lane 63's node after the wave
+    return lo, hi
+
+
+def _descs(sizes, edges=(G.EDGE_OPEN, G.EDGE_PERIODIC), outlet=1):
+    out = []
+    for nx, ny in sizes:
+        d = G.desc_for_case(G.CYLINDER, nx, ny, tau=0.6, u_in=0.1)
+        d.x_edge, d.y_edge = edges
+        d.outlet = outlet
+        out.append(d)
+    return out
+
+
+CASES = {
+    "exact_1024x512": [(1024, 512)],
+    "large_32768x16384": [(32768, 16384)],
+    "tiny": [(3, 3)],
+    "thin_rows_past_nx": [(65, 3)],
+    "thin_200x3": [(200, 3)],
+    "ensemble_uneven": [(300, 100), (1024, 96), (257, 384), (700, 200), (64, 64)],
+}
+
+
+@pytest.mark.parametrize("storage,bpp", [("f64", 8), ("f32", 4), ("f16s", 2)])
+@pytest.mark.parametrize("edges", [(G.EDGE_OPEN, G.EDGE_PERIODIC), (G.EDGE_OPEN, G.EDGE_OPEN),
+                                   (G.EDGE_WALL, G.EDGE_WALL), (G.EDGE_PERIODIC, G.EDGE_PERIODIC)])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_pull_addresses_stay_inside_the_buffer(name, edges, storage, bpp):
+    descs = _descs(CASES[name], edges)
+    f_off, pool, guard = G.population_layout(descs)
+    assert guard >= max((d.nx + 63) // 64 * 64 for d in descs) + 256
+    for d, off in zip(descs, f_off):
+        lo, hi = _extremes(d, off, bpp)
+        assert lo >= 0, (name, d.nx, d.ny, lo)
+        assert hi < pool + guard, (name, d.nx, d.ny, hi, pool, guard)
+
+
+@pytest.mark.parametrize("name", ["exact_1024x512", "large_32768x16384"])
+def test_without_the_guard_the_last_lattice_reads_past_its_pool(name):
+    """The check above has teeth: an exact-size buffer (pool only, the autotuner's old
+    copies) is overrun by the last lattice's pulls."""
+    descs = _descs(CASES[name])
+    f_off, pool, _ = G.population_layout(descs)
+    _, hi = _extremes(descs[-1], f_off[-1], 4)
+    assert hi >= pool
+
+
+def test_layout_matches_the_stated_rule():
+    """f_off: blocks of 9 ny pitch values rounded up to 256, in lattice order."""
+    descs = _descs([(300, 100), (1024, 96), (3, 3)])
+    f_off, pool, guard = G.population_layout(descs)
+    sizes = [-(-9 * d.ny * ((d.nx + 63) // 64 * 64) // 256) * 256 for d in descs]
+    assert f_off == [0, sizes[0], sizes[0] + sizes[1]]
+    assert pool == sum(sizes)
+    assert guard == 1024 + 256
+    assert (9 * 1024 * 512 * 4) % (2 << 20) == 0  # the exact-size case fills whole pages
+    assert G.population_layout(_descs([(1024, 512)]))[1] == 9 * 1024 * 512
