@@ -199,6 +199,22 @@ int lbm_macros(lbm_handle h, int32_t lattice, int32_t y0, int32_t y1, double* rh
 int lbm_step(lbm_handle h, int32_t n);
 /* Advances one lattice by n steps with one launch per step and no graph. */
 int lbm_step_lattice(lbm_handle h, int32_t lattice, int32_t n);
+/* Allocation-free stepping.  No counterpart in the reference: for callers that cannot
+ * allocate memory while they step (code under hipGraph stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Advances every lattice by n steps as lbm_step
+ * does -- same kernels, same results bit for bit -- with one launch per step and no
+ * hipGraph, the launches' force partials in a fixed scratch array of
+ * LBM_NOALLOC_SCRATCH_BYTES in device memory that each GPU holds for the life of the
+ * process.  It calls no hipMalloc or hipFree, instantiates no graph and allocates no host
+ * memory.  LBM_ERR_INVALID when the handle has changes not yet on the device (call
+ * lbm_step(h, 0) once after changing the geometry, parameters or launch shape, or after
+ * an upload), when one launch's partials exceed the scratch (16 B per boundary-link node
+ * plus 256 B per force slot: about one million link nodes), or when the per-step force
+ * history (4096 steps per lattice) would overwrite steps not yet read with lbm_forces.
+ * Every call on a GPU shares that GPU's scratch and leaves it holding its last partials:
+ * calls that overlap on one GPU (from two host threads, or on two streams) race. */
+#define LBM_NOALLOC_SCRATCH_BYTES (16u << 20)
+int lbm_step_noalloc(lbm_handle h, int32_t n);
 /* Launch options of the step kernel (bit field). */
 #define LBM_LAUNCH_XCD_ORDER 1     /* consecutive tiles stay on one XCD (one L2) */
 #define LBM_LAUNCH_STREAM_STORES 2 /* nontemporal stores of the new populations */

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1135,6 +1136,40 @@ int thermal_ok(Ensemble* E, int l)
 }  // namespace
 }  // namespace lbm
 
+// Device scratch of lbm_step_noalloc: the force partials of one call's launches
+// (fallback accumulators of every launch, then the per-record terms of every launch).
+// Each GPU holds its own copy for the life of the process.  In host code the name
+// stands for no device address: host code uses only what scratch_on_device returns.
+static __device__ __align__(256) double scratch[LBM_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace lbm {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Address of the current device's scratch, looked up once per device (with that
+// device current) by hipGetSymbolAddress; each device's copy has its own address.
+int scratch_on_device(double** out)
+{
+    int dev = -1;
+    LBM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(LBM_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        LBM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(LBM_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return LBM_OK;
+}
+
+}  // namespace
+}  // namespace lbm
+
 using lbm::Ensemble;
 using lbm::fail;
 
@@ -1719,6 +1754,55 @@ int lbm_step(lbm_handle h, int32_t n)
         } else {
             if ((rc = launch_batch(E, batch, -1)) != LBM_OK) return rc;
         }
+        const int flips = batch_flips(E, batch, -1);
+        for (auto& H : E->lat) {
+            H.steps += batch;
+            H.par ^= flips & 1;
+        }
+        n -= batch;
+    }
+    return LBM_OK;
+}
+
+int lbm_step_noalloc(lbm_handle h, int32_t n)
+{
+    using namespace lbm;
+    if (!h) return fail(LBM_ERR_INVALID, "null handle");
+    if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+    Ensemble* E = h;
+    int rc;
+    if ((rc = require_uploaded(E)) != LBM_OK) return rc;
+    // everything commit() would write must be on the device already: it may allocate
+    if (E->geom_dirty || E->desc_dirty || E->counters_stale || (E->tiles_dirty && E->lat.size() > 1) ||
+        !E->d_slot_start || !E->d_hist)
+        return fail(LBM_ERR_INVALID, "lbm_step_noalloc: the handle has changes not yet on the device (geometry, "
+                                     "parameters, launch shape or an upload): call lbm_step(h, 0) first");
+    if (n == 0) return LBM_OK;
+    const int64_t acc_per = static_cast<int64_t>(kAccCopies) * E->total_slots * 2;
+    const int64_t nf_per = static_cast<int64_t>(E->n_recs) * 2;
+    const int64_t cap = static_cast<int64_t>(LBM_NOALLOC_SCRATCH_BYTES / sizeof(double));
+    if (acc_per + nf_per > cap)
+        return fail(LBM_ERR_INVALID, "lbm_step_noalloc: one launch's force partials (" +
+                                         std::to_string((acc_per + nf_per) * 8) + " bytes for " +
+                                         std::to_string(E->n_recs) + " link records) exceed the " +
+                                         std::to_string(LBM_NOALLOC_SCRATCH_BYTES) + "-byte scratch");
+    // the flush writes history rows steps .. steps + n - 1 (mod R); draining them to the
+    // host would allocate, so rows not yet read must not be overwritten
+    for (auto& H : E->lat)
+        if (H.steps + n - H.read_pos > E->R)
+            return fail(LBM_ERR_INVALID, "lbm_step_noalloc: the force history holds " + std::to_string(E->R) +
+                                             " steps per lattice: read it with lbm_forces first");
+    const int64_t chunk = std::min<int64_t>(n, cap / (acc_per + nf_per));
+    double* base = nullptr;
+    if ((rc = scratch_on_device(&base)) != LBM_OK) return rc;
+    double* acc_base = base;
+    double* nf_base = base + chunk * acc_per;
+    // the accumulators start at zero (every flush clears what it read, but an earlier
+    // call may have stopped between its launches and its flush)
+    LBM_HIP(launch_fill_zero(acc_base, chunk * acc_per, E->stream));
+    while (n > 0) {
+        const int batch = static_cast<int>(std::min<int64_t>(n, chunk));
+        if ((rc = launch_batch(E, batch, -1, acc_base, nf_base)) != LBM_OK) return rc;
         const int flips = batch_flips(E, batch, -1);
         for (auto& H : E->lat) {
             H.steps += batch;

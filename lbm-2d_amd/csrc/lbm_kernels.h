@@ -219,6 +219,8 @@ hipError_t launch_step2(int storage, int tile_rows, int n_blocks, const StepArgs
 int list_blocks(int block_x, int n_links, int n_edges);
 hipError_t launch_state(int storage, const StateArgs& a, int nx, hipStream_t s);
 hipError_t launch_prologue(int storage, const PrologueArgs& a, int nx, hipStream_t s);
+// p[0 .. n) = 0 on stream s.
+hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 // steps += inc, buffer parity ^= flips & 1 for lattices [begin, begin+count)
 hipError_t launch_add_steps(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
                             int32_t flips, hipStream_t s);

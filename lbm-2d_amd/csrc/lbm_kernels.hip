@@ -1857,6 +1857,14 @@ __global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, in
     }
 }
 
+// Zeroes n doubles (a fill kernel rather than hipMemsetAsync: callers may capture it).
+__global__ __launch_bounds__(256) void fill_zero_kernel(double* p, int64_t n)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * 256)
+        p[i] = 0.0;
+}
+
 // Moves the force totals of a batch of n launches to the history ring.  Block (i, s)
 // sums, for launch i and force slot s, the terms its link records stored (records of
 // the slot in record order, strided over the block, then a fixed LDS tree) and the
@@ -2131,6 +2139,15 @@ hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hi
     case ST_F32: hipLaunchKernelGGL(thermal_load_kernel<ST_F32>, grid, dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t want = (n + 255) / 256;
+    const int64_t blocks = want < 2048 ? want : 2048;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
         "lbm_macros": (C.c_int, [_h, i32, i32, i32, _dp, _dp, _dp]),
         "lbm_step": (C.c_int, [_h, i32]),
         "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
+        "lbm_step_noalloc": (C.c_int, [_h, i32]),
         "lbm_set_launch": (C.c_int, [_h, i32, i32, i32, i32, i32]),
         "lbm_autotune": (C.c_int, [_h, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                    C.POINTER(i32), C.POINTER(C.c_float)]),
@@ -327,6 +328,11 @@ class Ensemble:
 
     def step(self, n: int = 1):
         _check(lib().lbm_step(self._h, n))
+
+    def step_noalloc(self, n: int = 1):
+        """lbm_step without allocating: one launch per step, force partials in the
+        library's fixed device scratch (call step(0) after changes first)."""
+        _check(lib().lbm_step_noalloc(self._h, n))
 
     def step_lattice(self, l: int, n: int = 1):
         _check(lib().lbm_step_lattice(self._h, l, n))
