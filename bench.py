@@ -57,9 +57,13 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def cylinder_lattice(nx, ny, cx, cy, r, tau, u, storage, stream, les=False, cs=0.1, perturb=True):
+def cylinder_lattice(nx, ny, cx, cy, r, tau, u, storage, stream, les=False, cs=0.1, perturb=True, spec=False):
+    """spec: the configs' boundary wording (equilibrium inlet, free-slip top and bottom)
+    instead of the reference's (Zou/He inlet, periodic y)."""
     import lbm2d as G
     d = G.desc_for_case(G.CYLINDER, nx, ny, tau=tau, u_in=u, les=les, cs=cs)
+    if spec:
+        d.inlet, d.y_edge = G.INLET_EQUILIBRIUM, G.EDGE_FREE_SLIP
     ens = G.Ensemble([d], storage=storage, stream=stream)
     mask = G.circle_mask(nx, ny, cx, cy, r)
     ens.set_obstacle(0, mask)
@@ -131,9 +135,10 @@ def kernel_launch_ms(ens, torch, two_step, short=20, long=40, replays=1):
     return (per[long] - per[short]) / extra, per
 
 
-def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1, launch="", sr=False):
+def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False, cs=0.1, launch="", sr=False,
+               spec=False):
     ens, mask = cylinder_lattice(nx, ny, nx // 4, ny // 2, r, tau, 0.1, storage, stream,
-                                 les=les, cs=cs, perturb=False)
+                                 les=les, cs=cs, perturb=False, spec=spec)
     if sr:
         ens.set_stochastic_rounding(True, 1)
     if launch:
@@ -147,6 +152,7 @@ def bench_form(nx, ny, r, tau, storage, steps, warmup, torch, stream, les=False,
     b = algorithmic_bytes(info)
     out = {
         "lattice": f"{nx}x{ny}", "storage": storage, "les": les, "rounding": "stochastic" if sr else "nearest-even",
+        "boundaries": "equilibrium inlet, free-slip y" if spec else "Zou/He inlet, periodic y",
         "GLUPS": info.cells * steps / t / 1e9,
         "ms_per_step": 1e3 * t / steps,
         "kernel_ms_per_step": kms,
@@ -474,6 +480,8 @@ def main():
             forms["fp16s_over_fp32_6400x2400"] = forms["f16s_6400x2400_les"]["GLUPS"] / forms["f32_6400x2400_les"]["GLUPS"]
             forms["f32_800x300_cache_resident"] = bench_form(800, 300, 30, 0.68, "f32", 2000, 100, torch,
                                                              stream.cuda_stream)
+            forms["f32_800x300_spec_boundaries"] = bench_form(800, 300, 30, 0.68, "f32", 2000, 100, torch,
+                                                              stream.cuda_stream, spec=True)
             forms["batched_256"] = bench_ensemble(1000, 100, torch, stream.cuda_stream)
             log(f"batched ensemble: {forms['batched_256']['GLUPS']:.1f} GLUPS")
             forms["thermal_f32_6400x2400"] = bench_thermal(6400, 2400, "f32", 200, 20, torch, stream.cuda_stream)

@@ -44,12 +44,19 @@ enum lbm_collision { LBM_MRT = 0, LBM_BGK = 1 };
 
 /* Treatment of each lattice edge (src/lbm.hpp:408-484): populations leaving through
  * an OPEN edge are dropped (zero) and an inlet/outlet may overwrite them; a WALL
- * edge bounces them back into the same node; PERIODIC wraps. */
-enum lbm_edge { LBM_EDGE_PERIODIC = 0, LBM_EDGE_OPEN = 1, LBM_EDGE_WALL = 2 };
+ * edge bounces them back into the same node; PERIODIC wraps.  FREE_SLIP (y edges of a
+ * lattice with an open x edge; no counterpart in the reference, which wraps y): a
+ * population leaving through the edge comes back specularly, its y velocity reversed,
+ * into the node beside its emitter (the method of images: dropped beyond the open x
+ * edge, bounced back into its emitter where that node is solid; the momentum exchange
+ * counts boundary links inside the lattice only); such lattices step one step per launch
+ * and carry no thermal populations. */
+enum lbm_edge { LBM_EDGE_PERIODIC = 0, LBM_EDGE_OPEN = 1, LBM_EDGE_WALL = 2, LBM_EDGE_FREE_SLIP = 3 };
 
-/* Velocity inlet at x = 0: Zou/He on every row (src/lbm.hpp:21-34) or the parabolic
- * profile on rows y >= inlet_y0 of fluid nodes (src/lbm.hpp:40-58). */
-enum lbm_inlet { LBM_INLET_NONE = 0, LBM_INLET_ZOU_HE = 1, LBM_INLET_PARABOLIC = 2 };
+/* Velocity inlet at x = 0: Zou/He on every row (src/lbm.hpp:21-34), the parabolic
+ * profile on rows y >= inlet_y0 of fluid nodes (src/lbm.hpp:40-58), or EQUILIBRIUM: every
+ * fluid node of x = 0 becomes feq(1, u_in, 0) (no counterpart in the reference). */
+enum lbm_inlet { LBM_INLET_NONE = 0, LBM_INLET_ZOU_HE = 1, LBM_INLET_PARABOLIC = 2, LBM_INLET_EQUILIBRIUM = 3 };
 
 /* Boundary-link schemes (src/lbm.hpp:226-288). */
 enum lbm_bb { LBM_BB_SIMPLE = 0, LBM_BB_MEI = 1, LBM_BB_BOUZIDI = 2 };
@@ -197,6 +204,13 @@ int lbm_macros(lbm_handle h, int32_t lattice, int32_t y0, int32_t y1, double* rh
 /* Advances every lattice by n steps: one launch per step covering all lattices,
  * replayed from a hipGraph of graph_steps launches when n allows. */
 int lbm_step(lbm_handle h, int32_t n);
+/* Debugging aid for race and coverage checks (no counterpart in the reference): fills
+ * lattice l's planes of the population buffer the next step launch writes (and of the
+ * thermal populations') with all-one bytes, a NaN in every storage form.  A launch that
+ * writes every slot it owns leaves the results unchanged; a slot it should write but
+ * does not is then read back as NaN by the following launch.  The state lbm_download
+ * would return before the next step is lost. */
+int lbm_debug_poison(lbm_handle h, int32_t lattice);
 /* Advances one lattice by n steps with one launch per step and no graph. */
 int lbm_step_lattice(lbm_handle h, int32_t lattice, int32_t n);
 /* Allocation-free stepping.  No counterpart in the reference: for callers that cannot

@@ -173,6 +173,9 @@ struct Ensemble {
     int total_slots = 0;
     int R = kHistoryRows;
     int bx = 256, rows = 1, vec = 1, options = 0, graph_steps = 100;
+    // the one-step launch that ends a batch of two-step launches: the autotuner's fastest
+    // one-step shape, else (64, 1, 1) with the two-step launch's order and store options
+    int tail[4] = {0, 0, 0, 0};
     int sr = 0;                    // stochastic rounding (16-bit forms)
     uint32_t sr_seed = 0;
     bool geom_dirty = true, desc_dirty = true, tiles_dirty = true, counters_stale = false;
@@ -452,10 +455,11 @@ int build_geometry(Ensemble* E, HostLattice& H)
     H.special.clear();
     // inlet and outlet nodes the step kernel's bulk tiles take: off the corners, fluid,
     // without links; at the outlet the copied node (nx-2) as well.  Their state is the
-    // plain pulls plus the inlet closure (the closure overwrites exactly the slots whose
-    // sources lie off the lattice), or the plain pulls of the copied node.
+    // plain pulls plus the inlet closure (Zou/He overwrites exactly the slots whose
+    // sources lie off the lattice, the equilibrium inlet every slot), or the plain pulls
+    // of the copied node.
     if (d.x_edge == LBM_EDGE_OPEN && nx >= 4) {
-        const int inlet_from = d.inlet == LBM_INLET_ZOU_HE       ? 1
+        const int inlet_from = (d.inlet == LBM_INLET_ZOU_HE || d.inlet == LBM_INLET_EQUILIBRIUM) ? 1
                                : d.inlet == LBM_INLET_PARABOLIC ? std::max(1, d.inlet_y0)
                                                                 : ny;
         LBM_HIP(launch_geom_columns(H.d_flags_new, nx, ny, H.pitch, inlet_from, d.outlet, E->stream));
@@ -802,6 +806,8 @@ int two_step_ok(Ensemble* E, int bx, int rows, int vec)
         return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1) and (64, 8, 1)");
     const lbm_desc& d = E->lat[0].d;
     if (d.outlet && d.nx % 64 == 1) return fail(LBM_ERR_INVALID, "two steps per launch: nx % 64 == 1 with an outlet");
+    // the second step gathers from LDS; a free-slip edge's reflection reads the buffer
+    if (d.y_edge == LBM_EDGE_FREE_SLIP) return fail(LBM_ERR_INVALID, "two steps per launch: free-slip y edges");
     return LBM_OK;
 }
 
@@ -881,8 +887,10 @@ int launch_batch(Ensemble* E, int n, int l, double* acc_base = nullptr, double* 
         }
         StepArgs a1 = base_args(E);
         int blocks1;
-        grid_for(E, l, a1, blocks1, 64, 1, 1);
-        a1.options = E->options & (LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES);
+        const bool tuned = E->tail[0] != 0;
+        const int tbx = tuned ? E->tail[0] : 64, trows = tuned ? E->tail[1] : 1, tvec = tuned ? E->tail[2] : 1;
+        grid_for(E, l, a1, blocks1, tbx, trows, tvec);
+        a1.options = (tuned ? E->tail[3] : E->options) & (LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES);
         for (int i = 2 * pairs; i < n; ++i, ++swaps) {
             const int sp = par ^ (swaps & 1);
             a1.src = E->buf[sp];
@@ -890,7 +898,7 @@ int launch_batch(Ensemble* E, int n, int l, double* acc_base = nullptr, double* 
             a1.acc = block_of(i);
             a1.nf = terms_of(i);
             a1.step_arg = i;
-            LBM_HIP(launch_step(E->storage, 64, 1, 1, true, blocks1, a1, E->stream));
+            LBM_HIP(launch_step(E->storage, tbx, trows, tvec, true, blocks1, a1, E->stream));
         }
     }
     for (int i = 0; i < n && !two && !E->thermal; ++i, ++swaps) {
@@ -1232,10 +1240,12 @@ int lbm_create(const lbm_desc* descs, int32_t n, int32_t storage, int32_t scale_
         // node indices y * pitch + x (pitch: nx rounded up to 64) are int32
         if (static_cast<int64_t>((d.nx + 63) / 64 * 64) * d.ny > (1ll << 31))
             return fail(LBM_ERR_INVALID, "lattice too large: its rows padded to 64 nodes exceed 2^31 nodes");
-        if (d.collision < 0 || d.collision > 1 || d.x_edge < 0 || d.x_edge > 2 || d.y_edge < 0 || d.y_edge > 2 ||
-            d.inlet < 0 || d.inlet > 2)
+        if (d.collision < 0 || d.collision > 1 || d.x_edge < 0 || d.x_edge > 2 || d.y_edge < 0 || d.y_edge > 3 ||
+            d.inlet < 0 || d.inlet > 3)
             return fail(LBM_ERR_INVALID, "bad enum in lbm_desc");
         if (d.tau <= 0.0) return fail(LBM_ERR_INVALID, "tau must be positive");
+        if (d.y_edge == LBM_EDGE_FREE_SLIP && d.x_edge != LBM_EDGE_OPEN)
+            return fail(LBM_ERR_INVALID, "free-slip y edges need an open x edge");
         if (d.inlet == LBM_INLET_PARABOLIC && d.inlet_h <= 0)
             return fail(LBM_ERR_INVALID, "parabolic inlet needs inlet_h > 0");
     }
@@ -1813,6 +1823,23 @@ int lbm_step_noalloc(lbm_handle h, int32_t n)
     return LBM_OK;
 }
 
+int lbm_debug_poison(lbm_handle h, int32_t l)
+{
+    using namespace lbm;
+    int rc = lattice_ok(h, l);
+    if (rc) return rc;
+    Ensemble* E = h;
+    HostLattice& H = E->lat[l];
+    if (!H.uploaded) return fail(LBM_ERR_INVALID, "lattice has no state");
+    // the next launch over this lattice reads buf[par] and writes buf[1 - par]
+    const size_t off = static_cast<size_t>(H.f_off) * E->bpp;
+    const size_t bytes = static_cast<size_t>(9) * H.d.ny * H.pitch * E->bpp;
+    LBM_HIP(hipMemsetAsync(static_cast<char*>(E->buf[1 - H.par]) + off, 0xff, bytes, E->stream));
+    if (E->thermal) LBM_HIP(hipMemsetAsync(static_cast<char*>(E->gbuf[1 - H.par]) + off, 0xff, bytes, E->stream));
+    LBM_HIP(hipStreamSynchronize(E->stream));
+    return LBM_OK;
+}
+
 int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
 {
     using namespace lbm;
@@ -1854,6 +1881,7 @@ int lbm_set_launch(lbm_handle h, int32_t bx, int32_t rows, int32_t nodes_per_thr
     if (graph_steps < 0 || (graph_steps > 0 && (graph_steps % 2 != 0 || graph_steps > 2000)))
         return fail(LBM_ERR_INVALID, "graph_steps must be even and at most 2000");
     if (nbx != E->bx || nrows != E->rows || nv != E->vec || nopt != E->options) {
+        for (int k = 0; k < 4; ++k) E->tail[k] = 0;
         E->bx = nbx;
         E->rows = nrows;
         E->vec = nv;
@@ -1923,6 +1951,8 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     float best = 1e30f;
     const int saved[4] = {E->bx, E->rows, E->vec, E->options};
     int bs[4] = {E->bx, E->rows, E->vec, E->options};
+    int bs1[4] = {64, 1, 1, 0};  // fastest one-step shape (the tail of two-step batches)
+    float best1 = 1e30f;
     rc = LBM_OK;
     for (int si = 0; si < 4 * n_shapes; ++si) {
         E->bx = shapes[si % n_shapes][0];
@@ -1965,6 +1995,13 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         if (!ok) {
             rc = fail(LBM_ERR_DEVICE, "autotune launch failed");
             break;
+        }
+        if (ms < best1) {
+            best1 = ms;
+            bs1[0] = E->bx;
+            bs1[1] = E->rows;
+            bs1[2] = E->vec;
+            bs1[3] = E->options;
         }
         if (ms < best) {
             best = ms;
@@ -2036,6 +2073,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
     E->rows = pick[1];
     E->vec = pick[2];
     E->options = pick[3];
+    for (int k = 0; k < 4; ++k) E->tail[k] = rc == LBM_OK && (bs[3] & LAUNCH_TWO_STEPS) ? bs1[k] : 0;
     E->tiles_dirty = true;
     E->graph_dirty = true;
     if (rc != LBM_OK) return rc;
@@ -2129,6 +2167,8 @@ int lbm_set_thermal(lbm_handle h, int32_t l, double omega_k, double beta, double
     if (rc) return rc;
     Ensemble* E = h;
     if (E->lat.size() != 1) return fail(LBM_ERR_INVALID, "the thermal populations need a handle of one lattice");
+    if (E->lat[0].d.y_edge == LBM_EDGE_FREE_SLIP)
+        return fail(LBM_ERR_INVALID, "the thermal populations stream periodic in y: no free-slip y edges");
     if (E->storage != ST_F64 && E->storage != ST_F32)
         return fail(LBM_ERR_INVALID, "the thermal populations need F64 or F32 storage");
     if (!std::isfinite(omega_k) || !std::isfinite(beta) || !std::isfinite(g_buoyancy) || !std::isfinite(t_ref))

@@ -38,8 +38,11 @@ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 constexpr double WD[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
                           1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 
-enum { EDGE_PERIODIC = 0, EDGE_OPEN = 1, EDGE_WALL = 2 };
-enum { INLET_NONE = 0, INLET_ZOU_HE = 1, INLET_PARABOLIC = 2 };
+enum { EDGE_PERIODIC = 0, EDGE_OPEN = 1, EDGE_WALL = 2, EDGE_FREE_SLIP = 3 };
+enum { INLET_NONE = 0, INLET_ZOU_HE = 1, INLET_PARABOLIC = 2, INLET_EQUILIBRIUM = 3 };
+// direction with the same x velocity and the opposite y velocity (specular reflection
+// off a free-slip y edge)
+constexpr int MIRY[9] = {0, 1, 4, 3, 2, 8, 7, 6, 5};
 enum { BB_SIMPLE = 0, BB_MEI = 1, BB_BOUZIDI = 2 };
 
 // ------------------------------------------------------------------ storage forms
@@ -453,6 +456,22 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
             continue;
         }
         const int sx = px - CX[j], sy = py - CY[j];
+        if ((sy < 0 || sy >= L.ny) && L.y_edge == EDGE_FREE_SLIP) {
+            // specular reflection: the population of direction mirror(j) that node
+            // (sx, py) emitted towards the edge comes back into slot j.  A solid node
+            // there emits nothing, and this node's own population of direction i = opp(j)
+            // meets that solid's mirror image: bounced back; beyond the open x edge it is
+            // dropped (zero)
+            if (sx < 0 || sx >= L.nx) {
+                d[j] = -wgt<R>(j);
+            } else {
+                const int64_t e = static_cast<int64_t>(py) * L.pitch + sx;
+                d[j] = (c.flags[L.cell_off + e] & CELL_SOLID)
+                           ? own[i]
+                           : Form<ST>::dec(c.base[MIRY[j] * L.plane + e], c.inv_scale, L.shift[MIRY[j]]);
+            }
+            continue;
+        }
         bool wall = false, open = false;
         if ((sx < 0 || sx >= L.nx) && L.x_edge != EDGE_PERIODIC) (L.x_edge == EDGE_WALL ? wall : open) = true;
         if ((sy < 0 || sy >= L.ny) && L.y_edge != EDGE_PERIODIC) (L.y_edge == EDGE_WALL ? wall : open) = true;
@@ -491,7 +510,16 @@ template <int ST> __device__ __forceinline__ int gathered_x(const LatticeDev& L,
 // parabolic profile on rows y >= inlet_y0 (src/lbm.hpp:21-58).
 template <typename R> __device__ __forceinline__ void inlet_closure(const LatticeDev& L, int y, R d[9])
 {
-    if (L.inlet == INLET_ZOU_HE) {
+#pragma clang fp contract(off)
+    if (L.inlet == INLET_EQUILIBRIUM) {
+        // the node becomes feq(1, u_in, 0), stored as its deviation from w
+        const R u = static_cast<R>(L.u_in);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const R eu = R(CX[i]) * u;
+            d[i] = wgt<R>(i) * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * u * u);
+        }
+    } else if (L.inlet == INLET_ZOU_HE) {
         zou_he<R>(d, static_cast<R>(L.u_in));
     } else if (y >= L.inlet_y0) {
         const double yy = static_cast<double>(y - L.inlet_y0);
@@ -505,6 +533,9 @@ __device__ __forceinline__ void state_general(const Ctx<ST>& c, int x, int y, bo
                                               const LinkVals<typename Form<ST>::real>& lk,
                                               typename Form<ST>::real d[9])
 {
+    // unfused, as gather_general: the lid's equilibrium rounds the same way in every
+    // kernel that inlines this
+#pragma clang fp contract(off)
     using R = typename Form<ST>::real;
     const LatticeDev& L = c.L;
     const int gx = gathered_x<ST>(L, x);
@@ -581,6 +612,7 @@ __device__ __forceinline__ void node_forces(const LatticeDev& L, const Ctx<ST>& 
                                             const LinkVals<typename Form<ST>::real>& ls_,
                                             const typename Form<ST>::real d[9])
 {
+#pragma clang fp contract(off)
     int ps = -1;
     double fx = 0.0, fy = 0.0;
 #pragma unroll

@@ -20,8 +20,8 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "lbm_gpu.h")
 STORAGE = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
 BYTES_PER_POP = {"f64": 8, "f32": 4, "f16s": 2, "bf16s": 2}
 MRT, BGK = 0, 1
-EDGE_PERIODIC, EDGE_OPEN, EDGE_WALL = 0, 1, 2
-INLET_NONE, INLET_ZOU_HE, INLET_PARABOLIC = 0, 1, 2
+EDGE_PERIODIC, EDGE_OPEN, EDGE_WALL, EDGE_FREE_SLIP = 0, 1, 2, 3
+INLET_NONE, INLET_ZOU_HE, INLET_PARABOLIC, INLET_EQUILIBRIUM = 0, 1, 2, 3
 BB_SIMPLE, BB_MEI, BB_BOUZIDI = 0, 1, 2
 
 # CaseType (src/lbm_types.hpp:20-23)
@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
         "lbm_macros": (C.c_int, [_h, i32, i32, i32, _dp, _dp, _dp]),
         "lbm_step": (C.c_int, [_h, i32]),
         "lbm_step_lattice": (C.c_int, [_h, i32, i32]),
+        "lbm_debug_poison": (C.c_int, [_h, i32]),
         "lbm_step_noalloc": (C.c_int, [_h, i32]),
         "lbm_set_launch": (C.c_int, [_h, i32, i32, i32, i32, i32]),
         "lbm_autotune": (C.c_int, [_h, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -333,6 +334,10 @@ class Ensemble:
         """lbm_step without allocating: one launch per step, force partials in the
         library's fixed device scratch (call step(0) after changes first)."""
         _check(lib().lbm_step_noalloc(self._h, n))
+
+    def debug_poison(self, l: int = 0):
+        """NaN into lattice l's planes of the buffer the next launch writes."""
+        _check(lib().lbm_debug_poison(self._h, l))
 
     def step_lattice(self, l: int, n: int = 1):
         _check(lib().lbm_step_lattice(self._h, l, n))

@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liblbm_oracle.so")
+# LBM_ORACLE_SO selects another build of the oracle (the sanitizer build of the tests).
+ORACLE_SO = os.environ.get("LBM_ORACLE_SO") or os.path.join(HERE, "liblbm_oracle.so")
 REF_CHECK_SO = os.path.join(HERE, "_ref", "libref_check.so")
 REF_FAST_SO = os.path.join(HERE, "_ref", "libref_fast.so")
 
@@ -80,6 +81,7 @@ def lib() -> C.CDLL:
         L.orc_set_mei.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_moving_wall.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_set_faithful.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_spec.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_place_walls.argtypes = [C.c_void_p]
         L.orc_place_cylinder.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_place_polygon.argtypes = [C.c_void_p, _dp, C.c_int]
@@ -238,6 +240,11 @@ class Oracle:
 
     def set_moving_wall(self, omega, rcx, rcy, on=True):
         lib().orc_set_moving_wall(self._h, int(on), omega, rcx, rcy)
+
+    def set_spec(self, inlet_equilibrium: bool = True, free_slip: bool = True):
+        """The configs' boundaries instead of the reference's (cylinder-type cases):
+        equilibrium inlet feq(1, u_in, 0) and free-slip y edges."""
+        lib().orc_set_spec(self._h, int(inlet_equilibrium), int(free_slip))
 
     def set_les(self, on: bool, cs: float):
         lib().orc_set_les(self._h, int(on), cs)

@@ -49,6 +49,8 @@ static const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 static const double W[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
                             1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 static const int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+/* same x velocity, opposite y velocity: specular reflection off a y edge */
+static const int MIRY[9] = {0, 1, 4, 3, 2, 8, 7, 6, 5};
 
 struct orc_lattice {
     int nx, ny;
@@ -59,6 +61,10 @@ struct orc_lattice {
     double body_force_x;
     int faithful;
     int wall_dist_valid;
+    /* the configs' wording where the reference differs (cylinder-type cases only):
+     * equilibrium inlet feq(1, u_in, 0) instead of Zou/He, free-slip y edges instead of
+     * periodic y.  No counterpart in the reference; pinned by invariants. */
+    int spec_inlet_eq, spec_free_slip;
     /* single interpolated-BB geometry (BounceBackGeometry, src/lbm_types.hpp:90-182) */
     double g_cx, g_cy, g_radius;
     double* poly;
@@ -351,6 +357,11 @@ void orc_set_les(orc_lattice* L, int on, double cs) { L->use_les = on; L->cs = c
 void orc_set_body_force(orc_lattice* L, double fx) { L->body_force_x = fx; }
 void orc_set_mei(orc_lattice* L, int on) { L->use_mei_bb = on; }
 void orc_set_faithful(orc_lattice* L, int on) { L->faithful = on; }
+void orc_set_spec(orc_lattice* L, int inlet_eq, int free_slip)
+{
+    L->spec_inlet_eq = inlet_eq;
+    L->spec_free_slip = free_slip;
+}
 void orc_obstacle_changed(orc_lattice* L) { L->wall_dist_valid = 0; }
 
 void orc_set_moving_wall(orc_lattice* L, int on, double omega, double rcx, double rcy)
@@ -680,6 +691,17 @@ void orc_step(orc_lattice* L, double tau, double u_in)
                     else fn[t * 9 + i] = fc[i];
                     continue;
                 } else {
+                    if (L->spec_free_slip && (ty < 0 || ty >= ny)) {
+                        /* free slip: the population returns into the node beside its
+                         * emitter along x, y velocity reversed (the emitter's mirror
+                         * image); dropped beyond the open x edge; bounced back into the
+                         * emitter when that node is solid (it met the solid's image) */
+                        if (tx < 0 || tx >= nx) continue;
+                        size_t t = (size_t)y * nx + tx;
+                        if (obs[t]) fn[c * 9 + OPP[i]] = fc[i];
+                        else fn[t * 9 + MIRY[i]] = fc[i];
+                        continue;
+                    }
                     /* src/lbm.hpp:457-484: y periodic, x-outgoing dropped */
                     if (ty < 0) ty += ny;
                     if (ty >= ny) ty -= ny;
@@ -748,6 +770,13 @@ void orc_step(orc_lattice* L, double tau, double u_in)
                 double u_loc = u_in * 4.0 * yy * (h - yy) / (h * h);
                 zou_he_west(&f[c * 9], u_loc);
             }
+        } else if (L->spec_inlet_eq) {
+            /* equilibrium inlet: every fluid node of x = 0 becomes feq(1, u_in, 0) */
+            for (int y = 0; y < ny; ++y) {
+                size_t c = (size_t)y * nx;
+                if (obs[c]) continue;
+                for (int i = 0; i < 9; ++i) f[c * 9 + i] = orc_equilibrium(i, 1.0, u_in, 0.0);
+            }
         } else {
             for (int y = 0; y < ny; ++y) zou_he_west(&f[(size_t)y * nx * 9], u_in);
         }
@@ -769,6 +798,7 @@ void orc_step(orc_lattice* L, double tau, double u_in)
                 if (obs[c]) continue;
                 for (int i = 0; i < 9; ++i) {
                     int tx = x + CX[i], ty = y + CY[i];
+                    if (L->spec_free_slip && (ty < 0 || ty >= ny)) continue;
                     if (ty < 0) ty += ny;
                     if (ty >= ny) ty -= ny;
                     if (is_periodic_x_case(cse)) {

@@ -70,6 +70,9 @@ void orc_set_moving_wall(orc_lattice* L, int on, double omega, double rot_cx, do
 /* faithful=1: per-step wall-distance relaxation sweep and serial memset/forces, as in
  * the reference; faithful=0: wall distance by BFS once per geometry (same integers). */
 void orc_set_faithful(orc_lattice* L, int faithful);
+/* The configs' boundary wording (cylinder-type cases): equilibrium inlet feq(1, u_in, 0)
+ * and free-slip (specular) y edges; no counterpart in the reference. */
+void orc_set_spec(orc_lattice* L, int inlet_eq, int free_slip);
 
 /* Obstacle placement (src/lbm.hpp:79-87, 931-964). */
 void orc_place_walls(orc_lattice* L);

@@ -131,8 +131,34 @@ CASES = {
 }
 
 
+def _cyl_and_edge_block(cx, cy, r):
+    """A circle, plus a block on the top edge row (the free-slip reflection's solid
+    emitters)."""
+    def f(s):
+        s.place_cylinder(cx, cy, r)
+        m = s.obstacle.copy()
+        m[-5:, 60:64] = 1
+        s.set_obstacle(m)
+    return f
+
+
+# The configs' boundary wording (equilibrium inlet, free-slip y edges): no reference
+# counterpart, so no reference fixtures; the oracle's option is pinned by invariants
+# (tests/test_oracle_spec_bcs.py).
+SPEC_CASES = {
+    "spec_cylinder": Case("spec_cylinder", 96, 48, O.CYLINDER, 0.56, 0.1, 100, _cyl_and_edge_block(24, 25, 6),
+                          pert=(29, 84), gpu_geom=(("circle", 24, 25, 6, G.BB_MEI),)),
+    "spec_les": Case("spec_les", 96, 48, O.CYLINDER, 0.52, 0.1, 60, _cyl(24, 25, 6), pert=(29, 84),
+                     les=True, cs=0.1, gpu_geom=(("circle", 24, 25, 6, G.BB_MEI),)),
+    "spec_bouzidi": Case("spec_bouzidi", 96, 48, O.ROTATING_CYLINDER, 0.6, 0.1, 60, _rot(24, 24, 6, 0.01, mei=False),
+                         gpu_geom=(("circle", 24, 24, 6, G.BB_BOUZIDI), ("moving", 0.01, 24, 24))),
+}
+
+
 def build_oracle(c: Case, cls=O.Oracle):
     o = cls(c.nx, c.ny, c.case, collision=c.collision, les=c.les, cs=c.cs)
+    if c.name.startswith("spec_"):
+        o.set_spec(inlet_equilibrium=True, free_slip=True)
     c.setup(o)
     if c.body_force:
         o.set_body_force(c.body_force)
@@ -143,8 +169,11 @@ def build_oracle(c: Case, cls=O.Oracle):
 
 
 def gpu_desc(c: Case):
-    return G.desc_for_case(c.case, c.nx, c.ny, tau=c.tau, u_in=c.u, les=c.les, cs=c.cs,
-                           collision=c.collision, body_force_x=c.body_force)
+    d = G.desc_for_case(c.case, c.nx, c.ny, tau=c.tau, u_in=c.u, les=c.les, cs=c.cs,
+                        collision=c.collision, body_force_x=c.body_force)
+    if c.name.startswith("spec_"):
+        d.inlet, d.y_edge = G.INLET_EQUILIBRIUM, G.EDGE_FREE_SLIP
+    return d
 
 
 def apply_gpu_geometry(ens: G.Ensemble, l: int, c: Case, obstacle: np.ndarray):
