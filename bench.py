@@ -204,13 +204,18 @@ def bench_thermal(nx, ny, storage, steps, warmup, torch, stream, launch=""):
     return out
 
 
-def bench_ensemble(steps, warmup, torch, stream, n=256):
+def bench_ensemble(steps, warmup, torch, stream, n=256, launch=""):
     """The seeded ensemble (workloads.ensemble_descs): one launch per step over every
-    lattice, hipGraphs of 100 steps, against the same lattices stepped one launch each."""
+    lattice (or one per two steps), hipGraphs of 100 steps, against the same lattices
+    stepped one launch each.  launch: bx,rows,nodes,options instead of the autotuner's."""
     import workloads as W
     lats = W.ensemble_descs(n=n)
     ens = W.build_ensemble(lats, "f32", stream)
-    shape, _ = ens.autotune()
+    if launch:
+        shape = tuple(int(v) for v in launch.split(","))
+        ens.set_launch(*shape)
+    else:
+        shape, _ = ens.autotune()
     ens.set_launch(graph_steps=100)
     info = ens.info()
     t = time_steps(ens, steps, warmup, torch)
@@ -400,7 +405,7 @@ def run_form(name, args, torch, stream, geom_cache):
         return bench_large(geom_cache["geom"], form, args.steps, args.warmup, torch, stream,
                            launch=args.launch, graph_steps=args.graph_steps)
     if name == "batched_256":
-        return bench_ensemble(max(args.steps, 200), 20, torch, stream)
+        return bench_ensemble(max(args.steps, 200), 20, torch, stream, launch=args.launch)
     if name.startswith("thermal_"):
         storage, size = name.split("_")[1:3]
         nx, ny = (int(v) for v in size.split("x"))

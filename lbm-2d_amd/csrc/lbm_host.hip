@@ -147,9 +147,12 @@ struct Ensemble {
     LinkRec* d_links = nullptr;
     size_t links_cap = 0;
     LatticeDev* d_lat = nullptr;
-    Tile* d_tiles = nullptr;
+    Tile* d_tiles = nullptr;       // ensembles: the one-step tiles
     size_t tiles_cap = 0;
     int n_tiles = 0;
+    Tile* d_tiles2 = nullptr;      // ensembles with two steps per launch: 64 x rows tiles
+    size_t tiles2_cap = 0;
+    int n_tiles2 = 0;
     int64_t* d_steps = nullptr;
     int32_t* d_par = nullptr;
     double* d_acc = nullptr;       // [acc_blocks][kAccCopies][total_slots][2]
@@ -545,6 +548,36 @@ LatticeDev make_dev(const HostLattice& H, int index, float scale)
 }
 
 // Rebuilds the geometry pools, slot layout and descriptors that changed.
+// Nodes on the lattice's edges: rows 0 and ny-1, columns 0 and nx-1 between them.
+int edge_nodes(int nx, int ny)
+{
+    return (ny == 1 ? nx : 2 * nx) + (nx == 1 ? 1 : 2) * std::max(ny - 2, 0);
+}
+
+// The shape and options of one-step launches {block_x, rows, nodes per thread, options}:
+// the handle's, or with two steps per launch that of the launches ending a batch (the
+// autotuner's fastest one-step shape, else (64, 1, 1) with the order and store options).
+void one_step_shape(const Ensemble* E, int s[4])
+{
+    if (!(E->options & LAUNCH_TWO_STEPS)) {
+        s[0] = E->bx;
+        s[1] = E->rows;
+        s[2] = E->vec;
+        s[3] = E->options;
+        return;
+    }
+    const bool tuned = E->tail[0] != 0;
+    s[0] = tuned ? E->tail[0] : 64;
+    s[1] = tuned ? E->tail[1] : 1;
+    s[2] = tuned ? E->tail[2] : 1;
+    s[3] = (tuned ? E->tail[3] : E->options) & (LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES);
+}
+
+// First column of a lattice's two-step tiles in an ensemble: 0, or -63 when the last of
+// the 64-wide tiles would hold only the outlet column (its outflow copy gathers the
+// column before it, which must lie inside the tile).
+int two_step_x_start(const lbm_desc& d) { return d.outlet && d.nx % 64 == 1 ? -63 : 0; }
+
 int commit(Ensemble* E)
 {
     int rc;
@@ -667,12 +700,17 @@ int commit(Ensemble* E)
         E->graph_dirty = true;  // one-lattice graphs hold the descriptor in their arguments
     }
     if (E->tiles_dirty && E->lat.size() > 1) {
+        // one-step launches: the tiles of every lattice (general-path nodes in place: list
+        // blocks here raised the kernel's registers from 93 to 136 and slowed the
+        // 256-lattice ensemble from 0.58 to 0.74 ms per step)
+        int s1[4];
+        one_step_shape(E, s1);
+        const int rows = s1[1], width = s1[0] * s1[2];
         std::vector<Tile> tiles;
         for (size_t l = 0; l < E->lat.size(); ++l) {
             const lbm_desc& d = E->lat[l].d;
-            for (int y0 = 0; y0 < d.ny; y0 += E->rows)
-                for (int x0 = 0; x0 < d.nx; x0 += E->bx * E->vec)
-                    tiles.push_back(Tile{static_cast<int32_t>(l), y0, x0, 0});
+            for (int y0 = 0; y0 < d.ny; y0 += rows)
+                for (int x0 = 0; x0 < d.nx; x0 += width) tiles.push_back(Tile{static_cast<int32_t>(l), y0, x0, 0});
         }
         if (tiles.size() > E->tiles_cap) {
             if ((rc = dev_alloc(E, &E->d_tiles, tiles.size())) != LBM_OK) return rc;
@@ -680,6 +718,24 @@ int commit(Ensemble* E)
         }
         if ((rc = h2d(E, E->d_tiles, tiles.data(), tiles.size() * sizeof(Tile))) != LBM_OK) return rc;
         E->n_tiles = static_cast<int>(tiles.size());
+        // two-step launches: 64 x rows tiles per lattice; rows of a lattice with an outlet
+        // and nx % 64 == 1 are right-aligned (the first tile starts at x = -63), so that the
+        // outflow copy's source column lies inside the last tile
+        std::vector<Tile> t2;
+        if (E->options & LAUNCH_TWO_STEPS) {
+            for (size_t l = 0; l < E->lat.size(); ++l) {
+                const lbm_desc& d = E->lat[l].d;
+                const int xs = two_step_x_start(d);
+                for (int y0 = 0; y0 < d.ny; y0 += E->rows)
+                    for (int x0 = xs; x0 < d.nx; x0 += 64) t2.push_back(Tile{static_cast<int32_t>(l), y0, x0, 0});
+            }
+            if (t2.size() > E->tiles2_cap) {
+                if ((rc = dev_alloc(E, &E->d_tiles2, t2.size())) != LBM_OK) return rc;
+                E->tiles2_cap = t2.size();
+            }
+            if ((rc = h2d(E, E->d_tiles2, t2.data(), t2.size() * sizeof(Tile))) != LBM_OK) return rc;
+        }
+        E->n_tiles2 = static_cast<int>(t2.size());
         E->tiles_dirty = false;
         E->graph_dirty = true;
     }
@@ -718,12 +774,6 @@ StepArgs base_args(Ensemble* E)
     a.sr = E->sr;
     a.sr_seed = E->sr_seed;
     return a;
-}
-
-// Nodes on the lattice's edges: rows 0 and ny-1, columns 0 and nx-1 between them.
-int edge_nodes(int nx, int ny)
-{
-    return (ny == 1 ? nx : 2 * nx) + (nx == 1 ? 1 : 2) * std::max(ny - 2, 0);
 }
 
 // Grid for one launch over one lattice (single >= 0, or an ensemble of one) or over
@@ -788,26 +838,31 @@ int ensure_history_room(Ensemble* E, int l, int64_t n)
 
 bool one_lattice(Ensemble* E, int l) { return l >= 0 || E->lat.size() == 1; }
 
-// Two-step launches in a batch of n steps (one-lattice launches with LAUNCH_TWO_STEPS);
-// the batch always ends with a one-step launch, so the last stored C precedes the state.
+// Two-step launches in a batch of n steps (LAUNCH_TWO_STEPS: a one-lattice handle, or
+// every lattice of an ensemble; one lattice of an ensemble alone steps one step per
+// launch); the batch always ends with a one-step launch, so the last stored C precedes
+// the state.
 int two_step_pairs(Ensemble* E, int n, int l)
 {
-    if (E->thermal || !one_lattice(E, l) || !(E->options & LAUNCH_TWO_STEPS) || n < 2) return 0;
+    if (E->thermal || !(E->options & LAUNCH_TWO_STEPS) || n < 2) return 0;
+    if (l >= 0 && E->lat.size() > 1) return 0;
     return (n - 1) / 2;
 }
 
-// Whether two steps per launch apply: one lattice, tile 64 x (4 or 8) with 256 threads
-// (block_x 64, one node per thread), and the outflow copy's source column (nx-2 and its
-// neighbours) inside the last tile's ring.
+// Whether two steps per launch apply: tile 64 x (4 or 8) with 256 threads (block_x 64,
+// one node per thread); on a one-lattice handle the outflow copy's source column (nx-2
+// and its neighbours) inside the last tile's ring (ensembles right-align such rows).
 int two_step_ok(Ensemble* E, int bx, int rows, int vec)
 {
-    if (E->lat.size() != 1) return fail(LBM_ERR_INVALID, "two steps per launch need a one-lattice handle");
     if (bx != 64 || vec != 1 || (rows != 4 && rows != 8))
         return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1) and (64, 8, 1)");
-    const lbm_desc& d = E->lat[0].d;
-    if (d.outlet && d.nx % 64 == 1) return fail(LBM_ERR_INVALID, "two steps per launch: nx % 64 == 1 with an outlet");
-    // the second step gathers from LDS; a free-slip edge's reflection reads the buffer
-    if (d.y_edge == LBM_EDGE_FREE_SLIP) return fail(LBM_ERR_INVALID, "two steps per launch: free-slip y edges");
+    for (const auto& H : E->lat) {
+        const lbm_desc& d = H.d;
+        if (E->lat.size() == 1 && d.outlet && d.nx % 64 == 1)
+            return fail(LBM_ERR_INVALID, "two steps per launch: nx % 64 == 1 with an outlet");
+        // the second step gathers from LDS; a free-slip edge's reflection reads the buffer
+        if (d.y_edge == LBM_EDGE_FREE_SLIP) return fail(LBM_ERR_INVALID, "two steps per launch: free-slip y edges");
+    }
     return LBM_OK;
 }
 
@@ -846,7 +901,6 @@ int launch_batch(Ensemble* E, int n, int l, double* acc_base = nullptr, double* 
     const bool single = grid_for(E, l, a, blocks);
     const int li = l >= 0 ? l : 0;
     const int32_t par = E->lat[li].par;
-    const bool two = !E->thermal && single && (E->options & LAUNCH_TWO_STEPS);
     const int pairs = two_step_pairs(E, n, l);
     if (!acc_base) acc_base = E->d_acc;
     if (!nf_base) nf_base = E->d_nf;
@@ -867,50 +921,55 @@ int launch_batch(Ensemble* E, int n, int l, double* acc_base = nullptr, double* 
             t.gdst = E->gbuf[1 - sp];
             LBM_HIP(launch_thermal(E->storage, a, t, E->stream));
         }
-    } else if (two) {
-        // two-step launches on 64 x rows tiles, then one-step launches of shape (64, 1, 1)
-        const lbm_desc& d = E->lat[li].d;
-        StepArgs a2 = a;
-        a2.tiles_x = (d.nx + 63) / 64;
-        a2.tiles_y = (d.ny + E->rows - 1) / E->rows;
-        const int blocks2 = a2.tiles_x * a2.tiles_y;
-        for (int i = 0; i < pairs; ++i, ++swaps) {
-            const int sp = par ^ (swaps & 1);
-            a2.src = E->buf[sp];
-            a2.dst = E->buf[1 - sp];
-            a2.acc = block_of(2 * i);
-            a2.acc2 = block_of(2 * i + 1);
-            a2.nf = terms_of(2 * i);
-            a2.nf2 = terms_of(2 * i + 1);
-            a2.step_arg = 2 * i;
-            LBM_HIP(launch_step2(E->storage, E->rows, blocks2, a2, E->stream));
+    } else {
+        // two-step launches on 64 x rows tiles (one lattice: a grid of tiles; an ensemble:
+        // its two-step tile table), then one-step launches
+        if (pairs > 0) {
+            StepArgs a2 = a;
+            int blocks2;
+            if (single) {
+                const lbm_desc& d = E->lat[li].d;
+                a2.tiles_x = (d.nx + 63) / 64;
+                a2.tiles_y = (d.ny + E->rows - 1) / E->rows;
+                blocks2 = a2.tiles_x * a2.tiles_y;
+            } else {
+                a2.tiles = E->d_tiles2;
+                a2.n_tiles = E->n_tiles2;
+                blocks2 = E->n_tiles2;
+            }
+            for (int i = 0; i < pairs; ++i, ++swaps) {
+                if (single) {
+                    const int sp = par ^ (swaps & 1);
+                    a2.src = E->buf[sp];
+                    a2.dst = E->buf[1 - sp];
+                }
+                a2.acc = block_of(2 * i);
+                a2.acc2 = block_of(2 * i + 1);
+                a2.nf = terms_of(2 * i);
+                a2.nf2 = terms_of(2 * i + 1);
+                a2.step_arg = 2 * i;
+                a2.flips = swaps;
+                LBM_HIP(launch_step2(E->storage, E->rows, single, blocks2, a2, E->stream));
+            }
         }
+        int s1[4];
+        one_step_shape(E, s1);
         StepArgs a1 = base_args(E);
         int blocks1;
-        const bool tuned = E->tail[0] != 0;
-        const int tbx = tuned ? E->tail[0] : 64, trows = tuned ? E->tail[1] : 1, tvec = tuned ? E->tail[2] : 1;
-        grid_for(E, l, a1, blocks1, tbx, trows, tvec);
-        a1.options = (tuned ? E->tail[3] : E->options) & (LAUNCH_XCD_ORDER | LAUNCH_STREAM_STORES);
+        grid_for(E, l, a1, blocks1, s1[0], s1[1], s1[2]);
+        a1.options = s1[3];
         for (int i = 2 * pairs; i < n; ++i, ++swaps) {
-            const int sp = par ^ (swaps & 1);
-            a1.src = E->buf[sp];
-            a1.dst = E->buf[1 - sp];
+            if (single) {
+                const int sp = par ^ (swaps & 1);
+                a1.src = E->buf[sp];
+                a1.dst = E->buf[1 - sp];
+            }
             a1.acc = block_of(i);
             a1.nf = terms_of(i);
             a1.step_arg = i;
-            LBM_HIP(launch_step(E->storage, tbx, trows, tvec, true, blocks1, a1, E->stream));
+            a1.flips = swaps;
+            LBM_HIP(launch_step(E->storage, s1[0], s1[1], s1[2], single, blocks1, a1, E->stream));
         }
-    }
-    for (int i = 0; i < n && !two && !E->thermal; ++i, ++swaps) {
-        a.acc = block_of(i);
-        a.nf = terms_of(i);
-        a.step_arg = i;
-        if (single) {
-            const int sp = par ^ (i & 1);
-            a.src = E->buf[sp];
-            a.dst = E->buf[1 - sp];
-        }
-        LBM_HIP(launch_step(E->storage, E->bx, E->rows, E->vec, single, blocks, a, E->stream));
     }
     FlushArgs f{};
     f.acc = acc_base;
@@ -1345,7 +1404,7 @@ int lbm_destroy(lbm_handle h)
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     lbm::drop_graphs(E);
     void* ptrs[] = {E->buf[0], E->buf[1], E->d_flags, E->d_lslot, E->d_wdist, E->d_links, E->d_lat,
-                    E->d_tiles, E->d_steps, E->d_par, E->d_acc, E->d_hist, E->d_dummy_acc, E->d_nan,
+                    E->d_tiles, E->d_tiles2, E->d_steps, E->d_par, E->d_acc, E->d_hist, E->d_dummy_acc, E->d_nan,
                     E->d_dummy_nan, E->d_mass, E->d_stage, E->d_slot_lat, E->d_special, E->d_nf,
                     E->d_dummy_nf, E->d_slot_start, E->d_slot_recs, E->gbuf[0], E->gbuf[1], E->d_tcode};
     for (void* p : ptrs)
@@ -1930,7 +1989,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         return rc;
     }
     // launch n of a candidate: one lattice reads copies[(par + n) & 1] and writes the
-    // other; an ensemble takes both from the per-lattice parity flipped by step_arg
+    // other; an ensemble takes both from the per-lattice parity flipped n times
     const int par0 = E->lat[0].par;
     auto ping = [&](StepArgs& a, bool single, int n) {
         if (single) {
@@ -1941,6 +2000,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
             a.buf1 = copies[1];
             a.dst = nullptr;
             a.step_arg = n;
+            a.flips = n;
         }
     };
     hipEvent_t e0, e1;
@@ -2011,37 +2071,45 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
             bs[3] = E->options;
         }
     }
-    // two steps per launch (one-lattice handles): per-step time of each tile height
-    for (int si = 0; si < 8 && rc == LBM_OK && E->lat.size() == 1; ++si) {
+    // two steps per launch: per-step time of each tile height
+    for (int si = 0; si < 8 && rc == LBM_OK; ++si) {
         const int rows = 4 << (si / 4), opt = LAUNCH_TWO_STEPS | (si & 3);
         if (two_step_ok(E, 64, rows, 1) != LBM_OK) continue;
         E->bx = 64;
         E->rows = rows;
         E->vec = 1;
         E->options = opt;
+        E->tiles_dirty = true;
+        if ((rc = commit(E)) != LBM_OK) break;
         StepArgs a = base_args(E);
         int blocks;
-        grid_for(E, -1, a, blocks);
-        const lbm_desc& d = E->lat[0].d;
-        a.tiles_x = (d.nx + 63) / 64;
-        a.tiles_y = (d.ny + rows - 1) / rows;
-        blocks = a.tiles_x * a.tiles_y;
+        const bool single = grid_for(E, -1, a, blocks);
+        if (single) {
+            const lbm_desc& d = E->lat[0].d;
+            a.tiles_x = (d.nx + 63) / 64;
+            a.tiles_y = (d.ny + rows - 1) / rows;
+            blocks = a.tiles_x * a.tiles_y;
+        } else {
+            a.tiles = E->d_tiles2;
+            a.n_tiles = E->n_tiles2;
+            blocks = E->n_tiles2;
+        }
         a.acc = a.acc2 = E->d_dummy_acc;
         a.nf = a.nf2 = E->d_dummy_nf;
         a.nan_flags = E->d_dummy_nan;
         int n = 0;
         hipError_t le = hipSuccess;
         for (int w = 0; w < 2 && le == hipSuccess; ++w, ++n) {
-            ping(a, true, n);
-            le = launch_step2(E->storage, rows, blocks, a, E->stream);
+            ping(a, single, n);
+            le = launch_step2(E->storage, rows, single, blocks, a, E->stream);
         }
         float ms = 0.f;
         bool ok = le == hipSuccess;
         for (int pass = 0, reps = 5; pass < 2 && ok; ++pass) {
             (void)hipEventRecord(e0, E->stream);
             for (int r = 0; r < reps && le == hipSuccess; ++r, ++n) {
-                ping(a, true, n);
-                le = launch_step2(E->storage, rows, blocks, a, E->stream);
+                ping(a, single, n);
+                le = launch_step2(E->storage, rows, single, blocks, a, E->stream);
             }
             (void)hipEventRecord(e1, E->stream);
             ok = le == hipSuccess && hipEventSynchronize(e1) == hipSuccess;

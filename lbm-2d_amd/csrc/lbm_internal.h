@@ -59,7 +59,13 @@ struct alignas(16) LinkRec {
 
 static_assert(offsetof(LinkRec, slot) == 8 && offsetof(LinkRec, cell) == 28, "LinkRec layout");
 
-// One block of work of the step kernel: lattice, first row, first column.
+// Lanes of a wave that take boundary-link list nodes (their loads scatter, and a wave's
+// memory requests issue one address at a time).
+constexpr int kListLanes = 16;
+
+// One block of work of an ensemble launch: lattice, first row, first column.  Two-step
+// tiles may start at a negative column (right-aligned rows: the nodes left of x = 0 are
+// not the tile's).
 struct Tile {
     int32_t lattice, y0, x0, pad;
 };

@@ -39,7 +39,8 @@ struct StepArgs {
     const LatticeDev* lat;
     const Tile* tiles;          // one entry per block
     int32_t n_tiles;
-    int32_t step_arg;           // launch index inside the batch (buffer parity)
+    int32_t step_arg;           // step index inside the batch (stochastic-rounding keys)
+    int32_t flips;              // buffer swaps before this launch in the batch (parity)
     const int32_t* par;         // per-lattice index of the buffer holding the state
     void* buf0;
     void* buf1;
@@ -212,8 +213,9 @@ hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hi
 hipError_t launch_step(int storage, int block_x, int rows, int vec, bool single, int n_blocks, const StepArgs& a,
                        hipStream_t s);
 hipError_t launch_flush(const FlushArgs& f, hipStream_t s);
-// Two steps of one lattice per launch: tiles of 64 x tile_rows (4 or 8), 256 threads.
-hipError_t launch_step2(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s);
+// Two steps per launch: tiles of 64 x tile_rows (4 or 8), 256 threads; single: one
+// lattice (StepArgs.L0, src, dst, tiles_x), else the ensemble's two-step tile table.
+hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a, hipStream_t s);
 // Blocks of block_x threads that take the general-path list nodes: n_links link
 // records (kListLanes per wave), then n_edges edge nodes (every lane).
 int list_blocks(int block_x, int n_links, int n_edges);

@@ -676,9 +676,6 @@ __device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs
         store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j), a.options & LAUNCH_STREAM_STORES);
 }
 
-// Lanes of a wave that take general-path list nodes.
-constexpr int kListLanes = 16;
-
 // Edge node number e of a lattice: rows 0 and ny-1, then columns 0 and nx-1 between them.
 __device__ __forceinline__ void edge_position(const LatticeDev& L, int e, int& x, int& y)
 {
@@ -696,18 +693,17 @@ __device__ __forceinline__ void edge_position(const LatticeDev& L, int e, int& x
     }
 }
 
-// The general-path node t of a one-lattice launch: boundary-link record t (t < n_links;
-// the record gives the position, so links, flags, stored values and sources load in
-// two rounds), else edge node t - n_links (nodes with links are the records'; solid
-// ones are written only when marked).  One call site of node_general keeps one copy
-// of the general path in the kernel.
+// A general-path list node of lattice L: boundary-link record t (is_link; the record
+// gives the position, so links, flags, stored values and sources load in two rounds),
+// else edge node t (nodes with links are the records'; solid ones are written only when
+// marked).  One call site of node_general keeps one copy of the general path in the
+// kernel.
 template <int ST>
 __device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
-                                          typename Form<ST>::store* dst, const ForceSink& fs, int t,
-                                          const Rounding& rd, bool& bad)
+                                          typename Form<ST>::store* dst, const ForceSink& fs, bool is_link,
+                                          int t, const Rounding& rd, bool& bad)
 {
     using R = typename Form<ST>::real;
-    const bool is_link = t < a.n_links;
     LinkVals<R> own_links;
     int x, y;
     if (is_link) {
@@ -717,7 +713,7 @@ __device__ __forceinline__ void list_node(const LatticeDev& L, const StepArgs& a
         y = e / L.pitch;
         x = e - y * L.pitch;
     } else {
-        edge_position(L, t - a.n_links, x, y);
+        edge_position(L, t, x, y);
     }
     const int gx = gathered_x<ST>(L, x);
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
@@ -966,7 +962,10 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
                     const int k = a.n_links + (b - gl) * BX + static_cast<int>(threadIdx.x);
                     if (k < a.n_links + a.n_edges) t = k;
                 }
-                if (t >= 0) list_node<ST>(L, a, c, dst, fs, t, rd, bad);
+                if (t >= 0) {
+                    const bool is_link = t < a.n_links;
+                    list_node<ST>(L, a, c, dst, fs, is_link, is_link ? t : t - a.n_links, rd, bad);
+                }
             } else {
                 // written solids (Bouzidi sources): zero populations, f - w = -w
                 const int t = (b - ge) * BX + static_cast<int>(threadIdx.x);
@@ -999,7 +998,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
         x0 = t.x0;
         L = a.lat[lat_id];
         if (rd.on) rd.key = step_key(a, lat_id, 0);
-        const int sp = a.par[lat_id] ^ (a.step_arg & 1);
+        const int sp = a.par[lat_id] ^ (a.flips & 1);
         src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
         dst = static_cast<S*>(a.dst ? a.dst : (sp ? a.buf0 : a.buf1));
     }
@@ -1047,7 +1046,25 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
 // Post-collision values of node (x,y) for the next state, read from the source buffer
 // like cell_update; forces counted only when accp is non-null.  false: a solid node
 // nothing reads (no value).
-template <int ST>
+// A fluid node whose general path reduces to its pulls (wrapped around the periodic
+// edges it lies on) and the collision: no boundary links, every edge it lies on periodic,
+// and no inlet, outflow copy or lid on it.
+__device__ __forceinline__ bool wrap_only(const LatticeDev& L, uint8_t fl, int x, int y)
+{
+    if (fl & (CELL_SOLID | CELL_LINKS | CELL_COLUMN)) return false;
+    if ((x == 0 || x == L.nx - 1) && L.x_edge != EDGE_PERIODIC) return false;
+    if ((y == 0 || y == L.ny - 1) && L.y_edge != EDGE_PERIODIC) return false;
+    if (L.lid && y == L.ny - 1) return false;
+    if (L.inlet != INLET_NONE && x == 0) return false;
+    if (L.outlet && x == L.nx - 1) return false;
+    return true;
+}
+
+// Bulk nodes take their pulls directly, the others the general path.  EDGES: so do the
+// inlet and outlet column nodes (as the one-step kernel's bulk tiles take them) and the
+// wrap-only nodes -- on small lattices, whose tiles often hold an edge, this keeps whole
+// tiles off the general path; on large ones the extra code costs more than it saves.
+template <int ST, bool EDGES>
 __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
                                             const ForceSink& fs, int x, int y, const Rounding& rd, bool& bad,
                                             typename Form<ST>::store out[9])
@@ -1057,15 +1074,29 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
     const uint8_t fl = a.flags[L.cell_off + idx];
     S raw[9];
-    const S* b = c.base + idx;
+    if (!EDGES || (x >= 1 && x <= L.nx - 2 && y >= 1 && y <= L.ny - 2)) {
+        const S* b = c.base + idx;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) raw[j] = b[j * L.plane - (CY[j] * L.pitch + CX[j])];
+        for (int j = 0; j < 9; ++j) raw[j] = b[j * L.plane - (CY[j] * L.pitch + CX[j])];
+    } else {
+        // edge nodes: the pulls of the gathered node (x - 1 on the outlet column),
+        // wrapped on periodic edges (unwrapped sources of open edges stay inside the pool)
+        const int px = gathered_x<ST>(L, x);
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            int sx = px - CX[j], sy = y - CY[j];
+            if (L.x_edge == EDGE_PERIODIC) sx = sx < 0 ? sx + L.nx : (sx >= L.nx ? sx - L.nx : sx);
+            if (L.y_edge == EDGE_PERIODIC) sy = sy < 0 ? sy + L.ny : (sy >= L.ny ? sy - L.ny : sy);
+            raw[j] = c.base[j * L.plane + static_cast<int64_t>(sy) * L.pitch + sx];
+        }
+    }
     uint16_t wraw = 0xffffu;
     if (L.les) wraw = a.wdist[L.cell_off + idx];
     __asm__ volatile("" ::"v"(raw[0]), "v"(raw[1]), "v"(raw[2]), "v"(raw[3]), "v"(raw[4]), "v"(raw[5]),
                      "v"(raw[6]), "v"(raw[7]), "v"(raw[8]), "v"(wraw));
     R d[9];
-    if (fl & CELL_SPECIAL) {
+    const bool column = EDGES && (fl & CELL_COLUMN) != 0;
+    if ((fl & CELL_SPECIAL) && !column && !(EDGES && wrap_only(L, fl, x, y))) {
         if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return false;
         const int gx = gathered_x<ST>(L, x);
         Pre<ST> p;
@@ -1078,6 +1109,8 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
     } else {
 #pragma unroll
         for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(raw[j], a.inv_scale, L.shift[j]);
+        // an inlet node: the closure replaces the three slots whose sources lie off the lattice
+        if (column && x == 0) inlet_closure<R>(L, y, d);
         R wd = R(0);
         if (L.les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
         collide<R>(d, L, wd, R(1));
@@ -1096,27 +1129,53 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
 // arithmetic step is the one-step kernel's, so results are bit for bit the same; C_k
 // itself is never written, so a batch ends with a one-step launch (downloads re-gather
 // the state from the last stored C).  Forces of step k go to acc, of step k+1 to acc2.
-template <int ST, int TY>
-__global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
+//
+// Ensemble launches (SINGLE false) take each block's lattice and tile origin from the
+// two-step tile table and the buffers from the lattice's parity; a tile may start left
+// of x = 0 (right-aligned rows, so that the outflow copy's source column lies inside the
+// last tile), and nodes left of x = 0 are then not the tile's.
+template <int ST, int TY, bool SINGLE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F64 ? 1 : 4))) void step2_kernel(
+    StepArgs a)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
     constexpr int TX = 64, WX = TX + 2, WY = TY + 2, P2 = TX * TY / 256;
     __shared__ S lds[9][WY][WX];
 
-    const LatticeDev L = a.L0;
-    const S* src = static_cast<const S*>(a.src);
-    S* dst = static_cast<S*>(a.dst);
+    LatticeDev L;
+    const S* src;
+    S* dst;
+    int lat_id;
     const int nb = static_cast<int>(gridDim.x);
-    __asm__ volatile("" ::"s"(src), "s"(dst), "s"(a.flags), "s"(a.wdist), "s"(a.acc), "s"(a.acc2),
-                     "s"(a.total_slots), "s"(a.tiles_x), "s"(a.options), "s"(nb), "s"(L.nx), "s"(L.ny),
-                     "s"(L.pitch), "s"(L.f_off), "s"(L.plane), "s"(L.cell_off), "s"(L.les));
+    if constexpr (SINGLE) {
+        L = a.L0;
+        lat_id = a.lattice0;
+        src = static_cast<const S*>(a.src);
+        dst = static_cast<S*>(a.dst);
+        __asm__ volatile("" ::"s"(src), "s"(dst), "s"(a.flags), "s"(a.wdist), "s"(a.acc), "s"(a.acc2),
+                         "s"(a.total_slots), "s"(a.tiles_x), "s"(a.options), "s"(nb), "s"(L.nx), "s"(L.ny),
+                         "s"(L.pitch), "s"(L.f_off), "s"(L.plane), "s"(L.cell_off), "s"(L.les));
+    }
     int b = static_cast<int>(blockIdx.x);
     if (a.options & LAUNCH_XCD_ORDER) {
         const int q = nb >> 3, r = nb & 7, xcd = b & 7;
         b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
     }
-    const int x0 = (b % a.tiles_x) * TX, y0 = (b / a.tiles_x) * TY;
+    int x0, y0;
+    if constexpr (SINGLE) {
+        x0 = (b % a.tiles_x) * TX;
+        y0 = (b / a.tiles_x) * TY;
+    } else {
+        const Tile t = a.tiles[b];
+        lat_id = t.lattice;
+        x0 = t.x0;
+        y0 = t.y0;
+        L = a.lat[lat_id];
+        const int sp = a.par[lat_id] ^ (a.flips & 1);
+        src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
+        dst = static_cast<S*>(a.dst ? a.dst : (sp ? a.buf0 : a.buf1));
+    }
     const int copy = static_cast<int>(blockIdx.x % kAccCopies);
     double* acc0 = a.acc + static_cast<int64_t>(copy) * a.total_slots * 2;
     double* acc1 = a.acc2 + static_cast<int64_t>(copy) * a.total_slots * 2;
@@ -1133,7 +1192,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
         fl2[k] = 0;
         ls2[k] = 0;
         wd2[k] = 0xffffu;
-        if (x < L.nx && y < L.ny) {
+        if (x < L.nx && y < L.ny && (SINGLE || x >= 0)) {
             const int64_t idx = L.cell_off + static_cast<int64_t>(y) * L.pitch + x;
             fl2[k] = a.flags[idx];
             ls2[k] = a.link_slot[idx];
@@ -1145,8 +1204,8 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
     bool bad = false;
     Rounding rd0{a.sr != 0, 0u}, rd1{a.sr != 0, 0u};
     if (rd0.on) {
-        rd0.key = step_key(a, a.lattice0, 0);
-        rd1.key = step_key(a, a.lattice0, 1);
+        rd0.key = step_key(a, lat_id, 0);
+        rd1.key = step_key(a, lat_id, 1);
     }
     // step k on the tile and its ring
     for (int r = tid; r < WX * WY; r += 256) {
@@ -1161,9 +1220,9 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
             if (L.y_edge != EDGE_PERIODIC) continue;
             gy = gy < 0 ? gy + L.ny : gy - L.ny;
         }
-        const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny;
+        const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny && (SINGLE || ux >= 0);
         S out[9];
-        if (node_values<ST>(L, a, c, own ? fs0 : none, gx, gy, rd0, bad, out)) {
+        if (node_values<ST, !SINGLE>(L, a, c, own ? fs0 : none, gx, gy, rd0, bad, out)) {
 #pragma unroll
             for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
         }
@@ -1174,7 +1233,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
 #pragma unroll 1
     for (int k = 0; k < P2; ++k) {
         const int n = tid + 256 * k, tx = n % TX, ty = n / TX, x = x0 + tx, y = y0 + ty;
-        if (x >= L.nx || y >= L.ny) continue;
+        if (x >= L.nx || y >= L.ny || (!SINGLE && x < 0)) continue;
         uint8_t fl = fl2[0];
         int32_t lsk = ls2[0];
         uint16_t wdk = wd2[0];
@@ -1187,7 +1246,8 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
             }
         const int lx = tx + 1, ly = ty + 1;
         R d[9];
-        if (fl & CELL_SPECIAL) {
+        const bool column = !SINGLE && (fl & CELL_COLUMN) != 0;
+        if ((fl & CELL_SPECIAL) && !column && !(!SINGLE && wrap_only(L, fl, x, y))) {
             if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) continue;
             const int gx = gathered_x<ST>(L, x), glx = gx - x0 + 1;
             Pre<ST> p;
@@ -1210,8 +1270,11 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
             if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + lsk], ls_);
             node_general_d<ST>(L, c, fs1, x, y, fl, p, lg, ls_, wdk, bad, d);
         } else {
+            // the ring holds the wrapped sources; the outlet column pulls those of x - 1
+            const int slx = (column && x == L.nx - 1 && L.outlet) ? lx - 1 : lx;
 #pragma unroll
-            for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(lds[j][ly - CY[j]][lx - CX[j]], a.inv_scale, L.shift[j]);
+            for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(lds[j][ly - CY[j]][slx - CX[j]], a.inv_scale, L.shift[j]);
+            if (column && x == 0) inlet_closure<R>(L, y, d);
             R wd = R(0);
             if (L.les) wd = wdk == 0xffffu ? R(1e9) : static_cast<R>(wdk);
             collide<R>(d, L, wd, R(1));
@@ -1224,7 +1287,7 @@ __global__ __launch_bounds__(256) void step2_kernel(StepArgs a)
         for (int j = 0; j < 9; ++j)
             store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j), a.options & LAUNCH_STREAM_STORES);
     }
-    if (bad) atomicOr(a.nan_flags + a.lattice0, 1);
+    if (bad) atomicOr(a.nan_flags + lat_id, 1);
 }
 
 // ------------------------------------------------------------------ state / prologue
@@ -2022,28 +2085,35 @@ int list_blocks(int block_x, int n_links, int n_edges)
     return (n_links + per_block - 1) / per_block + (n_edges + block_x - 1) / block_x;
 }
 
-template <int ST>
+template <int ST, bool SINGLE>
 hipError_t launch_step2_st(int ty, int n_blocks, const StepArgs& a, hipStream_t s)
 {
     if (ty == 4) {
-        hipLaunchKernelGGL((step2_kernel<ST, 4>), dim3(n_blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((step2_kernel<ST, 4, SINGLE>), dim3(n_blocks), dim3(256), 0, s, a);
     } else if (ty == 8) {
-        hipLaunchKernelGGL((step2_kernel<ST, 8>), dim3(n_blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((step2_kernel<ST, 8, SINGLE>), dim3(n_blocks), dim3(256), 0, s, a);
     } else {
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_step2(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s)
+template <bool SINGLE>
+hipError_t launch_step2_sg(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s)
 {
     switch (storage) {
-    case ST_F64: return launch_step2_st<ST_F64>(tile_rows, n_blocks, a, s);
-    case ST_F32: return launch_step2_st<ST_F32>(tile_rows, n_blocks, a, s);
-    case ST_F16S: return launch_step2_st<ST_F16S>(tile_rows, n_blocks, a, s);
-    case ST_BF16S: return launch_step2_st<ST_BF16S>(tile_rows, n_blocks, a, s);
+    case ST_F64: return launch_step2_st<ST_F64, SINGLE>(tile_rows, n_blocks, a, s);
+    case ST_F32: return launch_step2_st<ST_F32, SINGLE>(tile_rows, n_blocks, a, s);
+    case ST_F16S: return launch_step2_st<ST_F16S, SINGLE>(tile_rows, n_blocks, a, s);
+    case ST_BF16S: return launch_step2_st<ST_BF16S, SINGLE>(tile_rows, n_blocks, a, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a, hipStream_t s)
+{
+    return single ? launch_step2_sg<true>(storage, tile_rows, n_blocks, a, s)
+                  : launch_step2_sg<false>(storage, tile_rows, n_blocks, a, s);
 }
 
 hipError_t launch_flush(const FlushArgs& f, hipStream_t s)

@@ -267,6 +267,9 @@ class Oracle:
         lib().orc_init_equilibrium(self._h, rho, u, v)
 
     def perturb_seed42(self, x0, x1):
+        # the restated loop (like the driver's) does not clamp its column range
+        if not 0 <= x0 < x1 <= self.nx:
+            raise ValueError(f"perturbation columns [{x0}, {x1}) outside a lattice {self.nx} wide")
         lib().orc_perturb_seed42(self._h, x0, x1)
 
     def set_f(self, f):
@@ -399,6 +402,9 @@ class RefLattice:
         self.L.ref_init_equilibrium(rho, u, v)
 
     def perturb_seed42(self, x0, x1):
+        # the reference's loop does not clamp its column range
+        if not 0 <= x0 < x1 <= self.nx:
+            raise ValueError(f"perturbation columns [{x0}, {x1}) outside a lattice {self.nx} wide")
         self.L.ref_perturb_seed42(x0, x1)
 
     def set_f(self, f):

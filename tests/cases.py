@@ -154,6 +154,14 @@ SPEC_CASES = {
                          gpu_geom=(("circle", 24, 24, 6, G.BB_BOUZIDI), ("moving", 0.01, 24, 24))),
 }
 
+# Widths whose last 64-node column block holds only the outlet column (nx % 64 == 1):
+# an ensemble cuts their rows into right-aligned two-step tiles.  The perturbation
+# columns stay inside the lattice (the restated loop does not clamp them).
+NARROW_CASES = {
+    "cylinder_129": dataclasses.replace(CASES["cylinder"], name="cylinder_129", nx=129),
+    "les_65": dataclasses.replace(CASES["les"], name="les_65", nx=65, pert=(29, 60)),
+}
+
 
 def build_oracle(c: Case, cls=O.Oracle):
     o = cls(c.nx, c.ny, c.case, collision=c.collision, les=c.les, cs=c.cs)

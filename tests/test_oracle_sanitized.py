@@ -23,8 +23,8 @@ sys.path[:0] = [root, root + '/tests', root + '/lbm-2d_amd']
 import numpy as np
 import oracle as O
 assert O.ORACLE_SO.endswith('_san/liblbm_oracle.so'), O.ORACLE_SO
-from cases import CASES, THERMAL_CASES, build_oracle, build_thermal_oracle, thermal_step
-for name, c in sorted(CASES.items()):
+from cases import CASES, NARROW_CASES, SPEC_CASES, THERMAL_CASES, build_oracle, build_thermal_oracle, thermal_step
+for name, c in sorted({**CASES, **SPEC_CASES, **NARROW_CASES}.items()):
     o = build_oracle(c)
     o.step(c.tau, c.u, min(c.steps, 30))
     assert np.isfinite(o.f).all(), name
