@@ -232,6 +232,12 @@ int lbm_step_noalloc(lbm_handle h, int32_t n);
 /* Launch options of the step kernel (bit field). */
 #define LBM_LAUNCH_XCD_ORDER 1     /* consecutive tiles stay on one XCD (one L2) */
 #define LBM_LAUNCH_STREAM_STORES 2 /* nontemporal stores of the new populations */
+/* Two steps per launch: tiles of 64 x rows nodes (shape (64, rows, 1), rows 4, 8 or 16)
+ * with a one-node ring, the intermediate state in LDS.  Every lattice of the handle
+ * (an ensemble too; not thermal handles, not free-slip y edges, and on a one-lattice
+ * handle not an outlet with nx % 64 == 1); a batch ends with one-step launches.  The
+ * results are those of one step per launch, bit for bit. */
+#define LBM_LAUNCH_TWO_STEPS 4
 
 /* Launch shape of the step kernel (threads per block, rows per block, consecutive
  * nodes per thread), option bits and hipGraph length; 0 (-1 for options) keeps the

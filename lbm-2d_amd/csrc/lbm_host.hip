@@ -849,13 +849,13 @@ int two_step_pairs(Ensemble* E, int n, int l)
     return (n - 1) / 2;
 }
 
-// Whether two steps per launch apply: tile 64 x (4 or 8) with 256 threads (block_x 64,
+// Whether two steps per launch apply: tile 64 x (4, 8 or 16) with 256 threads (block_x 64,
 // one node per thread); on a one-lattice handle the outflow copy's source column (nx-2
 // and its neighbours) inside the last tile's ring (ensembles right-align such rows).
 int two_step_ok(Ensemble* E, int bx, int rows, int vec)
 {
-    if (bx != 64 || vec != 1 || (rows != 4 && rows != 8))
-        return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1) and (64, 8, 1)");
+    if (bx != 64 || vec != 1 || (rows != 4 && rows != 8 && rows != 16))
+        return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1), (64, 8, 1) and (64, 16, 1)");
     for (const auto& H : E->lat) {
         const lbm_desc& d = H.d;
         if (E->lat.size() == 1 && d.outlet && d.nx % 64 == 1)
@@ -2072,7 +2072,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         }
     }
     // two steps per launch: per-step time of each tile height
-    for (int si = 0; si < 8 && rc == LBM_OK; ++si) {
+    for (int si = 0; si < 12 && rc == LBM_OK; ++si) {
         const int rows = 4 << (si / 4), opt = LAUNCH_TWO_STEPS | (si & 3);
         if (two_step_ok(E, 64, rows, 1) != LBM_OK) continue;
         E->bx = 64;

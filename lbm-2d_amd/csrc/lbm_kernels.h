@@ -213,7 +213,7 @@ hipError_t launch_thermal_load(int storage, const ThermalLoadArgs& a, int nx, hi
 hipError_t launch_step(int storage, int block_x, int rows, int vec, bool single, int n_blocks, const StepArgs& a,
                        hipStream_t s);
 hipError_t launch_flush(const FlushArgs& f, hipStream_t s);
-// Two steps per launch: tiles of 64 x tile_rows (4 or 8), 256 threads; single: one
+// Two steps per launch: tiles of 64 x tile_rows (4, 8 or 16), 256 threads; single: one
 // lattice (StepArgs.L0, src, dst, tiles_x), else the ensemble's two-step tile table.
 hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a, hipStream_t s);
 // Blocks of block_x threads that take the general-path list nodes: n_links link

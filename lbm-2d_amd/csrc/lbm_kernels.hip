@@ -1064,17 +1064,26 @@ __device__ __forceinline__ bool wrap_only(const LatticeDev& L, uint8_t fl, int x
 // inlet and outlet column nodes (as the one-step kernel's bulk tiles take them) and the
 // wrap-only nodes -- on small lattices, whose tiles often hold an edge, this keeps whole
 // tiles off the general path; on large ones the extra code costs more than it saves.
-template <int ST, bool EDGES>
-__device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
-                                            const ForceSink& fs, int x, int y, const Rounding& rd, bool& bad,
-                                            typename Form<ST>::store out[9])
+//
+// MODE: NV_ALL computes either; NV_PLAIN computes plain nodes and returns -1 for general
+// ones (left for a later NV_GENERAL call, which takes the general path without the
+// plain pulls).  Returns 1 when out holds the node's values, 0 for a solid node nothing
+// reads.
+enum { NV_ALL = 0, NV_PLAIN = 1, NV_GENERAL = 2 };
+
+template <int ST, bool EDGES, int MODE>
+__device__ __forceinline__ int node_values(const LatticeDev& L, const StepArgs& a, const Ctx<ST>& c,
+                                           const ForceSink& fs, int x, int y, const Rounding& rd, bool& bad,
+                                           typename Form<ST>::store out[9])
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
     const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
     const uint8_t fl = a.flags[L.cell_off + idx];
     S raw[9];
-    if (!EDGES || (x >= 1 && x <= L.nx - 2 && y >= 1 && y <= L.ny - 2)) {
+    if (MODE == NV_GENERAL) {
+        // no plain pulls
+    } else if (!EDGES || (x >= 1 && x <= L.nx - 2 && y >= 1 && y <= L.ny - 2)) {
         const S* b = c.base + idx;
 #pragma unroll
         for (int j = 0; j < 9; ++j) raw[j] = b[j * L.plane - (CY[j] * L.pitch + CX[j])];
@@ -1092,12 +1101,14 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
     }
     uint16_t wraw = 0xffffu;
     if (L.les) wraw = a.wdist[L.cell_off + idx];
-    __asm__ volatile("" ::"v"(raw[0]), "v"(raw[1]), "v"(raw[2]), "v"(raw[3]), "v"(raw[4]), "v"(raw[5]),
-                     "v"(raw[6]), "v"(raw[7]), "v"(raw[8]), "v"(wraw));
+    if constexpr (MODE != NV_GENERAL)
+        __asm__ volatile("" ::"v"(raw[0]), "v"(raw[1]), "v"(raw[2]), "v"(raw[3]), "v"(raw[4]), "v"(raw[5]),
+                         "v"(raw[6]), "v"(raw[7]), "v"(raw[8]), "v"(wraw));
     R d[9];
     const bool column = EDGES && (fl & CELL_COLUMN) != 0;
-    if ((fl & CELL_SPECIAL) && !column && !(EDGES && wrap_only(L, fl, x, y))) {
-        if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return false;
+    const bool general = (fl & CELL_SPECIAL) && !column && !(EDGES && wrap_only(L, fl, x, y));
+    if (MODE == NV_GENERAL || (MODE == NV_ALL && general)) {
+        if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) return 0;
         const int gx = gathered_x<ST>(L, x);
         Pre<ST> p;
         preload<ST>(c, gx, y, p);
@@ -1107,6 +1118,7 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
         if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + a.link_slot[L.cell_off + idx]], ls_);
         node_general_d<ST>(L, c, fs, x, y, fl, p, lg, ls_, wraw, bad, d);
     } else {
+        if (MODE == NV_PLAIN && general) return (fl & CELL_SOLID) && !(fl & CELL_WRITE) ? 0 : -1;
 #pragma unroll
         for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(raw[j], a.inv_scale, L.shift[j]);
         // an inlet node: the closure replaces the three slots whose sources lie off the lattice
@@ -1119,7 +1131,7 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
     uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
 #pragma unroll
     for (int j = 0; j < 9; ++j) out[j] = enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j);
-    return true;
+    return 1;
 }
 
 // Steps k and k+1 of one lattice in one launch.  A block owns a 64 x TY tile: it
@@ -1133,7 +1145,11 @@ __device__ __forceinline__ bool node_values(const LatticeDev& L, const StepArgs&
 // Ensemble launches (SINGLE false) take each block's lattice and tile origin from the
 // two-step tile table and the buffers from the lattice's parity; a tile may start left
 // of x = 0 (right-aligned rows, so that the outflow copy's source column lies inside the
-// last tile), and nodes left of x = 0 are then not the tile's.
+// last tile), and nodes left of x = 0 are then not the tile's.  On their small lattices
+// one tile in twelve holds boundary-link nodes, and a wave holding one runs the long
+// general path beside the plain one: each step therefore queues its general-path nodes
+// in LDS (per-wave segments, positions from ballots) and the whole block takes them
+// after the plain nodes, a few dense waves instead of every wave of the tile.
 template <int ST, int TY, bool SINGLE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F64 ? 1 : 4))) void step2_kernel(
     StepArgs a)
@@ -1142,6 +1158,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
     using R = typename Form<ST>::real;
     constexpr int TX = 64, WX = TX + 2, WY = TY + 2, P2 = TX * TY / 256;
     __shared__ S lds[9][WY][WX];
+    // deferred general-path nodes (ensembles): per-wave segments of step-1 ring indices,
+    // then of step-2 tile indices; SEG covers a wave's share of the larger step
+    constexpr bool DEFER = !SINGLE;
+    constexpr int SEG = 64 * ((WX * WY + 255) / 256);
+    __shared__ int16_t queue[DEFER ? 4 * SEG : 1];
+    __shared__ int qcnt[4];
 
     LatticeDev L;
     const S* src;
@@ -1207,48 +1229,157 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
         rd0.key = step_key(a, lat_id, 0);
         rd1.key = step_key(a, lat_id, 1);
     }
-    // step k on the tile and its ring
-    for (int r = tid; r < WX * WY; r += 256) {
-        const int lx = r % WX, ly = r / WX;
+    // ensembles: ring index r -> LDS position and lattice node (wrapped); false: off a
+    // non-periodic edge
+    auto ring_node = [&](int r, int& lx, int& ly, int& gx, int& gy, bool& own) {
+        lx = r % WX;
+        ly = r / WX;
         const int ux = x0 - 1 + lx, uy = y0 - 1 + ly;  // unwrapped position
-        int gx = ux, gy = uy;
+        gx = ux;
+        gy = uy;
         if (gx < 0 || gx >= L.nx) {
-            if (L.x_edge != EDGE_PERIODIC) continue;
+            if (L.x_edge != EDGE_PERIODIC) return false;
             gx = gx < 0 ? gx + L.nx : gx - L.nx;
         }
         if (gy < 0 || gy >= L.ny) {
-            if (L.y_edge != EDGE_PERIODIC) continue;
+            if (L.y_edge != EDGE_PERIODIC) return false;
             gy = gy < 0 ? gy + L.ny : gy - L.ny;
         }
-        const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny && (SINGLE || ux >= 0);
-        S out[9];
-        if (node_values<ST, !SINGLE>(L, a, c, own ? fs0 : none, gx, gy, rd0, bad, out)) {
+        own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny && ux >= 0;
+        return true;
+    };
+    const int lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    // step k on the tile and its ring
+    if constexpr (!DEFER) {
+        for (int r = tid; r < WX * WY; r += 256) {
+            const int lx = r % WX, ly = r / WX;
+            const int ux = x0 - 1 + lx, uy = y0 - 1 + ly;  // unwrapped position
+            int gx = ux, gy = uy;
+            if (gx < 0 || gx >= L.nx) {
+                if (L.x_edge != EDGE_PERIODIC) continue;
+                gx = gx < 0 ? gx + L.nx : gx - L.nx;
+            }
+            if (gy < 0 || gy >= L.ny) {
+                if (L.y_edge != EDGE_PERIODIC) continue;
+                gy = gy < 0 ? gy + L.ny : gy - L.ny;
+            }
+            const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < L.nx && uy < L.ny;
+            S out[9];
+            if (node_values<ST, false, NV_ALL>(L, a, c, own ? fs0 : none, gx, gy, rd0, bad, out)) {
 #pragma unroll
-            for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
+                for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
+            }
+        }
+    } else {
+        int wcnt = 0;
+        for (int r0 = 0; r0 < WX * WY; r0 += 256) {
+            const int r = r0 + tid;
+            bool defer = false;
+            int lx, ly, gx, gy;
+            bool own;
+            if (r < WX * WY && ring_node(r, lx, ly, gx, gy, own)) {
+                S out[9];
+                const int st = node_values<ST, true, NV_PLAIN>(L, a, c, own ? fs0 : none, gx, gy, rd0, bad, out);
+                if (st > 0) {
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
+                }
+                defer = st < 0;
+            }
+            const uint64_t m = __ballot(defer);
+            if (defer) queue[wave * SEG + wcnt + __popcll(m & below)] = static_cast<int16_t>(r);
+            wcnt += __popcll(m);
+        }
+        if (lane == 0) qcnt[wave] = wcnt;
+        __syncthreads();
+        const int c0 = qcnt[0], c1 = qcnt[1], c2 = qcnt[2], c3 = qcnt[3];
+        for (int k = tid; k < c0 + c1 + c2 + c3; k += 256) {
+            int i = k, w = 0;
+            if (i >= c0) {
+                i -= c0;
+                w = 1;
+                if (i >= c1) {
+                    i -= c1;
+                    w = 2;
+                    if (i >= c2) {
+                        i -= c2;
+                        w = 3;
+                    }
+                }
+            }
+            int lx, ly, gx, gy;
+            bool own;
+            ring_node(queue[w * SEG + i], lx, ly, gx, gy, own);
+            S out[9];
+            if (node_values<ST, true, NV_GENERAL>(L, a, c, own ? fs0 : none, gx, gy, rd0, bad, out)) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j) lds[j][ly][lx] = out[j];
+            }
         }
     }
     __syncthreads();
 
-    // step k+1 on the tile, from LDS (one copy of the loop body: the general path is long)
+    if constexpr (!DEFER) {
+        // step k+1 on the tile, from LDS (one copy of the loop body: the general path is long)
 #pragma unroll 1
-    for (int k = 0; k < P2; ++k) {
-        const int n = tid + 256 * k, tx = n % TX, ty = n / TX, x = x0 + tx, y = y0 + ty;
-        if (x >= L.nx || y >= L.ny || (!SINGLE && x < 0)) continue;
-        uint8_t fl = fl2[0];
-        int32_t lsk = ls2[0];
-        uint16_t wdk = wd2[0];
+        for (int k = 0; k < P2; ++k) {
+            const int n = tid + 256 * k, tx = n % TX, ty = n / TX, x = x0 + tx, y = y0 + ty;
+            if (x >= L.nx || y >= L.ny) continue;
+            uint8_t fl = fl2[0];
+            int32_t lsk = ls2[0];
+            uint16_t wdk = wd2[0];
 #pragma unroll
-        for (int q = 1; q < P2; ++q)
-            if (k == q) {
-                fl = fl2[q];
-                lsk = ls2[q];
-                wdk = wd2[q];
+            for (int q = 1; q < P2; ++q)
+                if (k == q) {
+                    fl = fl2[q];
+                    lsk = ls2[q];
+                    wdk = wd2[q];
+                }
+            const int lx = tx + 1, ly = ty + 1;
+            R d[9];
+            if (fl & CELL_SPECIAL) {
+                if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) continue;
+                const int gx = gathered_x<ST>(L, x), glx = gx - x0 + 1;
+                Pre<ST> p;
+                if (gx == x) {
+                    p.fl = fl;
+                    p.ls = lsk;
+                } else {
+                    const int64_t gidx = L.cell_off + static_cast<int64_t>(y) * L.pitch + gx;
+                    p.fl = a.flags[gidx];
+                    p.ls = a.link_slot[gidx];
+                }
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    p.own[j] = lds[j][ly][glx];
+                    p.pull[j] = lds[j][ly - CY[j]][glx - CX[j]];
+                }
+                LinkVals<R> lg;
+                links_of<ST>(c, p, lg);
+                LinkVals<R> ls_ = lg;
+                if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + lsk], ls_);
+                node_general_d<ST>(L, c, fs1, x, y, fl, p, lg, ls_, wdk, bad, d);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+                    d[j] = Form<ST>::dec(lds[j][ly - CY[j]][lx - CX[j]], a.inv_scale, L.shift[j]);
+                R wd = R(0);
+                if (L.les) wd = wdk == 0xffffu ? R(1e9) : static_cast<R>(wdk);
+                collide<R>(d, L, wd, R(1));
+                if (!(d[0] == d[0])) bad = true;
             }
-        const int lx = tx + 1, ly = ty + 1;
-        R d[9];
-        const bool column = !SINGLE && (fl & CELL_COLUMN) != 0;
-        if ((fl & CELL_SPECIAL) && !column && !(!SINGLE && wrap_only(L, fl, x, y))) {
-            if ((fl & CELL_SOLID) && !(fl & CELL_WRITE)) continue;
+            const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+            S* o = dst + L.f_off + idx;
+            uint32_t h = rd1.on ? rd1.node(L.cell_off + idx) : 0u;
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+                store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j),
+                          a.options & LAUNCH_STREAM_STORES);
+        }
+    } else {
+        // step k+1 at tile node (x, y), from LDS: the general path
+        auto general2 = [&](int x, int y, int lx, int ly, uint8_t fl, int32_t lsk, uint16_t wdk, R d[9]) {
             const int gx = gathered_x<ST>(L, x), glx = gx - x0 + 1;
             Pre<ST> p;
             if (gx == x) {
@@ -1269,23 +1400,81 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
             LinkVals<R> ls_ = lg;
             if (gx != x && (fl & CELL_LINKS)) load_link(c.links[L.link_off + lsk], ls_);
             node_general_d<ST>(L, c, fs1, x, y, fl, p, lg, ls_, wdk, bad, d);
-        } else {
-            // the ring holds the wrapped sources; the outlet column pulls those of x - 1
-            const int slx = (column && x == L.nx - 1 && L.outlet) ? lx - 1 : lx;
+        };
+        auto store2 = [&](int x, int y, const R d[9]) {
+            const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
+            S* o = dst + L.f_off + idx;
+            uint32_t h = rd1.on ? rd1.node(L.cell_off + idx) : 0u;
 #pragma unroll
-            for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec(lds[j][ly - CY[j]][slx - CX[j]], a.inv_scale, L.shift[j]);
-            if (column && x == 0) inlet_closure<R>(L, y, d);
-            R wd = R(0);
-            if (L.les) wd = wdk == 0xffffu ? R(1e9) : static_cast<R>(wdk);
-            collide<R>(d, L, wd, R(1));
-            if (!(d[0] == d[0])) bad = true;
+            for (int j = 0; j < 9; ++j)
+                store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j),
+                          a.options & LAUNCH_STREAM_STORES);
+        };
+        // step k+1 on the tile, from LDS: plain nodes now, general-path nodes queued
+        int wcnt2 = 0;
+#pragma unroll 1
+        for (int k = 0; k < P2; ++k) {
+            const int n = tid + 256 * k, tx = n % TX, ty = n / TX, x = x0 + tx, y = y0 + ty;
+            bool defer = false;
+            if (x < L.nx && y < L.ny && x >= 0) {
+                uint8_t fl = fl2[0];
+                uint16_t wdk = wd2[0];
+#pragma unroll
+                for (int q = 1; q < P2; ++q)
+                    if (k == q) {
+                        fl = fl2[q];
+                        wdk = wd2[q];
+                    }
+                const int lx = tx + 1, ly = ty + 1;
+                const bool column = (fl & CELL_COLUMN) != 0;
+                if ((fl & CELL_SPECIAL) && !column && !wrap_only(L, fl, x, y)) {
+                    defer = !((fl & CELL_SOLID) && !(fl & CELL_WRITE));
+                } else {
+                    // the ring holds the wrapped sources; the outlet column pulls those of x - 1
+                    const int slx = (column && x == L.nx - 1 && L.outlet) ? lx - 1 : lx;
+                    R d[9];
+#pragma unroll
+                    for (int j = 0; j < 9; ++j)
+                        d[j] = Form<ST>::dec(lds[j][ly - CY[j]][slx - CX[j]], a.inv_scale, L.shift[j]);
+                    if (column && x == 0) inlet_closure<R>(L, y, d);
+                    R wd = R(0);
+                    if (L.les) wd = wdk == 0xffffu ? R(1e9) : static_cast<R>(wdk);
+                    collide<R>(d, L, wd, R(1));
+                    if (!(d[0] == d[0])) bad = true;
+                    store2(x, y, d);
+                }
+            }
+            const uint64_t m = __ballot(defer);
+            if (defer) queue[wave * SEG + wcnt2 + __popcll(m & below)] = static_cast<int16_t>(n);
+            wcnt2 += __popcll(m);
         }
-        const int64_t idx = static_cast<int64_t>(y) * L.pitch + x;
-        S* o = dst + L.f_off + idx;
-        uint32_t h = rd1.on ? rd1.node(L.cell_off + idx) : 0u;
-#pragma unroll
-        for (int j = 0; j < 9; ++j)
-            store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j), a.options & LAUNCH_STREAM_STORES);
+        // every thread read the step-1 counts before the barrier that opened step 2
+        if (lane == 0) qcnt[wave] = wcnt2;
+        __syncthreads();
+        const int c0 = qcnt[0], c1 = qcnt[1], c2 = qcnt[2], c3 = qcnt[3];
+        for (int k = tid; k < c0 + c1 + c2 + c3; k += 256) {
+            int i = k, w = 0;
+            if (i >= c0) {
+                i -= c0;
+                w = 1;
+                if (i >= c1) {
+                    i -= c1;
+                    w = 2;
+                    if (i >= c2) {
+                        i -= c2;
+                        w = 3;
+                    }
+                }
+            }
+            const int n = queue[w * SEG + i], tx = n % TX, ty = n / TX, x = x0 + tx, y = y0 + ty;
+            const int64_t idx = L.cell_off + static_cast<int64_t>(y) * L.pitch + x;
+            const uint8_t fl = a.flags[idx];
+            const int32_t lsk = a.link_slot[idx];
+            const uint16_t wdk = L.les ? a.wdist[idx] : uint16_t(0xffffu);
+            R d[9];
+            general2(x, y, tx + 1, ty + 1, fl, lsk, wdk, d);
+            store2(x, y, d);
+        }
     }
     if (bad) atomicOr(a.nan_flags + lat_id, 1);
 }
@@ -2092,6 +2281,8 @@ hipError_t launch_step2_st(int ty, int n_blocks, const StepArgs& a, hipStream_t 
         hipLaunchKernelGGL((step2_kernel<ST, 4, SINGLE>), dim3(n_blocks), dim3(256), 0, s, a);
     } else if (ty == 8) {
         hipLaunchKernelGGL((step2_kernel<ST, 8, SINGLE>), dim3(n_blocks), dim3(256), 0, s, a);
+    } else if (ty == 16) {
+        hipLaunchKernelGGL((step2_kernel<ST, 16, SINGLE>), dim3(n_blocks), dim3(256), 0, s, a);
     } else {
         return hipErrorInvalidValue;
     }

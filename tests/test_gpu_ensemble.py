@@ -18,9 +18,9 @@ pytestmark = pytest.mark.gpu
 
 NAMES = ["cylinder", "cavity", "step", "plate", "rotating", "bouzidi", "les", "bgk", "hills", "near_wall"]
 # (block_x, rows, nodes per thread, options, graph_steps or 0): one-step shapes with and
-# without XCD order and streaming stores, then two steps per launch on 64 x 4 / 64 x 8
+# without XCD order and streaming stores, then two steps per launch on 64 x 4 / 8 / 16
 LAUNCHES = [(256, 1, 1, 0, 10), (64, 1, 1, 3, 10), (128, 1, 2, 1, 0), (256, 2, 1, 2, 10),
-            (64, 4, 1, 4, 10), (64, 8, 1, 5, 10), (64, 8, 1, 6, 0), (64, 4, 1, 7, 10)]
+            (64, 4, 1, 4, 10), (64, 8, 1, 5, 10), (64, 8, 1, 6, 0), (64, 4, 1, 7, 10), (64, 16, 1, 5, 10)]
 
 
 def _lattices():
@@ -103,7 +103,7 @@ def test_ensemble_stochastic_rounding_repeats_in_every_launch_form(storage):
     one-step and two-step launches, graphs and single launches give the same bits."""
     cs = _lattices()
     ref = None
-    for launch in ((256, 1, 1, 0, 10), (64, 8, 1, 5, 6), (64, 4, 1, 6, 0), (64, 1, 4, 3, 10)):
+    for launch in ((256, 1, 1, 0, 10), (64, 8, 1, 5, 6), (64, 4, 1, 6, 0), (64, 1, 4, 3, 10), (64, 16, 1, 7, 6)):
         ens, _ = _ensemble(cs, storage)
         ens.set_stochastic_rounding(True, 77)
         _set(ens, launch)

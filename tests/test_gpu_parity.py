@@ -132,10 +132,10 @@ def test_autotune_keeps_state():
     ens.step(3)
     before = ens.download(0)
     (bx, rows, nodes, options), ms = ens.autotune()
-    assert bx in (64, 128, 256, 512) and rows in (1, 2, 4, 8) and nodes in (1, 2, 4, 8) and ms > 0
+    assert bx in (64, 128, 256, 512) and rows in (1, 2, 4, 8, 16) and nodes in (1, 2, 4, 8) and ms > 0
     assert options in range(8)  # XCD order, streaming stores, two steps per launch
     if options & 4:
-        assert (bx, nodes) == (64, 1) and rows in (4, 8)
+        assert (bx, nodes) == (64, 1) and rows in (4, 8, 16)
     np.testing.assert_array_equal(ens.download(0), before)
     ens.step(5)
     o.step(c.tau, c.u, 8)
@@ -599,7 +599,7 @@ def test_two_steps_per_launch_is_bitwise_identical(name, storage):
     a, _ = build_gpu(c, storage)
     b, _ = build_gpu(c, storage)
     a.set_launch(64, 1, 1, 0)
-    for rows, opt in ((4, 4), (8, 5), (8, 6), (4, 7)):
+    for rows, opt in ((4, 4), (8, 5), (8, 6), (4, 7), (16, 5)):
         try:
             b.set_launch(64, rows, 1, opt, graph_steps=10)
         except G.LbmError:

@@ -51,15 +51,16 @@ def test_poisoned_destination_16bit(name, storage):
     _twin_runs(lambda: build_gpu(c, storage)[0], 5)
 
 
+@pytest.mark.parametrize("rows", [4, 16])
 @pytest.mark.parametrize("name", ["cylinder", "bouzidi", "step"])
-def test_poisoned_destination_two_steps_per_launch(name):
+def test_poisoned_destination_two_steps_per_launch(name, rows):
     """Calls of 3 steps: a two-step launch, then a one-step launch; the poison precedes
     the two-step launch of each call."""
     c = CASES[name]
 
     def make():
         e, _ = build_gpu(c, "f32")
-        e.set_launch(64, 4, 1, 4)
+        e.set_launch(64, rows, 1, 4)
         return e
     _twin_runs(make, 4, steps_per_call=3)
 
