@@ -80,13 +80,14 @@ template <> struct Form<ST_F16S> {
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(h)); }
     static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
     static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
-    // Stochastic rounding: add the top 13 bits of rnd below the last kept mantissa bit,
-    // then truncate; exact in the binary16 normal range (rounded to nearest below it).
+    // Stochastic rounding (v_cvt_sr_f16_f32): in the binary16 normal range the top 13
+    // bits of rnd are added below the last kept mantissa bit and the sum truncated; below
+    // it the instruction keeps rounding stochastically at the subnormal spacing.
     static __device__ __forceinline__ store enc_sr_raw(real h, uint32_t rnd)
     {
-        uint32_t b = __float_as_uint(h);
-        if ((b & 0x7f800000u) != 0x7f800000u) b += rnd >> 19;
-        return enc_raw(__uint_as_float(b & ~0x1fffu));
+        typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+        const half2_t v = __builtin_amdgcn_cvt_sr_f16_f32(half2_t{0, 0}, h, rnd, false);
+        return __builtin_bit_cast(uint16_t, v.x);
     }
 };
 template <> struct Form<ST_BF16S> {
@@ -98,12 +99,13 @@ template <> struct Form<ST_BF16S> {
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(h)); }
     static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
     static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
-    // Stochastic rounding: add the top 16 bits of rnd below the last kept bit, truncate.
+    // Stochastic rounding (v_cvt_sr_bf16_f32): the top 16 bits of rnd are added below the
+    // last kept bit and the sum truncated; infinities stay, NaN stays NaN.
     static __device__ __forceinline__ store enc_sr_raw(real h, uint32_t rnd)
     {
-        uint32_t b = __float_as_uint(h);
-        if ((b & 0x7f800000u) != 0x7f800000u) b += rnd >> 16;
-        return static_cast<uint16_t>(b >> 16);
+        typedef short short2_t __attribute__((ext_vector_type(2)));
+        const short2_t v = __builtin_amdgcn_cvt_sr_bf16_f32(short2_t{0, 0}, h, rnd, false);
+        return static_cast<uint16_t>(v.x);
     }
 };
 
@@ -241,11 +243,24 @@ __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist
             R cs2 = static_cast<R>(L.cs2);
             const R nu = static_cast<R>(L.nu);
             if (nu > R(0)) {
-                const R y_plus = wall_dist * dsqrt<R>(fma_<R>(u, u, fma_<R>(v, v, R(1e-12)))) * static_cast<R>(L.inv_nu);
-                if (y_plus > R(0)) {
-                    const R damp = R(1) - dexp<R>(y_plus * R(-1.0 / 26.0));
-                    const R csd = static_cast<R>(L.cs) * damp;
+                const R uu = fma_<R>(u, u, fma_<R>(v, v, R(1e-12)));
+                const R inu = static_cast<R>(L.inv_nu);
+                // Far from walls the damping 1 - exp(-y+/26) rounds to exactly 1 (once
+                // y+/26 > 17.4 in FP32, 37.4 in FP64): the same bits without the square
+                // root and the exponential.  The test on y+^2 is exact enough -- either
+                // side of its threshold gives damp = 1 there.
+                constexpr double kFar = sizeof(R) == 4 ? 18.0 * 26.0 : 38.0 * 26.0;
+                const R wi = wall_dist * inu;
+                if (wi * wi * uu > R(kFar * kFar)) {
+                    const R csd = static_cast<R>(L.cs);
                     cs2 = csd * csd;
+                } else {
+                    const R y_plus = wall_dist * dsqrt<R>(uu) * inu;
+                    if (y_plus > R(0)) {
+                        const R damp = R(1) - dexp<R>(y_plus * R(-1.0 / 26.0));
+                        const R csd = static_cast<R>(L.cs) * damp;
+                        cs2 = csd * csd;
+                    }
                 }
             }
             // A = 9 cs^2 Q / (2 rho); Q/rho is scale free
