@@ -21,7 +21,8 @@ duration measured here with HIP events on the library's stream (torch events rec
 the stream the library launches on): the difference between replays of hipGraphs of 40
 and of 20 steps, which differ by exactly 10 two-step launches (or 20 one-step launches),
 against 8 TB/s.  ``traffic`` is the HBM bytes per launch read from the committed
-rocprofv3 PMC summary of the same workload and launch shape.
+rocprofv3 PMC summary of the same workload and launch shape (null when the autotuner
+picks a shape the summary does not hold).
 
 ``forms`` (same JSON line): the 16-bit storage forms on the same domain (stochastic
 rounding, the form that holds Cd/St within 1% of FP64, then round-to-nearest), FP32 with
@@ -407,12 +408,14 @@ print(json.dumps({{'steps': n, 'seconds': total, 'threads': int(os.environ['OMP_
             "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "cpu_model": model}
 
 
-def profiled_traffic(path=LARGE_PROFILE):
-    """HBM bytes per launch of the headline kernel from the committed rocprofv3 PMC
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration)."""
+def profiled_traffic(launch, path=LARGE_PROFILE):
+    """HBM bytes per launch of the headline kernel in this launch shape from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration);
+    None when the summary holds no figure for the shape."""
+    key = "traffic_bytes_per_launch[" + ",".join(str(int(v)) for v in launch) + "]:"
     try:
         for line in open(path):
-            if line.strip().startswith("traffic_bytes_per_launch:"):
+            if line.strip().startswith(key):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -555,8 +558,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(),
-            "kernel": "step2_kernel<F32, tile rows>" if two else "step_kernel<F32, ...>",
+            "traffic": profiled_traffic(head["launch"]),
+            "kernel": (f"step2_kernel<1, {head['launch'][1]}, true>" if two else
+                       "step_kernel<1, {}, {}, {}, true>".format(*head["launch"][:3])),
             "steps_per_launch": head["steps_per_launch"],
             "launch_ms": head["launch_ms"],
             "bytes_per_launch": bytes_launch,

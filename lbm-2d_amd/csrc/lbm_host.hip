@@ -583,10 +583,12 @@ int commit(Ensemble* E)
     int rc;
     if (E->geom_dirty) {
         bool any = false;
-        for (auto& H : E->lat)
-            if (H.geom_dirty) {
-                if ((rc = build_geometry(E, H)) != LBM_OK) return rc;
+        std::vector<char> rebuilt(E->lat.size(), 0);
+        for (size_t l = 0; l < E->lat.size(); ++l)
+            if (E->lat[l].geom_dirty) {
+                if ((rc = build_geometry(E, E->lat[l])) != LBM_OK) return rc;
                 any = true;
+                rebuilt[l] = 1;
             }
         if (any) {
             size_t nrec = 0;
@@ -656,11 +658,14 @@ int commit(Ensemble* E)
                     LBM_HIP(hipStreamSynchronize(E->stream));
                     drop_new_geometry(H);
                 }
-            // wall distance of every lattice from its flags, on the device (read by the
-            // LES lattices' collisions and by lbm_wall_distance)
-            for (auto& H : E->lat)
+            // wall distance of each lattice just built from its flags, on the device (read
+            // by the LES lattices' collisions and by lbm_wall_distance)
+            for (size_t l = 0; l < E->lat.size(); ++l) {
+                if (!rebuilt[l]) continue;
+                const HostLattice& H = E->lat[l];
                 LBM_HIP(launch_wall_distance(E->d_flags + H.cell_off, E->d_wdist + H.cell_off, H.d.nx, H.d.ny,
                                              H.pitch, E->stream));
+            }
             if (slots != E->total_slots) {
                 E->total_slots = slots;
                 const size_t dummy_n = static_cast<size_t>(slots) * kAccCopies * 2;

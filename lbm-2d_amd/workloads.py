@@ -119,9 +119,11 @@ def build_ensemble(lats, storage="f32", stream=None, scale_exp=8, links=None):
     (links[l] when given, else computed) and the uniform start feq(1, 0.1, 0) built on
     the device."""
     ens = G.Ensemble([d for d, _, _ in lats], storage=storage, stream=stream, scale_exp=scale_exp)
+    # every geometry first, so that the first initialisation builds them in one commit
     for l, (d, mask, circles) in enumerate(lats):
         ens.set_obstacle(l, mask)
         ens.set_links(l, links[l] if links is not None else ensemble_links(mask, circles, d.nx, d.ny))
+    for l in range(len(lats)):
         ens.init_equilibrium(l, 1.0, U_IN, 0.0)
     return ens
 
