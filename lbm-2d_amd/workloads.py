@@ -133,7 +133,7 @@ def build_ensemble(lats, storage="f32", stream=None, scale_exp=8, links=None):
 LARGE = dict(nx=32768, ny=16384, tau=0.505, u=U_IN, cs=CS_DEFAULT, n_circles=64, n_plates=64, seed=42)
 
 
-def large_domain(seed=42, nx=32768, ny=16384, n_circles=64, n_plates=64, with_shapes=False):
+def large_domain(seed=42, nx=32768, ny=16384, n_circles=64, n_plates=64, with_shapes=False, scheme=G.BB_MEI):
     """Obstacles of the large domain: n_circles circles r ~ U[20,60] (exact-circle Mei
     links, force slots 1..n_circles) and n_plates flat plates 155 x 2 at an angle of
     attack ~ U[-15, 15] deg (polygon staircase, simple bounce-back, force slot 0).
@@ -174,7 +174,7 @@ def large_domain(seed=42, nx=32768, ny=16384, n_circles=64, n_plates=64, with_sh
                 hit = (sub[ys + G.CYV[i], xs + G.CXV[i]] != 0) & (np.sqrt(sx * sx + sy * sy) < r)
                 for yl, xl in zip(ys[hit], xs[hit]):
                     x, y = x0 + int(xl), y0 + int(yl)
-                    links.append((y * nx + x, i, G.BB_MEI, 1 + k, q_circle(cx, cy, r, x, y, i), 0.0, 0.0))
+                    links.append((y * nx + x, i, scheme, 1 + k, q_circle(cx, cy, r, x, y, i), 0.0, 0.0))
         else:
             xy = G.plate_polygon(float(cx), float(cy), 155.0, aoa, 2.0)
             mask[y0:y0 + h, x0:x0 + w] |= G.polygon_mask(w, h, xy - np.array([x0, y0], float))
