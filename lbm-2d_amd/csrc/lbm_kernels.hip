@@ -1250,6 +1250,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
         lx = r % WX;
         ly = r / WX;
         const int ux = x0 - 1 + lx, uy = y0 - 1 + ly;  // unwrapped position
+        // no tile node has a neighbour further than one node off the lattice (positions
+        // there exist when the lattice is narrower or lower than the tile and its ring)
+        if (ux < -1 || ux > L.nx || uy < -1 || uy > L.ny) return false;
         gx = ux;
         gy = uy;
         if (gx < 0 || gx >= L.nx) {
@@ -1270,6 +1273,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
         for (int r = tid; r < WX * WY; r += 256) {
             const int lx = r % WX, ly = r / WX;
             const int ux = x0 - 1 + lx, uy = y0 - 1 + ly;  // unwrapped position
+            // no tile node has a neighbour further than one node off the lattice
+            if (ux > L.nx || uy > L.ny) continue;
             int gx = ux, gy = uy;
             if (gx < 0 || gx >= L.nx) {
                 if (L.x_edge != EDGE_PERIODIC) continue;

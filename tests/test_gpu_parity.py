@@ -456,6 +456,70 @@ def test_tiny_and_thin_lattices_match_oracle(case, shape):
     assert err <= TOL["f64"]
 
 
+TINY_SHAPES = [(3, 3), (5, 200), (200, 3), (65, 4), (130, 7)]
+
+
+def _tiny_lattice(case, nx, ny):
+    import oracle as O
+    kind = {"cylinder": (G.CYLINDER, O.CYLINDER, 0.05), "hills": (G.PERIODIC_HILLS, O.PERIODIC_HILLS, 0.0)}[case]
+    mask = np.zeros((ny, nx), np.uint8)
+    if nx >= 5 and ny >= 5:
+        mask[ny // 2, nx // 2] = 1
+    desc = G.desc_for_case(kind[0], nx, ny, tau=0.7, u_in=kind[2])
+    o = O.Oracle(nx, ny, kind[1])
+    o.set_obstacle(mask)
+    o.init_equilibrium(1.0, kind[2], 0.0)
+    rng = np.random.default_rng(nx * 1000 + ny)
+    o.f[...] = o.f * (1.0 + 0.01 * rng.standard_normal(o.f.shape)) * (mask == 0)[..., None]
+    return desc, mask, o, kind[2]
+
+
+@pytest.mark.parametrize("rows", [8, 16])
+@pytest.mark.parametrize("case", ["cylinder", "hills"])
+def test_tiny_and_thin_lattices_two_steps_per_launch(case, rows):
+    """Lattices lower or narrower than a two-step tile and its ring (the ring's positions
+    more than one node off a periodic edge are skipped): each alone where the one-lattice
+    kernel takes it, and all in one ensemble, two steps per launch, equal bit for bit to
+    one step per launch and to the FP64 oracle within its bound."""
+    lats = [_tiny_lattice(case, nx, ny) for nx, ny in TINY_SHAPES]
+    ref = []
+    for desc, mask, o, u in lats:
+        e = G.Ensemble([desc], storage="f64")
+        e.set_obstacle(0, mask)
+        e.upload(0, o.f.copy())
+        e.step(12)
+        ref.append(e.download(0))
+        e.close()
+    alone = 0
+    for (desc, mask, o, u), r in zip(lats, ref):
+        e = G.Ensemble([desc], storage="f64")
+        e.set_obstacle(0, mask)
+        e.upload(0, o.f.copy())
+        try:
+            e.set_launch(64, rows, 1, 4, graph_steps=6)
+        except G.LbmError:
+            continue  # an outlet with nx % 64 == 1: the ensemble below takes it
+        e.step(5)
+        e.step(7)
+        np.testing.assert_array_equal(e.download(0), r, err_msg=f"{desc.nx}x{desc.ny} alone")
+        alone += 1
+        e.close()
+    assert alone >= 3
+    ens = G.Ensemble([d for d, _, _, _ in lats], storage="f64")
+    for l, (desc, mask, o, u) in enumerate(lats):
+        ens.set_obstacle(l, mask)
+        ens.upload(l, o.f.copy())
+    ens.set_launch(64, rows, 1, 5, graph_steps=6)
+    ens.step(5)
+    ens.step(7)
+    for l, ((desc, mask, o, u), r) in enumerate(zip(lats, ref)):
+        np.testing.assert_array_equal(ens.download(l), r, err_msg=f"{desc.nx}x{desc.ny} in the ensemble")
+        for _ in range(12):
+            o.step(0.7, u)
+        assert np.abs(r - o.f).max() <= TOL["f64"]
+    ens.close()
+
+
 def _q_circle(ccx, ccy, r, xf, yf, i):
     """q of a link to a circle (q_cylinder, src/lbm_types.hpp:118-138), FP64."""
     dx, dy = xf - ccx, yf - ccy

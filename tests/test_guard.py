@@ -130,3 +130,65 @@ def test_layout_matches_the_stated_rule():
     assert guard == 1024 + 256
     assert (9 * 1024 * 512 * 4) % (2 << 20) == 0  # the exact-size case fills whole pages
     assert G.population_layout(_descs([(1024, 512)]))[1] == 9 * 1024 * 512
+
+
+# ---- the two-step kernel's ring ------------------------------------------------------
+#
+# step2_kernel computes its tile's 66 x (rows + 2) positions (tile and one-node ring) at
+# lattice nodes: positions off a periodic edge wrap once, positions off another edge are
+# skipped, and positions further than one node off the lattice are skipped (no tile node
+# has a neighbour there).  Those nodes' own loads are the in-lattice nodes checked above,
+# so the ring is safe when every position it computes is a lattice node.
+
+def _ring_nodes(d, rows, ensemble, skip_far=True):
+    nx, ny = d.nx, d.ny
+    x_start = -63 if (ensemble and d.outlet and nx % 64 == 1) else 0
+    LX, LY = np.meshgrid(np.arange(66), np.arange(rows + 2))
+    gxs, gys = [], []
+    for y0 in range(0, ny, rows):
+        for x0 in range(x_start, nx, 64):
+            ux, uy = x0 - 1 + LX, y0 - 1 + LY
+            keep = np.ones(ux.shape, bool)
+            if skip_far:
+                keep &= (ux >= -1) & (ux <= nx) & (uy >= -1) & (uy <= ny)
+            gx, gy = ux.copy(), uy.copy()
+            if d.x_edge == G.EDGE_PERIODIC:
+                gx = np.where(gx < 0, gx + nx, np.where(gx >= nx, gx - nx, gx))
+            else:
+                keep &= (gx >= 0) & (gx < nx)
+            if d.y_edge == G.EDGE_PERIODIC:
+                gy = np.where(gy < 0, gy + ny, np.where(gy >= ny, gy - ny, gy))
+            else:
+                keep &= (gy >= 0) & (gy < ny)
+            gxs.append(gx[keep])
+            gys.append(gy[keep])
+    return np.concatenate(gxs), np.concatenate(gys)
+
+
+RING_SIZES = [(3, 3), (65, 3), (200, 3), (3, 200), (64, 17), (129, 48), (130, 9), (1, 1)]
+
+
+@pytest.mark.parametrize("ensemble", [False, True])
+@pytest.mark.parametrize("rows", [4, 8, 16])
+@pytest.mark.parametrize("edges", [(G.EDGE_OPEN, G.EDGE_PERIODIC), (G.EDGE_PERIODIC, G.EDGE_PERIODIC),
+                                   (G.EDGE_WALL, G.EDGE_WALL)])
+def test_two_step_ring_positions_are_lattice_nodes(edges, rows, ensemble):
+    for nx, ny in RING_SIZES:
+        if nx < 3 or ny < 3:
+            continue  # below the library's minimum lattice
+        (d,) = _descs([(nx, ny)], edges)
+        gx, gy = _ring_nodes(d, rows, ensemble)
+        assert gx.size > 0
+        assert gx.min() >= 0 and gx.max() < nx, (nx, ny, rows, edges)
+        assert gy.min() >= 0 and gy.max() < ny, (nx, ny, rows, edges)
+
+
+def test_two_step_ring_needs_the_far_positions_skipped():
+    """Wrapping once is not enough on a periodic lattice lower or narrower than the tile
+    and its ring: without the skip, positions fall outside the lattice."""
+    (d,) = _descs([(200, 3)], (G.EDGE_OPEN, G.EDGE_PERIODIC))
+    _, gy = _ring_nodes(d, 8, False, skip_far=False)
+    assert gy.max() >= 3
+    (d,) = _descs([(3, 3)], (G.EDGE_PERIODIC, G.EDGE_PERIODIC))
+    gx, _ = _ring_nodes(d, 4, False, skip_far=False)
+    assert gx.max() >= 3
