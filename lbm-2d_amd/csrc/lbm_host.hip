@@ -153,6 +153,16 @@ struct Ensemble {
     Tile* d_tiles2 = nullptr;      // ensembles with two steps per launch: 64 x rows tiles
     size_t tiles2_cap = 0;
     int n_tiles2 = 0;
+    // two-step launches of a one-lattice handle, per tile height (8, 16, 32): the bulk
+    // tiles (step2_bulk_kernel) and then the others (step2_kernel on 64 x rest_rows tiles,
+    // rest_rows = min(rows, 16)), origins in one device list, classified from the flags
+    struct BulkPlan {
+        int rows = 0;  // 0: not built
+        int rest_rows = 0;
+        TileXY* d = nullptr;
+        int n_bulk = 0, n_rest = 0;
+    };
+    BulkPlan plan[3];
     int64_t* d_steps = nullptr;
     int32_t* d_par = nullptr;
     double* d_acc = nullptr;       // [acc_blocks][kAccCopies][total_slots][2]
@@ -537,12 +547,20 @@ LatticeDev make_dev(const HostLattice& H, int index, float scale)
     // Fscale (src/lbm.hpp:296-297)
     L.fscale = d.body_force_x != 0.0 ? (1.0 - 1.0 / (2.0 * d.tau)) * 3.0 * d.body_force_x : 0.0;
     // storage shift feq_i(1, u_s, v_s) - w_i (compute_equilibrium, src/lbm_types.hpp:269-273)
+    L.shifted = 0;
     for (int i = 0; i < 9; ++i) {
         const double w = i == 0 ? 4.0 / 9.0 : (i < 5 ? 1.0 / 9.0 : 1.0 / 36.0);
         const double eu = CX[i] * H.shift_u + CY[i] * H.shift_v;
         const double uu = H.shift_u * H.shift_u + H.shift_v * H.shift_v;
         L.shift[i] = static_cast<float>(w * (3.0 * eu + 4.5 * eu * eu - 1.5 * uu));
+        if (L.shift[i] != 0.0f) L.shifted = 1;
+    }
+    for (int i = 0; i < 9; ++i) {
+        // no shift: -0.0f both ways (adding -0 leaves every value, -0 included, as it is)
+        if (!L.shifted) L.shift[i] = -0.0f;
         L.shift_s[i] = L.shift[i] * scale;  // exact: scale is a power of two
+        L.nshift[i] = L.shifted ? -L.shift[i] : -0.0f;
+        L.nshift_s[i] = L.shifted ? -L.shift_s[i] : -0.0f;
     }
     return L;
 }
@@ -577,6 +595,92 @@ void one_step_shape(const Ensemble* E, int s[4])
 // the 64-wide tiles would hold only the outlet column (its outflow copy gathers the
 // column before it, which must lie inside the tile).
 int two_step_x_start(const lbm_desc& d) { return d.outlet && d.nx % 64 == 1 ? -63 : 0; }
+
+int plan_slot(int rows) { return rows == 8 ? 0 : (rows == 16 ? 1 : (rows == 32 ? 2 : -1)); }
+
+// Whether two-step launches of tile height rows split into bulk and other tiles: a
+// one-lattice momentum handle, FP32 or a 16-bit form, rows 8, 16 or 32, and rows short
+// enough for the bulk kernel's 32-bit offsets within a tile's ring and 24-bit row products.
+bool bulk_eligible(const Ensemble* E, int rows)
+{
+    if (E->lat.size() != 1 || E->thermal || !bulk_supported(E->storage, rows)) return false;
+    const int64_t row_bytes = static_cast<int64_t>(E->lat[0].pitch) * E->bpp;
+    return (rows + 3) * row_bytes < (int64_t(1) << 31) && E->lat[0].pitch < (1 << 24);
+}
+
+void drop_plans(Ensemble* E)
+{
+    for (auto& p : E->plan) {
+        if (p.d) (void)hipFree(p.d);
+        p = Ensemble::BulkPlan{};
+    }
+}
+
+// Classifies the 64 x rows tiles of the one-lattice handle from its device flags and
+// uploads the bulk tiles' origins, then those of the other tiles cut to rest_rows rows.
+int build_plan(Ensemble* E, int rows)
+{
+    const HostLattice& H = E->lat[0];
+    const lbm_desc& d = H.d;
+    Ensemble::BulkPlan& P = E->plan[plan_slot(rows)];
+    if (P.d) (void)hipFree(P.d);
+    P = Ensemble::BulkPlan{};
+    const int tiles_x = (d.nx + 63) / 64, tiles_y = (d.ny + rows - 1) / rows;
+    const size_t n = static_cast<size_t>(tiles_x) * tiles_y;
+    uint8_t* d_cls = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&d_cls), n) != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "hipMalloc: two-step tile classes");
+    std::vector<uint8_t> cls(n);
+    hipError_t e = launch_tile_class(E->d_flags + H.cell_off, d.nx, d.ny, H.pitch, tiles_x, tiles_y, rows, d_cls,
+                                     E->stream);
+    int rc = e == hipSuccess ? d2h(E, cls.data(), d_cls, n)
+                             : fail(LBM_ERR_DEVICE, std::string("tile classes: ") + hipGetErrorString(e));
+    (void)hipFree(d_cls);
+    if (rc != LBM_OK) return rc;
+    const int rest_rows = std::min(rows, 16);
+    std::vector<TileXY> bulk, rest;
+    for (int ty = 0; ty < tiles_y; ++ty)
+        for (int tx = 0; tx < tiles_x; ++tx) {
+            const int x0 = tx * 64, y0 = ty * rows;
+            if (cls[static_cast<size_t>(ty) * tiles_x + tx]) {
+                bulk.push_back(TileXY{x0, y0});
+            } else {
+                for (int y = y0; y < std::min(y0 + rows, d.ny); y += rest_rows) rest.push_back(TileXY{x0, y});
+            }
+        }
+    std::vector<TileXY> all(bulk);
+    all.insert(all.end(), rest.begin(), rest.end());
+    if ((rc = dev_alloc(E, &P.d, all.size())) != LBM_OK) return rc;
+    if ((rc = h2d(E, P.d, all.data(), all.size() * sizeof(TileXY))) != LBM_OK) return rc;
+    P.rows = rows;
+    P.rest_rows = rest_rows;
+    P.n_bulk = static_cast<int>(bulk.size());
+    P.n_rest = static_cast<int>(rest.size());
+    return LBM_OK;
+}
+
+// The plan of the handle's two-step tile height, or null (not split or not built).
+const Ensemble::BulkPlan* current_plan(const Ensemble* E)
+{
+    if (!(E->options & LAUNCH_TWO_STEPS) || !bulk_eligible(E, E->rows)) return nullptr;
+    const Ensemble::BulkPlan& P = E->plan[plan_slot(E->rows)];
+    return P.rows == E->rows ? &P : nullptr;
+}
+
+// One two-step launch of a batch: on a one-lattice handle with a plan, the bulk tiles and
+// then the others; else the grid of tiles (one lattice) or the ensemble's tile table.
+hipError_t launch_pair(Ensemble* E, const StepArgs& a2, bool single, int blocks2)
+{
+    const Ensemble::BulkPlan* P = single ? current_plan(E) : nullptr;
+    if (!P) return launch_step2(E->storage, E->rows, single, blocks2, a2, E->stream);
+    StepArgs ab = a2;
+    ab.tile_list = P->d;
+    hipError_t e = launch_step2_bulk(E->storage, P->rows, P->n_bulk, ab, E->stream);
+    if (e != hipSuccess || P->n_rest == 0) return e;
+    StepArgs ar = a2;
+    ar.tile_list = P->d + P->n_bulk;
+    return launch_step2(E->storage, P->rest_rows, true, P->n_rest, ar, E->stream);
+}
 
 int commit(Ensemble* E)
 {
@@ -646,6 +750,7 @@ int commit(Ensemble* E)
             }
             if ((rc = h2d(E, E->d_special, special.data(), special.size() * 4)) != LBM_OK) return rc;
             E->graph_dirty = true;
+            drop_plans(E);  // the tile classes follow the flags
             // flags and link slots of the lattices just built, device to device (the others
             // are in place)
             for (auto& H : E->lat)
@@ -742,6 +847,10 @@ int commit(Ensemble* E)
         }
         E->n_tiles2 = static_cast<int>(t2.size());
         E->tiles_dirty = false;
+        E->graph_dirty = true;
+    }
+    if ((E->options & LAUNCH_TWO_STEPS) && bulk_eligible(E, E->rows) && E->plan[plan_slot(E->rows)].rows != E->rows) {
+        if ((rc = build_plan(E, E->rows)) != LBM_OK) return rc;
         E->graph_dirty = true;
     }
     if (E->counters_stale) {
@@ -855,12 +964,16 @@ int two_step_pairs(Ensemble* E, int n, int l)
 }
 
 // Whether two steps per launch apply: tile 64 x (4, 8 or 16) with 256 threads (block_x 64,
-// one node per thread); on a one-lattice handle the outflow copy's source column (nx-2
-// and its neighbours) inside the last tile's ring (ensembles right-align such rows).
+// one node per thread), or 64 x 32 where the tiles split into bulk and other tiles; on a
+// one-lattice handle the outflow copy's source column (nx-2 and its neighbours) inside the
+// last tile's ring (ensembles right-align such rows).
 int two_step_ok(Ensemble* E, int bx, int rows, int vec)
 {
-    if (bx != 64 || vec != 1 || (rows != 4 && rows != 8 && rows != 16))
-        return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1), (64, 8, 1) and (64, 16, 1)");
+    if (bx != 64 || vec != 1 || (rows != 4 && rows != 8 && rows != 16 && rows != 32))
+        return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1), (64, 8, 1), (64, 16, 1) and "
+                                     "(64, 32, 1)");
+    if (rows == 32 && !bulk_eligible(E, 32))
+        return fail(LBM_ERR_INVALID, "two steps per launch on 64 x 32 tiles: one lattice in FP32 or a 16-bit form");
     for (const auto& H : E->lat) {
         const lbm_desc& d = H.d;
         if (E->lat.size() == 1 && d.outlet && d.nx % 64 == 1)
@@ -954,7 +1067,7 @@ int launch_batch(Ensemble* E, int n, int l, double* acc_base = nullptr, double* 
                 a2.nf2 = terms_of(2 * i + 1);
                 a2.step_arg = 2 * i;
                 a2.flips = swaps;
-                LBM_HIP(launch_step2(E->storage, E->rows, single, blocks2, a2, E->stream));
+                LBM_HIP(launch_pair(E, a2, single, blocks2));
             }
         }
         int s1[4];
@@ -1414,6 +1527,7 @@ int lbm_destroy(lbm_handle h)
                     E->d_dummy_nf, E->d_slot_start, E->d_slot_recs, E->gbuf[0], E->gbuf[1], E->d_tcode};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    lbm::drop_plans(E);
     for (auto& H : E->lat) lbm::drop_new_geometry(H);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
     delete h;
@@ -1848,7 +1962,8 @@ int lbm_step_noalloc(lbm_handle h, int32_t n)
     if ((rc = require_uploaded(E)) != LBM_OK) return rc;
     // everything commit() would write must be on the device already: it may allocate
     if (E->geom_dirty || E->desc_dirty || E->counters_stale || (E->tiles_dirty && E->lat.size() > 1) ||
-        !E->d_slot_start || !E->d_hist)
+        !E->d_slot_start || !E->d_hist ||
+        ((E->options & LAUNCH_TWO_STEPS) && bulk_eligible(E, E->rows) && !current_plan(E)))
         return fail(LBM_ERR_INVALID, "lbm_step_noalloc: the handle has changes not yet on the device (geometry, "
                                      "parameters, launch shape or an upload): call lbm_step(h, 0) first");
     if (n == 0) return LBM_OK;
@@ -2077,7 +2192,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         }
     }
     // two steps per launch: per-step time of each tile height
-    for (int si = 0; si < 12 && rc == LBM_OK; ++si) {
+    for (int si = 0; si < 16 && rc == LBM_OK; ++si) {
         const int rows = 4 << (si / 4), opt = LAUNCH_TWO_STEPS | (si & 3);
         if (two_step_ok(E, 64, rows, 1) != LBM_OK) continue;
         E->bx = 64;
@@ -2106,7 +2221,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
         hipError_t le = hipSuccess;
         for (int w = 0; w < 2 && le == hipSuccess; ++w, ++n) {
             ping(a, single, n);
-            le = launch_step2(E->storage, rows, single, blocks, a, E->stream);
+            le = launch_pair(E, a, single, blocks);
         }
         float ms = 0.f;
         bool ok = le == hipSuccess;
@@ -2114,7 +2229,7 @@ int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* be
             (void)hipEventRecord(e0, E->stream);
             for (int r = 0; r < reps && le == hipSuccess; ++r, ++n) {
                 ping(a, single, n);
-                le = launch_step2(E->storage, rows, single, blocks, a, E->stream);
+                le = launch_pair(E, a, single, blocks);
             }
             (void)hipEventRecord(e1, E->stream);
             ok = le == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
@@ -2221,6 +2336,10 @@ int lbm_get_info(lbm_handle h, lbm_info* out)
     r.rows = h->rows;
     r.nodes_per_thread = h->vec;
     r.launch_options = h->options;
+    if (const Ensemble::BulkPlan* P = current_plan(h)) {
+        r.bulk_tiles = P->n_bulk;
+        r.other_tiles = P->n_rest;
+    }
     *out = r;
     return LBM_OK;
 }

@@ -38,9 +38,14 @@ struct alignas(16) LatticeDev {
     double s_shear;                    // MRT shear rate; s_bulk = 1.2, s_normal = 1 are fixed
                                        // (MRTParams::from_tau, src/lbm_types.hpp:72-76)
     double tau, inv_tau, u_in, cs, cs2, nu, inv_nu, fscale;
-    // 16-bit storage shift: stored h_i = 2^k (f_i - w_i - shift[i]); shift_s = 2^k shift
+    // 16-bit storage shift: stored h_i = 2^k (f_i - w_i - shift[i]); shift_s = 2^k shift.
+    // Decoding adds shift, encoding adds nshift = -shift.  Without a shift (shifted == 0)
+    // both hold -0.0f, for which x + (-0) = x for every x: a kernel may then skip the
+    // additions and still give the bits of one that performs them.
     float shift[9], shift_s[9];
-    float pad1[2];
+    float nshift[9], nshift_s[9];
+    int32_t shifted;
+    float pad1;
 };
 
 constexpr uint32_t LINK_MOVING = 1u << 31;
@@ -68,6 +73,11 @@ constexpr int kListLanes = 16;
 // not the tile's).
 struct Tile {
     int32_t lattice, y0, x0, pad;
+};
+
+// Origin of a two-step tile of a one-lattice handle (tile lists).
+struct TileXY {
+    int32_t x0, y0;
 };
 
 }  // namespace lbm

@@ -27,6 +27,8 @@
 
 #include <cstdint>
 
+#include <type_traits>
+
 #include "lbm_kernels.h"
 
 namespace lbm {
@@ -50,7 +52,8 @@ enum { BB_SIMPLE = 0, BB_MEI = 1, BB_BOUZIDI = 2 };
 // dec/enc convert between a stored value and the deviation f - w; dec_raw/enc_raw
 // keep the stored scale (h = 2^k (f - w - s_i)), for arithmetic done in scaled units.
 // The 16-bit forms store the deviation less a per-direction constant s_i (the lattice's
-// storage shift, zero unless set), passed as `sh`; the FP64 and FP32 forms ignore it.
+// storage shift): dec adds sh = s_i, enc adds sh = -s_i (both -0.0f without a shift, an
+// exact identity); the FP64 and FP32 forms ignore them.
 
 template <int ST> struct Form;
 template <> struct Form<ST_F64> {
@@ -79,7 +82,7 @@ template <> struct Form<ST_F16S> {
     // Conversion to binary16 rounds to nearest even (v_cvt_f16_f32).
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(h)); }
     static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
-    static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
+    static __device__ __forceinline__ store enc(real d, float scale, float nsh) { return enc_raw((d + nsh) * scale); }
     // Stochastic rounding (v_cvt_sr_f16_f32): in the binary16 normal range the top 13
     // bits of rnd are added below the last kept mantissa bit and the sum truncated; below
     // it the instruction keeps rounding stochastically at the subnormal spacing.
@@ -98,7 +101,7 @@ template <> struct Form<ST_BF16S> {
     // Round to nearest even (v_cvt_pk_bf16_f32 on gfx950); NaN stays NaN.
     static __device__ __forceinline__ store enc_raw(real h) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(h)); }
     static __device__ __forceinline__ real dec(store v, float inv_scale, float sh) { return dec_raw(v) * inv_scale + sh; }
-    static __device__ __forceinline__ store enc(real d, float scale, float sh) { return enc_raw((d - sh) * scale); }
+    static __device__ __forceinline__ store enc(real d, float scale, float nsh) { return enc_raw((d + nsh) * scale); }
     // Stochastic rounding (v_cvt_sr_bf16_f32): the top 16 bits of rnd are added below the
     // last kept bit and the sum truncated; infinities stay, NaN stays NaN.
     static __device__ __forceinline__ store enc_sr_raw(real h, uint32_t rnd)
@@ -136,20 +139,23 @@ struct Rounding {
 // in its caller's loop index, so the sequence is fixed per node.
 __device__ __forceinline__ uint32_t next_rand(uint32_t& h, int j)
 {
-    if (j & 1) return h << 16;
+    // h << 16 as a byte permute: written as a shift, the compiler folds it into the
+    // multiply of the next word (a second 32-bit multiply, a quarter-rate instruction)
+    if (j & 1) return __builtin_amdgcn_perm(h, h, 0x01000c0cu);
     if (j) h *= 0x85ebca6bu;
     return h & 0xffff0000u;
 }
 
-// Encode deviation d for storage, rounding to nearest even or stochastically.
+// Encode deviation d for storage (nsh: minus the shift), rounding to nearest even or
+// stochastically.
 template <int ST>
-__device__ __forceinline__ typename Form<ST>::store enc_node(typename Form<ST>::real d, float scale, float sh,
+__device__ __forceinline__ typename Form<ST>::store enc_node(typename Form<ST>::real d, float scale, float nsh,
                                                              const Rounding& rd, uint32_t& h, int j)
 {
     if constexpr (Form<ST>::shifted) {
-        if (rd.on) return Form<ST>::enc_sr_raw((d - sh) * scale, next_rand(h, j));
+        if (rd.on) return Form<ST>::enc_sr_raw((d + nsh) * scale, next_rand(h, j));
     }
-    return Form<ST>::enc(d, scale, sh);
+    return Form<ST>::enc(d, scale, nsh);
 }
 
 // The same for a value already in stored units (less the shift).
@@ -688,7 +694,7 @@ __device__ __forceinline__ void node_general(const LatticeDev& L, const StepArgs
     uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
 #pragma unroll
     for (int j = 0; j < 9; ++j)
-        store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j), a.options & LAUNCH_STREAM_STORES);
+        store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.nshift[j], rd, h, j), a.options & LAUNCH_STREAM_STORES);
 }
 
 // Edge node number e of a lattice: rows 0 and ny-1, then columns 0 and nx-1 between them.
@@ -792,7 +798,7 @@ __device__ __forceinline__ void cell_update(const LatticeDev& L, const StepArgs&
     uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
 #pragma unroll
     for (int j = 0; j < 9; ++j)
-        store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j), a.options & LAUNCH_STREAM_STORES);
+        store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.nshift[j], rd, h, j), a.options & LAUNCH_STREAM_STORES);
 }
 
 // Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane i-1 /
@@ -898,7 +904,7 @@ __device__ __forceinline__ void group_update(const LatticeDev& L, const StepArgs
         uint32_t h = rd.on ? rd.node(L.cell_off + idx + k) : 0u;
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
-            if constexpr (Form<ST>::shifted) d[j] -= L.shift_s[j];
+            if constexpr (Form<ST>::shifted) d[j] += L.nshift_s[j];
             out[j].v[k] = enc_node_raw<ST>(d[j], rd, h, j);
         }
     }
@@ -989,7 +995,7 @@ __global__ __launch_bounds__(BX) void step_kernel(StepArgs a)
 #pragma unroll
                     for (int j = 0; j < 9; ++j)
                         o[j * L.plane] = Form<ST>::enc(-static_cast<typename Form<ST>::real>(WD[j]), a.scale,
-                                                       L.shift[j]);
+                                                       L.nshift[j]);
                 }
             }
             if (bad) atomicOr(a.nan_flags + lat_id, 1);
@@ -1145,7 +1151,7 @@ __device__ __forceinline__ int node_values(const LatticeDev& L, const StepArgs& 
     }
     uint32_t h = rd.on ? rd.node(L.cell_off + idx) : 0u;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) out[j] = enc_node<ST>(d[j], a.scale, L.shift[j], rd, h, j);
+    for (int j = 0; j < 9; ++j) out[j] = enc_node<ST>(d[j], a.scale, L.nshift[j], rd, h, j);
     return 1;
 }
 
@@ -1201,8 +1207,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
     }
     int x0, y0;
     if constexpr (SINGLE) {
-        x0 = (b % a.tiles_x) * TX;
-        y0 = (b / a.tiles_x) * TY;
+        if (a.tile_list) {
+            // the tiles the bulk kernel does not take (64 x TY, origins from the list)
+            const TileXY t = a.tile_list[b];
+            x0 = t.x0;
+            y0 = t.y0;
+        } else {
+            x0 = (b % a.tiles_x) * TX;
+            y0 = (b / a.tiles_x) * TY;
+        }
     } else {
         const Tile t = a.tiles[b];
         lat_id = t.lattice;
@@ -1394,7 +1407,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
             uint32_t h = rd1.on ? rd1.node(L.cell_off + idx) : 0u;
 #pragma unroll
             for (int j = 0; j < 9; ++j)
-                store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j),
+                store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.nshift[j], rd1, h, j),
                           a.options & LAUNCH_STREAM_STORES);
         }
     } else {
@@ -1427,7 +1440,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
             uint32_t h = rd1.on ? rd1.node(L.cell_off + idx) : 0u;
 #pragma unroll
             for (int j = 0; j < 9; ++j)
-                store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.shift[j], rd1, h, j),
+                store_val(o + j * L.plane, enc_node<ST>(d[j], a.scale, L.nshift[j], rd1, h, j),
                           a.options & LAUNCH_STREAM_STORES);
         };
         // step k+1 on the tile, from LDS: plain nodes now, general-path nodes queued
@@ -1497,6 +1510,230 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
         }
     }
     if (bad) atomicOr(a.nan_flags + lat_id, 1);
+}
+
+// ------------------------------------------------------------------ two steps per launch: bulk tiles
+
+// Row of ring position r of a 66-wide ring by a 24-bit multiply and a shift: floor(r * 993 /
+// 2^16) = floor(r / 66) while r < 32768 (993 / 2^16 exceeds 1/66 by 4.6e-7, and
+// 65/66 + 32768 * 4.6e-7 < 1).
+__host__ __device__ constexpr int ring_row(int r) { return static_cast<int>((static_cast<uint32_t>(r) * 993u) >> 16); }
+
+constexpr bool ring_row_exact(int n)
+{
+    for (int r = 0; r < n; ++r)
+        if (ring_row(r) != r / 66) return false;
+    return true;
+}
+
+// Value at a byte offset from a base pointer the whole wave shares: the offset is a 32-bit
+// lane value, so the address is the scalar base plus the zero-extended offset.
+template <typename T> __device__ __forceinline__ T load_at(const T* base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Stochastic rounding of x (the 16-bit forms' enc_sr_raw) into the low or high half of
+// pk, keeping the other half.
+template <int ST, bool HI> __device__ __forceinline__ uint32_t cvt_sr_half(uint32_t pk, float x, uint32_t rnd)
+{
+    if constexpr (ST == ST_F16S) {
+        typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+        return __builtin_bit_cast(uint32_t,
+                                  __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(half2_t, pk), x, rnd, HI));
+    } else {
+        typedef short short2_t __attribute__((ext_vector_type(2)));
+        return __builtin_bit_cast(uint32_t,
+                                  __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(short2_t, pk), x, rnd, HI));
+    }
+}
+
+// Steps k and k+1 of the bulk tiles of a one-lattice handle: 64 x TY tiles whose ring of
+// nodes (66 x (TY + 2)) lies inside the lattice and holds no general-path node -- no edge,
+// boundary link, solid, inlet, outlet or lid node -- so every node of the launch is a plain
+// pull and a collision.  The host lists them (a.tile_list, origins); the other tiles go to
+// step2_kernel through a list of their own, in a launch after this one.  Without the
+// general path the kernel keeps its values and pointers in registers: the pull addresses
+// are per-plane scalar bases plus one 32-bit lane offset, the ring's row and column come
+// from 24-bit integer arithmetic, and the stochastic-rounding choice is compiled in (SR).
+// Decoding, collision and encoding are group_update's, in stored units, so the results are
+// bit for bit those of every other kernel.
+template <int ST, int TY, bool SR>
+__global__ __launch_bounds__(256) void step2_bulk_kernel(StepArgs a)
+{
+    using S = typename Form<ST>::store;
+    using R = typename Form<ST>::real;
+    constexpr int TX = 64, WX = TX + 2, WY = TY + 2, NR = WX * WY, P1 = (NR + 255) / 256, P2 = TX * TY / 256;
+    static_assert(ring_row_exact(NR), "ring_row must be exact over the ring");
+    static_assert(P2 * 256 == TX * TY, "whole tiles of 256 threads");
+    __shared__ S lds[9][WY][WX];
+
+    const LatticeDev& L = a.L0;
+    const int nb = static_cast<int>(gridDim.x);
+    int b = static_cast<int>(blockIdx.x);
+    if (a.options & LAUNCH_XCD_ORDER) {
+        const int q = nb >> 3, r = nb & 7, xcd = b & 7;
+        b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+    }
+    const TileXY t = a.tile_list[b];
+    // ring position (lx, ly) is lattice node (x0 - 1 + lx, y0 - 1 + ly): element ly * pitch
+    // + lx from the ring origin (less than (TY + 2) pitch + 66, a 32-bit offset)
+    const int64_t org = static_cast<int64_t>(t.y0 - 1) * L.pitch + (t.x0 - 1);
+    const S* src = static_cast<const S*>(a.src) + L.f_off + org;
+    S* dst = static_cast<S*>(a.dst) + L.f_off + org;
+    const uint16_t* wdb = a.wdist + L.cell_off + org;
+    const uint32_t cell0 = static_cast<uint32_t>(L.cell_off + org);
+    const int pitch = L.pitch;
+    const bool nt = (a.options & LAUNCH_STREAM_STORES) != 0;
+    const bool les = L.les != 0;
+    const R sc = static_cast<R>(a.scale);
+    uint32_t key0 = 0, key1 = 0;
+    if constexpr (SR) {
+        key0 = step_key(a, a.lattice0, 0);
+        key1 = step_key(a, a.lattice0, 1);
+    }
+    const int tid = static_cast<int>(threadIdx.x);
+    const int tx = tid & 63, ty0 = tid >> 6;
+
+    // the second step's wall distances, fetched with the first step's loads
+    uint16_t wd2[P2];
+#pragma unroll
+    for (int k = 0; k < P2; ++k) {
+        wd2[k] = 0xffffu;
+        if (les) wd2[k] = load_at(wdb, static_cast<uint32_t>(((ty0 + 4 * k + 1) * pitch + tx + 1) * 2));
+    }
+
+    // decode, collide, encode one node in place (stored units, as group_update; the shift
+    // additions are skipped without a shift, where they are exact identities)
+    auto update = [&](S v[9], uint16_t wraw, uint32_t key, uint32_t cell, bool& bad) {
+        R d[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec_raw(v[j]);
+        if constexpr (Form<ST>::shifted) {
+            if (L.shifted) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j) d[j] += L.shift_s[j];
+            }
+        }
+        R wd = R(0);
+        if (les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
+        collide<R>(d, L, wd, sc);
+        if (!(d[0] == d[0])) bad = true;
+        if constexpr (Form<ST>::shifted) {
+            if (L.shifted) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j) d[j] += L.nshift_s[j];
+            }
+        }
+        if constexpr (SR) {
+            // enc_node_raw's words and conversions, two populations per register (each
+            // conversion writes one half and keeps the other)
+            uint32_t h = mix32(key ^ (cell * 0x9e3779b1u));
+#pragma unroll
+            for (int j = 0; j < 9; j += 2) {
+                // the pair's register starts as the first random word (dead after its use)
+                const uint32_t r0 = next_rand(h, j);
+                uint32_t pk = cvt_sr_half<ST, false>(r0, d[j], r0);
+                if (j < 8) pk = cvt_sr_half<ST, true>(pk, d[j + 1], next_rand(h, j + 1));
+                v[j] = static_cast<S>(pk);
+                if (j < 8) v[j + 1] = static_cast<S>(pk >> 16);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) v[j] = Form<ST>::enc_raw(d[j]);
+        }
+    };
+
+    // pulls and wall distance of ring position r (r < NR), one register each (kept apart
+    // until used: packing two 16-bit values into one register would wait for both loads)
+    using W = typename std::conditional<sizeof(S) == 2, uint32_t, S>::type;
+    auto fetch = [&](int r, W v[9], uint32_t& wraw) {
+        const int ly = ring_row(r), lx = r - 66 * ly;
+        const uint32_t o = __umul24(ly, pitch) + lx;  // pitch < 2^24
+#pragma unroll
+        for (int j = 0; j < 9; ++j) v[j] = load_at(src + j * L.plane - (CY[j] * pitch + CX[j]), o * sizeof(S));
+        wraw = 0xffffu;
+        if (les) wraw = load_at(wdb, o * 2u);
+    };
+
+    bool bad = false;
+    // step k on the tile and its ring, from the source buffer.  A thread's ring positions
+    // are tid, tid + 256, ...; the loads of the next PF positions are in flight while one
+    // is computed (16-bit loads move 128 bytes a wave: more of them must be outstanding to
+    // cover the memory latency).  The last position of the ring stands in for those past
+    // it: loaded, not used.
+    constexpr int PF = sizeof(S) == 2 ? 2 : 1;
+    W q[PF][9];
+    uint32_t wq[PF];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) fetch(tid + 256 * p < NR ? tid + 256 * p : NR - 1, q[p], wq[p]);
+    // unrolled: the queue is renamed registers, not copies (a copy waits for its load)
+#pragma unroll
+    for (int k = 0; k < P1; ++k) {
+        const int r = tid + 256 * k;
+        W nxt[9];
+        uint32_t wnxt = 0xffffu;
+        if (k + PF < P1) fetch(r + 256 * PF < NR ? r + 256 * PF : NR - 1, nxt, wnxt);
+        if (r < NR) {
+            const int ly = ring_row(r), lx = r - 66 * ly;
+            S v[9];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) v[j] = static_cast<S>(q[0][j]);
+            update(v, static_cast<uint16_t>(wq[0]), key0, cell0 + __umul24(ly, pitch) + lx, bad);
+#pragma unroll
+            for (int j = 0; j < 9; ++j) lds[j][ly][lx] = v[j];
+        }
+#pragma unroll
+        for (int p = 0; p + 1 < PF; ++p) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) q[p][j] = q[p + 1][j];
+            wq[p] = wq[p + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < 9; ++j) q[PF - 1][j] = nxt[j];
+        wq[PF - 1] = wnxt;
+    }
+    __syncthreads();
+    // step k+1 on the tile, from LDS
+#pragma unroll 1
+    for (int k = 0; k < P2; ++k) {
+        const int lx = tx + 1, ly = ty0 + 4 * k + 1;
+        const int o = static_cast<int>(__umul24(ly, pitch)) + lx;
+        uint16_t wraw = wd2[0];
+#pragma unroll
+        for (int q = 1; q < P2; ++q)
+            if (k == q) wraw = wd2[q];
+        S v[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) v[j] = lds[j][ly - CY[j]][lx - CX[j]];
+        update(v, wraw, key1, cell0 + static_cast<uint32_t>(o), bad);
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+            store_val(reinterpret_cast<S*>(reinterpret_cast<char*>(dst + j * L.plane) + static_cast<uint32_t>(o) * sizeof(S)),
+                      v[j], nt);
+    }
+    if (bad) atomicOr(a.nan_flags + a.lattice0, 1);
+}
+
+// Bulk tiles of a one-lattice handle's two-step launches: out[t] = 1 when tile t (64 x ty,
+// row-major over tiles_x columns) has its ring of nodes inside the lattice and no
+// general-path node in it (edge nodes are general-path nodes, so the ring's pulls stay
+// inside the lattice too).
+__global__ __launch_bounds__(256) void tile_class_kernel(const uint8_t* flags, int nx, int ny, int pitch,
+                                                         int tiles_x, int ty, uint8_t* out)
+{
+    const int t = static_cast<int>(blockIdx.x);
+    const int x0 = (t % tiles_x) * 64, y0 = (t / tiles_x) * ty;
+    const bool inside = x0 >= 1 && x0 + 64 <= nx - 1 && y0 >= 1 && y0 + ty <= ny - 1;
+    int special = 0;
+    if (inside) {
+        for (int r = static_cast<int>(threadIdx.x); r < 66 * (ty + 2); r += 256) {
+            const int ly = r / 66, lx = r - 66 * ly;
+            if (flags[static_cast<int64_t>(y0 - 1 + ly) * pitch + (x0 - 1 + lx)] & CELL_SPECIAL) special = 1;
+        }
+    }
+    const int any = __syncthreads_or(special);
+    if (threadIdx.x == 0) out[t] = inside && !any ? 1 : 0;
 }
 
 // ------------------------------------------------------------------ state / prologue
@@ -1569,7 +1806,7 @@ __global__ __launch_bounds__(256) void state_kernel(StateArgs a)
             }
             S* co = static_cast<S*>(a.coll_dst) + L.f_off + idx;
 #pragma unroll
-            for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
+            for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.nshift[j]);
         }
     }
     if (a.mass_dev) {
@@ -1607,7 +1844,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
     S* st = static_cast<S*>(a.state_dst) + L.f_off + idx;
     S* co = static_cast<S*>(a.coll_dst) + L.f_off + idx;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) st[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
+    for (int j = 0; j < 9; ++j) st[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.nshift[j]);
     if (!(a.flags[L.cell_off + idx] & CELL_SOLID)) {
         R wd = R(0);
         if (L.les) {
@@ -1617,7 +1854,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(PrologueArgs a)
         collide<R>(d, L, wd, R(1));
     }
 #pragma unroll
-    for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.shift[j]);
+    for (int j = 0; j < 9; ++j) co[j * L.plane] = Form<ST>::enc(d[j], a.scale, L.nshift[j]);
 }
 
 // ------------------------------------------------------------------ initial perturbation
@@ -1651,7 +1888,7 @@ __global__ __launch_bounds__(256) void perturb_kernel(PerturbArgs a)
         const double eu = CX[j] * u + CY[j] * v;
         const double uu = u * u + v * v;
         const double feq = WD[j] * rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uu);
-        st[j * L.plane] = Form<ST>::enc(static_cast<typename Form<ST>::real>(feq - WD[j]), a.scale, L.shift[j]);
+        st[j * L.plane] = Form<ST>::enc(static_cast<typename Form<ST>::real>(feq - WD[j]), a.scale, L.nshift[j]);
     }
 }
 
@@ -2325,6 +2562,53 @@ hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, c
 {
     return single ? launch_step2_sg<true>(storage, tile_rows, n_blocks, a, s)
                   : launch_step2_sg<false>(storage, tile_rows, n_blocks, a, s);
+}
+
+namespace {
+template <int ST, int TY> hipError_t launch_bulk_t(int n_blocks, const StepArgs& a, hipStream_t s)
+{
+    if (Form<ST>::shifted && a.sr)
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, Form<ST>::shifted>), dim3(n_blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, false>), dim3(n_blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int ST> hipError_t launch_bulk_st(int ty, int n_blocks, const StepArgs& a, hipStream_t s)
+{
+    switch (ty) {
+    case 8: return launch_bulk_t<ST, 8>(n_blocks, a, s);
+    case 16: return launch_bulk_t<ST, 16>(n_blocks, a, s);
+    case 32: return launch_bulk_t<ST, 32>(n_blocks, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+}  // namespace
+
+bool bulk_supported(int storage, int tile_rows)
+{
+    return storage != ST_F64 && (tile_rows == 8 || tile_rows == 16 || tile_rows == 32);
+}
+
+hipError_t launch_step2_bulk(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s)
+{
+    if (n_blocks <= 0) return hipSuccess;
+    switch (storage) {
+    case ST_F32: return launch_bulk_st<ST_F32>(tile_rows, n_blocks, a, s);
+    case ST_F16S: return launch_bulk_st<ST_F16S>(tile_rows, n_blocks, a, s);
+    case ST_BF16S: return launch_bulk_st<ST_BF16S>(tile_rows, n_blocks, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_tile_class(const uint8_t* flags, int nx, int ny, int pitch, int tiles_x, int tiles_y, int ty,
+                             uint8_t* out, hipStream_t s)
+{
+    const int64_t n = static_cast<int64_t>(tiles_x) * tiles_y;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tile_class_kernel, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, flags, nx, ny, pitch,
+                       tiles_x, ty, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_flush(const FlushArgs& f, hipStream_t s)

@@ -1,0 +1,152 @@
+"""Two steps per launch with bulk tiles: on a one-lattice handle the 64 x rows tiles whose
+node ring lies inside the lattice and holds no general-path node go to the bulk kernel,
+the others to the general two-step kernel.  The split must not change a bit: populations
+and force totals equal one step per launch, in every storage form, with and without
+stochastic rounding, for every boundary kind next to the bulk tiles."""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import lbm2d as G
+import oracle as O
+from cases import CASES, _hills, _mask_setup, build_gpu, build_oracle
+
+pytestmark = pytest.mark.gpu
+
+HILLS_BIG = _hills(384, 144)
+
+# the small cases scaled up until most of their tiles are bulk tiles
+BULK_CASES = {
+    "cylinder": dataclasses.replace(CASES["cylinder"], name="cylinder_big", nx=448, ny=160, pert=(100, 300),
+                                    setup=lambda s: s.place_cylinder(112, 81, 12),
+                                    gpu_geom=(("circle", 112, 81, 12, G.BB_MEI),)),
+    "les": dataclasses.replace(CASES["les"], name="les_big", nx=448, ny=160, pert=(100, 300),
+                               setup=lambda s: s.place_cylinder(112, 81, 12),
+                               gpu_geom=(("circle", 112, 81, 12, G.BB_MEI),)),
+    "bgk": dataclasses.replace(CASES["bgk"], name="bgk_big", nx=448, ny=160, pert=(100, 300),
+                               setup=lambda s: s.place_cylinder(112, 81, 12),
+                               gpu_geom=(("circle", 112, 81, 12, G.BB_MEI),)),
+    "bouzidi": dataclasses.replace(CASES["bouzidi"], name="bouzidi_big", nx=448, ny=160,
+                                   setup=lambda s: (s.place_cylinder(112, 80, 12), s.set_moving_wall(0.01, 112, 80),
+                                                    s.set_mei(False)),
+                                   gpu_geom=(("circle", 112, 80, 12, G.BB_BOUZIDI), ("moving", 0.01, 112, 80))),
+    "hills": dataclasses.replace(CASES["hills"], name="hills_big", nx=384, ny=144, setup=_mask_setup(HILLS_BIG)),
+    "cavity": dataclasses.replace(CASES["cavity"], name="cavity_big", nx=256, ny=256),
+}
+
+
+def expected_split(obstacle, x_periodic, y_periodic, rows):
+    """Bulk and other tiles of 64 x rows tiles from the mask: a node needs the general
+    path when it is solid, on an edge, or fluid next to a solid (periodic edges wrap)."""
+    ny, nx = obstacle.shape
+    solid = obstacle.astype(bool)
+    near = np.zeros_like(solid)
+    for cx, cy in ((1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)):
+        s = np.roll(np.roll(solid, -cy, axis=0), -cx, axis=1)
+        if not x_periodic:
+            if cx == 1:
+                s[:, -1] = False
+            if cx == -1:
+                s[:, 0] = False
+        if not y_periodic:
+            if cy == 1:
+                s[-1, :] = False
+            if cy == -1:
+                s[0, :] = False
+        near |= s
+    special = solid | near
+    special[0, :] = special[-1, :] = True
+    special[:, 0] = special[:, -1] = True
+    bulk = other = 0
+    for y0 in range(0, ny, rows):
+        for x0 in range(0, nx, 64):
+            inside = x0 >= 1 and x0 + 64 <= nx - 1 and y0 >= 1 and y0 + rows <= ny - 1
+            if inside and not special[y0 - 1:y0 + rows + 1, x0 - 1:x0 + 65].any():
+                bulk += 1
+            else:
+                other += -(-(min(rows, ny - y0)) // min(rows, 16))
+    return bulk, other
+
+
+FORMS = [("f32", False), ("f16s", False), ("f16s", True), ("bf16s", False), ("bf16s", True)]
+
+
+@pytest.mark.parametrize("storage,sr", FORMS, ids=[f"{s}{'_sr' if r else ''}" for s, r in FORMS])
+@pytest.mark.parametrize("name", sorted(BULK_CASES))
+def test_bulk_tiles_bitwise_equal_one_step_per_launch(name, storage, sr):
+    c = BULK_CASES[name]
+    a, o = build_gpu(c, storage)
+    b, _ = build_gpu(c, storage, o=o)
+    if sr:
+        a.set_stochastic_rounding(True, 77)
+        b.set_stochastic_rounding(True, 77)
+    a.set_launch(64, 1, 1, 0)
+    desc = a.info()
+    assert desc.bulk_tiles == 0
+    d = G.desc_for_case(c.case, c.nx, c.ny)
+    seen = 0
+    for rows, opt in ((16, 5), (32, 7), (8, 6), (32, 4), (16, 7)):
+        b.set_launch(64, rows, 1, opt, graph_steps=10)
+        info = b.info()
+        exp = expected_split(o.obstacle, d.x_edge == G.EDGE_PERIODIC, d.y_edge == G.EDGE_PERIODIC, rows)
+        assert (info.bulk_tiles, info.other_tiles) == exp, f"rows {rows}"
+        seen += info.bulk_tiles
+        for n in (1, 5, 12):
+            a.step(n)
+            b.step(n)
+            np.testing.assert_array_equal(b.download(0), a.download(0), err_msg=f"rows {rows} options {opt} n {n}")
+    assert seen > 0
+    fa, fya = a.forces(0)
+    fb, fyb = b.forces(0)
+    np.testing.assert_array_equal(fb, fa)
+    np.testing.assert_array_equal(fyb, fya)
+    assert not b.nan_flag(0)
+    a.close()
+    b.close()
+
+
+def test_bulk_tiles_fp32_against_oracle():
+    """The split launch in FP32 against the FP64 oracle on the LES case."""
+    c = BULK_CASES["les"]
+    ens, o = build_gpu(c, "f32")
+    ens.set_launch(64, 32, 1, 5, graph_steps=10)
+    ens.step(30)
+    assert ens.info().bulk_tiles > 0
+    o.step(c.tau, c.u, 30)
+    assert np.abs(ens.download(0) - o.f).max() <= 2e-6
+    fx, fy = ens.forces(0)
+    ofx, ofy = o.total_force()
+    assert abs(fx[-1, 0] - ofx) <= 1e-4 * (abs(ofx) + 1e-3)
+    ens.close()
+
+
+def test_bulk_plan_follows_geometry_and_refusals():
+    """A new obstacle rebuilds the tile classes; 64 x 32 tiles need the split (one lattice,
+    FP32 or a 16-bit form)."""
+    c = BULK_CASES["cylinder"]
+    ens, o = build_gpu(c, "f32")
+    ens.set_launch(64, 16, 1, 5)
+    ens.step(2)
+    d = G.desc_for_case(c.case, c.nx, c.ny)
+    first = ens.info().bulk_tiles
+    m = o.obstacle.copy()
+    m[40:120, 300:310] = 1
+    ens.set_obstacle(0, m)
+    ens.upload(0, o.f.copy())
+    ens.step(2)
+    info = ens.info()
+    exp = expected_split(m, d.x_edge == G.EDGE_PERIODIC, d.y_edge == G.EDGE_PERIODIC, 16)
+    assert (info.bulk_tiles, info.other_tiles) == exp
+    assert info.bulk_tiles < first
+    ens.close()
+    f64, _ = build_gpu(c, "f64")
+    with pytest.raises(G.LbmError):
+        f64.set_launch(64, 32, 1, 5)
+    f64.close()
+    two = G.Ensemble([G.desc_for_case(G.CYLINDER, 256, 128)] * 2, storage="f32")
+    with pytest.raises(G.LbmError):
+        two.set_launch(64, 32, 1, 5)
+    two.close()

@@ -192,3 +192,56 @@ def test_two_step_ring_needs_the_far_positions_skipped():
     (d,) = _descs([(3, 3)], (G.EDGE_PERIODIC, G.EDGE_PERIODIC))
     gx, _ = _ring_nodes(d, 4, False, skip_far=False)
     assert gx.max() >= 3
+
+
+# ---- bulk tiles of two-step launches (lbm_kernels.hip step2_bulk_kernel) -------------
+#
+# A bulk tile's ring (66 x (rows + 2) nodes from (x0 - 1, y0 - 1)) lies inside the lattice
+# and holds no general-path node; every edge node is a general-path node, so each ring
+# node lies off the edges and its nine pulls are lattice nodes.  The kernel forms those
+# addresses as the ring origin plus a 32-bit offset (ly * pitch + lx, ly * pitch by a
+# 24-bit multiply) and finds a ring position's row by a multiply-shift.
+
+def _bulk_tiles(special, rows):
+    ny, nx = special.shape
+    out = []
+    for y0 in range(0, ny, rows):
+        for x0 in range(0, nx, 64):
+            if (x0 >= 1 and x0 + 64 <= nx - 1 and y0 >= 1 and y0 + rows <= ny - 1
+                    and not special[y0 - 1:y0 + rows + 1, x0 - 1:x0 + 65].any()):
+                out.append((x0, y0))
+    return out
+
+
+@pytest.mark.parametrize("rows", [8, 16, 32])
+def test_bulk_tile_pulls_are_lattice_nodes(rows):
+    rng = np.random.default_rng(rows)
+    for nx, ny in ((300, 140), (1000, 333), (129, 97)):
+        solid = rng.random((ny, nx)) < 0.0005
+        special = solid.copy()
+        for cx, cy in zip(CX[1:], CY[1:]):
+            special |= np.roll(np.roll(solid, -cy, 0), -cx, 1)
+        special[0, :] = special[-1, :] = special[:, 0] = special[:, -1] = True
+        tiles = _bulk_tiles(special, rows)
+        assert tiles or nx < 200
+        for x0, y0 in tiles:
+            ly, lx = np.meshgrid(np.arange(rows + 2), np.arange(66), indexing="ij")
+            x, y = x0 - 1 + lx, y0 - 1 + ly
+            assert not special[y, x].any()
+            for j in range(9):
+                sx, sy = x - CX[j], y - CY[j]
+                assert sx.min() >= 0 and sx.max() < nx and sy.min() >= 0 and sy.max() < ny
+
+
+def test_bulk_ring_row_and_offsets():
+    """floor(r * 993 / 2^16) = r // 66 over every ring size compiled (up to 66 x 34); the
+    host admits a lattice to the bulk kernel only while (rows + 3) rows of population bytes
+    fit in 31 bits and pitch < 2^24 (lbm_host.hip bulk_eligible)."""
+    r = np.arange(66 * 34)
+    assert np.array_equal((r * 993) >> 16, r // 66)
+    r = np.arange(32768)
+    assert np.array_equal((r * 993) >> 16, r // 66)
+    pitch, bpp, rows = (32768 + 63) // 64 * 64, 4, 32
+    assert (rows + 3) * pitch * bpp < 2 ** 31 and pitch < 2 ** 24
+    # the largest offset a ring load forms, in bytes
+    assert ((rows + 1) * pitch + 65 + pitch + 1) * bpp < 2 ** 32
