@@ -94,6 +94,7 @@ def time_steps(ens, steps, warmup, torch):
     synchronisations of the library's stream and the device."""
     if warmup:
         ens.step(warmup)
+    ens.step(0)  # commits and builds the step graphs outside the timed region
     ens.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -108,6 +109,7 @@ def graph_step_ms(ens, torch, replays=20, graph_steps=100):
     replay of a hipGraph of graph_steps steps."""
     ens.set_launch(graph_steps=graph_steps)
     stream = torch.cuda.current_stream()
+    ens.step(0)
     ens.step(graph_steps)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(replays + 1)]
     evs[0].record(stream)

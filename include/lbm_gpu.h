@@ -203,8 +203,9 @@ int lbm_perturb_velocity(lbm_handle h, int32_t lattice, int32_t x0, int32_t x1, 
  * writers take (src/lbm.hpp:588-700) at a third of the transfer. */
 int lbm_macros(lbm_handle h, int32_t lattice, int32_t y0, int32_t y1, double* rho, double* u, double* v);
 
-/* Advances every lattice by n steps: one launch per step covering all lattices,
- * replayed from a hipGraph of graph_steps launches when n allows. */
+/* Advances every lattice by n steps: one launch per step (or per two steps) covering
+ * all lattices, replayed from a hipGraph of graph_steps steps when n allows.  n = 0
+ * commits pending changes and prepares those graphs without stepping. */
 int lbm_step(lbm_handle h, int32_t n);
 /* Debugging aid for race and coverage checks (no counterpart in the reference): fills
  * lattice l's planes of the population buffer the next step launch writes (and of the

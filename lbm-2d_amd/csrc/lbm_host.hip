@@ -159,10 +159,15 @@ struct Ensemble {
     struct BulkPlan {
         int rows = 0;  // 0: not built
         int rest_rows = 0;
-        TileXY* d = nullptr;
+        TileXY* d = nullptr;   // bulk tiles
+        Tile* dr = nullptr;    // the others, as an ensemble tile table of lattice 0
         int n_bulk = 0, n_rest = 0;
     };
     BulkPlan plan[3];
+    // a second stream for the other tiles of a split launch, so that they run beside the
+    // bulk tiles (fork and join by events; graph capture records them as two branches)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int64_t* d_steps = nullptr;
     int32_t* d_par = nullptr;
     double* d_acc = nullptr;       // [acc_blocks][kAccCopies][total_slots][2]
@@ -612,18 +617,32 @@ void drop_plans(Ensemble* E)
 {
     for (auto& p : E->plan) {
         if (p.d) (void)hipFree(p.d);
+        if (p.dr) (void)hipFree(p.dr);
         p = Ensemble::BulkPlan{};
     }
 }
 
 // Classifies the 64 x rows tiles of the one-lattice handle from its device flags and
-// uploads the bulk tiles' origins, then those of the other tiles cut to rest_rows rows.
+// uploads the bulk tiles' origins, and the other tiles cut to rest_rows rows as a tile
+// table for the ensemble form of the two-step kernel (which takes inlet, outlet and
+// periodic-edge nodes on its plain path and queues the remaining general-path nodes of a
+// tile: the other tiles are those holding such nodes).
 int build_plan(Ensemble* E, int rows)
 {
+    if (!E->side) {
+        if (hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking) != hipSuccess) {
+            E->side = nullptr;
+            return fail(LBM_ERR_DEVICE, "hipStreamCreateWithFlags: side stream");
+        }
+        if (hipEventCreateWithFlags(&E->fork_ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&E->join_ev, hipEventDisableTiming) != hipSuccess)
+            return fail(LBM_ERR_DEVICE, "hipEventCreateWithFlags: fork / join events");
+    }
     const HostLattice& H = E->lat[0];
     const lbm_desc& d = H.d;
     Ensemble::BulkPlan& P = E->plan[plan_slot(rows)];
     if (P.d) (void)hipFree(P.d);
+    if (P.dr) (void)hipFree(P.dr);
     P = Ensemble::BulkPlan{};
     const int tiles_x = (d.nx + 63) / 64, tiles_y = (d.ny + rows - 1) / rows;
     const size_t n = static_cast<size_t>(tiles_x) * tiles_y;
@@ -638,20 +657,21 @@ int build_plan(Ensemble* E, int rows)
     (void)hipFree(d_cls);
     if (rc != LBM_OK) return rc;
     const int rest_rows = std::min(rows, 16);
-    std::vector<TileXY> bulk, rest;
+    std::vector<TileXY> bulk;
+    std::vector<Tile> rest;
     for (int ty = 0; ty < tiles_y; ++ty)
         for (int tx = 0; tx < tiles_x; ++tx) {
             const int x0 = tx * 64, y0 = ty * rows;
             if (cls[static_cast<size_t>(ty) * tiles_x + tx]) {
                 bulk.push_back(TileXY{x0, y0});
             } else {
-                for (int y = y0; y < std::min(y0 + rows, d.ny); y += rest_rows) rest.push_back(TileXY{x0, y});
+                for (int y = y0; y < std::min(y0 + rows, d.ny); y += rest_rows) rest.push_back(Tile{0, y, x0, 0});
             }
         }
-    std::vector<TileXY> all(bulk);
-    all.insert(all.end(), rest.begin(), rest.end());
-    if ((rc = dev_alloc(E, &P.d, all.size())) != LBM_OK) return rc;
-    if ((rc = h2d(E, P.d, all.data(), all.size() * sizeof(TileXY))) != LBM_OK) return rc;
+    if ((rc = dev_alloc(E, &P.d, bulk.size())) != LBM_OK) return rc;
+    if ((rc = h2d(E, P.d, bulk.data(), bulk.size() * sizeof(TileXY))) != LBM_OK) return rc;
+    if ((rc = dev_alloc(E, &P.dr, rest.size())) != LBM_OK) return rc;
+    if ((rc = h2d(E, P.dr, rest.data(), rest.size() * sizeof(Tile))) != LBM_OK) return rc;
     P.rows = rows;
     P.rest_rows = rest_rows;
     P.n_bulk = static_cast<int>(bulk.size());
@@ -675,11 +695,21 @@ hipError_t launch_pair(Ensemble* E, const StepArgs& a2, bool single, int blocks2
     if (!P) return launch_step2(E->storage, E->rows, single, blocks2, a2, E->stream);
     StepArgs ab = a2;
     ab.tile_list = P->d;
-    hipError_t e = launch_step2_bulk(E->storage, P->rows, P->n_bulk, ab, E->stream);
-    if (e != hipSuccess || P->n_rest == 0) return e;
     StepArgs ar = a2;
-    ar.tile_list = P->d + P->n_bulk;
-    return launch_step2(E->storage, P->rest_rows, true, P->n_rest, ar, E->stream);
+    ar.tiles = P->dr;
+    ar.n_tiles = P->n_rest;
+    if (P->n_rest == 0) return launch_step2_bulk(E->storage, P->rows, P->n_bulk, ab, E->stream);
+    if (P->n_bulk == 0) return launch_step2(E->storage, P->rest_rows, false, P->n_rest, ar, E->stream);
+    // the other tiles (few blocks, long general-path chains) on the side stream, started
+    // first, beside the bulk tiles; both read the source buffer and write disjoint nodes of
+    // the destination, and the next launch on the stream waits for both
+    hipError_t e;
+    if ((e = hipEventRecord(E->fork_ev, E->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(E->side, E->fork_ev, 0)) != hipSuccess) return e;
+    if ((e = launch_step2(E->storage, P->rest_rows, false, P->n_rest, ar, E->side)) != hipSuccess) return e;
+    if ((e = hipEventRecord(E->join_ev, E->side)) != hipSuccess) return e;
+    if ((e = launch_step2_bulk(E->storage, P->rows, P->n_bulk, ab, E->stream)) != hipSuccess) return e;
+    return hipStreamWaitEvent(E->stream, E->join_ev, 0);
 }
 
 int commit(Ensemble* E)
@@ -1529,6 +1559,12 @@ int lbm_destroy(lbm_handle h)
         if (p) (void)hipFree(p);
     lbm::drop_plans(E);
     for (auto& H : E->lat) lbm::drop_new_geometry(H);
+    if (E->side) {
+        (void)hipStreamSynchronize(E->side);
+        (void)hipStreamDestroy(E->side);
+    }
+    if (E->fork_ev) (void)hipEventDestroy(E->fork_ev);
+    if (E->join_ev) (void)hipEventDestroy(E->join_ev);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
     delete h;
     return LBM_OK;
@@ -1930,6 +1966,19 @@ int lbm_step(lbm_handle h, int32_t n)
     if ((rc = require_uploaded(E)) != LBM_OK) return rc;
     if ((rc = commit(E)) != LBM_OK) return rc;
     if (E->graph_dirty) drop_graphs(E);
+    if (n == 0 && E->graph_steps >= 2) {
+        // prepare the graphs later calls replay: one per starting buffer parity of a
+        // one-lattice handle (a batch may leave either), one for an ensemble
+        const int np = E->lat.size() == 1 ? 2 : 1;
+        const int32_t par = E->lat[0].par;
+        for (int p = 0; p < np && rc == LBM_OK; ++p)
+            if (!E->graph[p]) {
+                if (np == 2) E->lat[0].par = p;  // the capture takes the buffers of parity p
+                rc = build_graph(E, p);
+            }
+        E->lat[0].par = par;
+        if (rc != LBM_OK) return rc;
+    }
     while (n > 0) {
         const bool use_graph = E->graph_steps >= 2 && n >= E->graph_steps;
         const int batch = use_graph ? E->graph_steps : std::min(n, E->acc_blocks);

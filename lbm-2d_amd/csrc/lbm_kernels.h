@@ -24,7 +24,7 @@ struct StepArgs {
     int32_t lattice0;
     int32_t tiles_x, tiles_y;   // tiles per row band, row bands
     int32_t options;            // LAUNCH_* bits
-    const TileXY* tile_list;    // two-step launches: one tile origin per block (null: the grid)
+    const TileXY* tile_list;    // two-step bulk launches: one tile origin per block
     // general-path nodes, taken by the first special_blocks blocks: the lattice's
     // boundary-link records, its edge nodes, then its written solids (list of
     // in-lattice indices y*pitch+x)
@@ -35,7 +35,9 @@ struct StepArgs {
     int32_t special_blocks;
     const void* src;            // post-collision populations of the previous step
     void* dst;                  // new post-collision populations (ensemble mode: non-null
-                                // overrides the per-lattice buffer, for the autotuner)
+                                // overrides the per-lattice buffer, for the autotuner; with a
+                                // non-null src both replace the per-lattice buffers -- the
+                                // other tiles of a one-lattice handle's split launch)
     // ensemble launches
     const LatticeDev* lat;
     const Tile* tiles;          // one entry per block

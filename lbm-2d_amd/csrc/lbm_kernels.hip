@@ -1207,24 +1207,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ST == ST_F6
     }
     int x0, y0;
     if constexpr (SINGLE) {
-        if (a.tile_list) {
-            // the tiles the bulk kernel does not take (64 x TY, origins from the list)
-            const TileXY t = a.tile_list[b];
-            x0 = t.x0;
-            y0 = t.y0;
-        } else {
-            x0 = (b % a.tiles_x) * TX;
-            y0 = (b / a.tiles_x) * TY;
-        }
+        x0 = (b % a.tiles_x) * TX;
+        y0 = (b / a.tiles_x) * TY;
     } else {
         const Tile t = a.tiles[b];
         lat_id = t.lattice;
         x0 = t.x0;
         y0 = t.y0;
         L = a.lat[lat_id];
-        const int sp = a.par[lat_id] ^ (a.flips & 1);
-        src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
-        dst = static_cast<S*>(a.dst ? a.dst : (sp ? a.buf0 : a.buf1));
+        if (a.src) {
+            // the other tiles of a one-lattice handle whose bulk tiles go to step2_bulk_kernel
+            src = static_cast<const S*>(a.src);
+            dst = static_cast<S*>(a.dst);
+        } else {
+            const int sp = a.par[lat_id] ^ (a.flips & 1);
+            src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
+            dst = static_cast<S*>(a.dst ? a.dst : (sp ? a.buf0 : a.buf1));
+        }
     }
     const int copy = static_cast<int>(blockIdx.x % kAccCopies);
     double* acc0 = a.acc + static_cast<int64_t>(copy) * a.total_slots * 2;
