@@ -544,6 +544,8 @@ LatticeDev make_dev(const HostLattice& H, int index, float scale)
     L.s_shear = s < 0.5 ? 0.5 : (s > 1.99 ? 1.99 : s);
     L.tau = d.tau;
     L.inv_tau = 1.0 / d.tau;
+    L.half_tau = 0.5 * d.tau;
+    L.qtau2 = 0.25 * d.tau * d.tau;
     L.u_in = d.u_in;
     L.cs = d.les ? d.cs : 0.0;
     L.cs2 = d.les ? d.cs * d.cs : 0.0;

@@ -38,6 +38,7 @@ struct alignas(16) LatticeDev {
     double s_shear;                    // MRT shear rate; s_bulk = 1.2, s_normal = 1 are fixed
                                        // (MRTParams::from_tau, src/lbm_types.hpp:72-76)
     double tau, inv_tau, u_in, cs, cs2, nu, inv_nu, fscale;
+    double half_tau, qtau2;            // tau / 2, tau^2 / 4 (the LES relaxation time)
     // 16-bit storage shift: stored h_i = 2^k (f_i - w_i - shift[i]); shift_s = 2^k shift.
     // Decoding adds shift, encoding adds nshift = -shift.  Without a shift (shifted == 0)
     // both hold -0.0f, for which x + (-0) = x for every x: a kernel may then skip the

@@ -173,9 +173,14 @@ template <typename R> __device__ __forceinline__ R wgt(int i) { return static_ca
 
 // ------------------------------------------------------------------ collision
 
+// max(0.5, min(s, 1.99)): one v_med3_f32 in FP32
 template <typename R> __device__ __forceinline__ R clamp_rate(R s)
 {
-    return s < R(0.5) ? R(0.5) : (s > R(1.99) ? R(1.99) : s);
+    return __builtin_fmax(R(0.5), __builtin_fmin(s, R(1.99)));
+}
+template <> __device__ __forceinline__ float clamp_rate<float>(float s)
+{
+    return __builtin_fmaxf(0.5f, __builtin_fminf(s, 1.99f));
 }
 
 // FP32: the hardware square root, reciprocal and exp2 (1 ulp class); FP64: correctly rounded.
@@ -269,11 +274,11 @@ __device__ __forceinline__ void collide(R d[9], const LatticeDev& L, R wall_dist
                     }
                 }
             }
-            // A = 9 cs^2 Q / (2 rho); Q/rho is scale free
-            const R Q = dsqrt<R>(R(2) * fma_<R>(npxx, npxx, npxy * npxy));
-            const R A = R(4.5) * cs2 * Q * ir;
-            const R tau = static_cast<R>(L.tau);
-            const R tau_eff = (tau + dsqrt<R>(fma_<R>(tau, tau, R(4) * A))) * R(0.5);
+            // A = 9 cs^2 Q / (2 rho), Q = sqrt(2 S) (Q/rho is scale free), and
+            // tau_eff = (tau + sqrt(tau^2 + 4 A)) / 2 = tau/2 + sqrt(tau^2/4 + A)
+            const R S = fma_<R>(npxx, npxx, npxy * npxy);
+            const R A = (cs2 * R(4.5 * 1.4142135623730950488)) * (dsqrt<R>(S) * ir);
+            const R tau_eff = static_cast<R>(L.half_tau) + dsqrt<R>(static_cast<R>(L.qtau2) + A);
             ss = clamp_rate<R>(recip<R>(tau_eff));
         }
         // M^-1 S (m - m_eq), s_bulk = 1.2 and s_normal = 1
@@ -1547,6 +1552,26 @@ template <int ST, bool HI> __device__ __forceinline__ uint32_t cvt_sr_half(uint3
     }
 }
 
+// a and b rounded to nearest even into the low and high halves (the 16-bit forms).
+template <int ST> __device__ __forceinline__ uint32_t cvt_pk_rne(float a, float b)
+{
+    typedef float float2_t __attribute__((ext_vector_type(2)));
+    if constexpr (ST == ST_F16S) {
+        typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(float2_t{a, b}, half2_t));
+    } else {
+        typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(float2_t{a, b}, bf162_t));
+    }
+}
+
+// Threads of a bulk-kernel block: 512 when the tile's LDS (9 planes of 66 x (ty + 2) stored
+// values) exceeds 24 KiB, so that the few blocks a CU holds still bring enough waves.
+__host__ __device__ constexpr int bulk_threads(int bytes_per_value, int ty)
+{
+    return 9 * 66 * (ty + 2) * bytes_per_value > 24 * 1024 ? 512 : 256;
+}
+
 // Steps k and k+1 of the bulk tiles of a one-lattice handle: 64 x TY tiles whose ring of
 // nodes (66 x (TY + 2)) lies inside the lattice and holds no general-path node -- no edge,
 // boundary link, solid, inlet, outlet or lid node -- so every node of the launch is a plain
@@ -1558,13 +1583,14 @@ template <int ST, bool HI> __device__ __forceinline__ uint32_t cvt_sr_half(uint3
 // Decoding, collision and encoding are group_update's, in stored units, so the results are
 // bit for bit those of every other kernel.
 template <int ST, int TY, bool SR>
-__global__ __launch_bounds__(256) void step2_bulk_kernel(StepArgs a)
+__global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY)) void step2_bulk_kernel(StepArgs a)
 {
     using S = typename Form<ST>::store;
     using R = typename Form<ST>::real;
-    constexpr int TX = 64, WX = TX + 2, WY = TY + 2, NR = WX * WY, P1 = (NR + 255) / 256, P2 = TX * TY / 256;
+    constexpr int NT = bulk_threads(sizeof(S), TY);
+    constexpr int TX = 64, WX = TX + 2, WY = TY + 2, NR = WX * WY, P1 = (NR + NT - 1) / NT, P2 = TX * TY / NT;
     static_assert(ring_row_exact(NR), "ring_row must be exact over the ring");
-    static_assert(P2 * 256 == TX * TY, "whole tiles of 256 threads");
+    static_assert(P2 * NT == TX * TY, "whole tiles of NT threads");
     __shared__ S lds[9][WY][WX];
 
     const LatticeDev& L = a.L0;
@@ -1593,13 +1619,14 @@ __global__ __launch_bounds__(256) void step2_bulk_kernel(StepArgs a)
     }
     const int tid = static_cast<int>(threadIdx.x);
     const int tx = tid & 63, ty0 = tid >> 6;
+    constexpr int RS = NT / 64;  // tile rows per step-2 iteration
 
     // the second step's wall distances, fetched with the first step's loads
     uint16_t wd2[P2];
 #pragma unroll
     for (int k = 0; k < P2; ++k) {
         wd2[k] = 0xffffu;
-        if (les) wd2[k] = load_at(wdb, static_cast<uint32_t>(((ty0 + 4 * k + 1) * pitch + tx + 1) * 2));
+        if (les) wd2[k] = load_at(wdb, static_cast<uint32_t>(((ty0 + RS * k + 1) * pitch + tx + 1) * 2));
     }
 
     // decode, collide, encode one node in place (stored units, as group_update; the shift
@@ -1637,6 +1664,16 @@ __global__ __launch_bounds__(256) void step2_bulk_kernel(StepArgs a)
                 v[j] = static_cast<S>(pk);
                 if (j < 8) v[j + 1] = static_cast<S>(pk >> 16);
             }
+        } else if constexpr (Form<ST>::shifted) {
+            // round to nearest even, two conversions per instruction (v_cvt_pk_f16_f32,
+            // v_cvt_pk_bf16_f32): the bits of enc_raw's one at a time
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                const uint32_t pk = cvt_pk_rne<ST>(d[j], d[j + 1]);
+                v[j] = static_cast<S>(pk);
+                v[j + 1] = static_cast<S>(pk >> 16);
+            }
+            v[8] = Form<ST>::enc_raw(d[8]);
         } else {
 #pragma unroll
             for (int j = 0; j < 9; ++j) v[j] = Form<ST>::enc_raw(d[j]);
@@ -1657,7 +1694,7 @@ __global__ __launch_bounds__(256) void step2_bulk_kernel(StepArgs a)
 
     bool bad = false;
     // step k on the tile and its ring, from the source buffer.  A thread's ring positions
-    // are tid, tid + 256, ...; the loads of the next PF positions are in flight while one
+    // are tid, tid + NT, ...; the loads of the next PF positions are in flight while one
     // is computed (16-bit loads move 128 bytes a wave: more of them must be outstanding to
     // cover the memory latency).  The last position of the ring stands in for those past
     // it: loaded, not used.
@@ -1665,14 +1702,14 @@ __global__ __launch_bounds__(256) void step2_bulk_kernel(StepArgs a)
     W q[PF][9];
     uint32_t wq[PF];
 #pragma unroll
-    for (int p = 0; p < PF; ++p) fetch(tid + 256 * p < NR ? tid + 256 * p : NR - 1, q[p], wq[p]);
+    for (int p = 0; p < PF; ++p) fetch(tid + NT * p < NR ? tid + NT * p : NR - 1, q[p], wq[p]);
     // unrolled: the queue is renamed registers, not copies (a copy waits for its load)
 #pragma unroll
     for (int k = 0; k < P1; ++k) {
-        const int r = tid + 256 * k;
+        const int r = tid + NT * k;
         W nxt[9];
         uint32_t wnxt = 0xffffu;
-        if (k + PF < P1) fetch(r + 256 * PF < NR ? r + 256 * PF : NR - 1, nxt, wnxt);
+        if (k + PF < P1) fetch(r + NT * PF < NR ? r + NT * PF : NR - 1, nxt, wnxt);
         if (r < NR) {
             const int ly = ring_row(r), lx = r - 66 * ly;
             S v[9];
@@ -1696,7 +1733,7 @@ __global__ __launch_bounds__(256) void step2_bulk_kernel(StepArgs a)
     // step k+1 on the tile, from LDS
 #pragma unroll 1
     for (int k = 0; k < P2; ++k) {
-        const int lx = tx + 1, ly = ty0 + 4 * k + 1;
+        const int lx = tx + 1, ly = ty0 + RS * k + 1;
         const int o = static_cast<int>(__umul24(ly, pitch)) + lx;
         uint16_t wraw = wd2[0];
 #pragma unroll
@@ -2566,10 +2603,11 @@ hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, c
 namespace {
 template <int ST, int TY> hipError_t launch_bulk_t(int n_blocks, const StepArgs& a, hipStream_t s)
 {
+    constexpr int nt = bulk_threads(sizeof(typename Form<ST>::store), TY);
     if (Form<ST>::shifted && a.sr)
-        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, Form<ST>::shifted>), dim3(n_blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, Form<ST>::shifted>), dim3(n_blocks), dim3(nt), 0, s, a);
     else
-        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, false>), dim3(n_blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, false>), dim3(n_blocks), dim3(nt), 0, s, a);
     return hipGetLastError();
 }
 
