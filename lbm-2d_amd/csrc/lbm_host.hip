@@ -153,13 +153,13 @@ struct Ensemble {
     Tile* d_tiles2 = nullptr;      // ensembles with two steps per launch: 64 x rows tiles
     size_t tiles2_cap = 0;
     int n_tiles2 = 0;
-    // two-step launches of a one-lattice handle, per tile height (8, 16, 32): the bulk
-    // tiles (step2_bulk_kernel) and then the others (step2_kernel on 64 x rest_rows tiles,
-    // rest_rows = min(rows, 16)), origins in one device list, classified from the flags
+    // two-step launches, per tile height (8, 16, 32): the bulk tiles (step2_bulk_kernel)
+    // and the others (the ensemble form of step2_kernel on 64 x rest_rows tiles, rest_rows
+    // = min(rows, 16)), classified from the flags
     struct BulkPlan {
         int rows = 0;  // 0: not built
         int rest_rows = 0;
-        TileXY* d = nullptr;   // bulk tiles
+        Tile* d = nullptr;     // bulk tiles
         Tile* dr = nullptr;    // the others, as an ensemble tile table of lattice 0
         int n_bulk = 0, n_rest = 0;
     };
@@ -606,13 +606,17 @@ int two_step_x_start(const lbm_desc& d) { return d.outlet && d.nx % 64 == 1 ? -6
 int plan_slot(int rows) { return rows == 8 ? 0 : (rows == 16 ? 1 : (rows == 32 ? 2 : -1)); }
 
 // Whether two-step launches of tile height rows split into bulk and other tiles: a
-// one-lattice momentum handle, FP32 or a 16-bit form, rows 8, 16 or 32, and rows short
-// enough for the bulk kernel's 32-bit offsets within a tile's ring and 24-bit row products.
+// momentum handle (one lattice or an ensemble), FP32 or a 16-bit form, rows 8, 16 or 32,
+// and every lattice's rows short enough for the bulk kernel's 32-bit offsets within a
+// tile's ring and its 24-bit row products.
 bool bulk_eligible(const Ensemble* E, int rows)
 {
-    if (E->lat.size() != 1 || E->thermal || !bulk_supported(E->storage, rows)) return false;
-    const int64_t row_bytes = static_cast<int64_t>(E->lat[0].pitch) * E->bpp;
-    return (rows + 3) * row_bytes < (int64_t(1) << 31) && E->lat[0].pitch < (1 << 24);
+    if (E->thermal || !bulk_supported(E->storage, rows)) return false;
+    for (const auto& H : E->lat) {
+        const int64_t row_bytes = static_cast<int64_t>(H.pitch) * E->bpp;
+        if ((rows + 3) * row_bytes >= (int64_t(1) << 31) || H.pitch >= (1 << 24)) return false;
+    }
+    return true;
 }
 
 void drop_plans(Ensemble* E)
@@ -624,9 +628,10 @@ void drop_plans(Ensemble* E)
     }
 }
 
-// Classifies the 64 x rows tiles of the one-lattice handle from its device flags and
-// uploads the bulk tiles' origins, and the other tiles cut to rest_rows rows as a tile
-// table for the ensemble form of the two-step kernel (which takes inlet, outlet and
+// Classifies the 64 x rows two-step tiles of every lattice (rows of an ensemble lattice
+// with an outlet and nx % 64 == 1 right-aligned, as its two-step tile table) from the
+// device flags, and uploads the bulk tiles, and the other tiles cut to rest_rows rows as a
+// tile table for the ensemble form of the two-step kernel (which takes inlet, outlet and
 // periodic-edge nodes on its plain path and queues the remaining general-path nodes of a
 // tile: the other tiles are those holding such nodes).
 int build_plan(Ensemble* E, int rows)
@@ -640,38 +645,48 @@ int build_plan(Ensemble* E, int rows)
             hipEventCreateWithFlags(&E->join_ev, hipEventDisableTiming) != hipSuccess)
             return fail(LBM_ERR_DEVICE, "hipEventCreateWithFlags: fork / join events");
     }
-    const HostLattice& H = E->lat[0];
-    const lbm_desc& d = H.d;
     Ensemble::BulkPlan& P = E->plan[plan_slot(rows)];
     if (P.d) (void)hipFree(P.d);
     if (P.dr) (void)hipFree(P.dr);
     P = Ensemble::BulkPlan{};
-    const int tiles_x = (d.nx + 63) / 64, tiles_y = (d.ny + rows - 1) / rows;
-    const size_t n = static_cast<size_t>(tiles_x) * tiles_y;
+    std::vector<Tile> tiles;
+    for (size_t l = 0; l < E->lat.size(); ++l) {
+        const lbm_desc& d = E->lat[l].d;
+        const int xs = E->lat.size() > 1 ? two_step_x_start(d) : 0;
+        for (int y0 = 0; y0 < d.ny; y0 += rows)
+            for (int x0 = xs; x0 < d.nx; x0 += 64) tiles.push_back(Tile{static_cast<int32_t>(l), y0, x0, 0});
+    }
+    const size_t n = tiles.size();
+    Tile* d_tiles = nullptr;
     uint8_t* d_cls = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(&d_cls), n) != hipSuccess)
+    if (hipMalloc(reinterpret_cast<void**>(&d_tiles), n * sizeof(Tile)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&d_cls), n) != hipSuccess) {
+        if (d_tiles) (void)hipFree(d_tiles);
         return fail(LBM_ERR_NOMEM, "hipMalloc: two-step tile classes");
+    }
     std::vector<uint8_t> cls(n);
-    hipError_t e = launch_tile_class(E->d_flags + H.cell_off, d.nx, d.ny, H.pitch, tiles_x, tiles_y, rows, d_cls,
-                                     E->stream);
-    int rc = e == hipSuccess ? d2h(E, cls.data(), d_cls, n)
+    int rc = h2d(E, d_tiles, tiles.data(), n * sizeof(Tile));
+    if (rc == LBM_OK) {
+        hipError_t e = launch_tile_class(d_tiles, static_cast<int>(n), E->d_lat, E->d_flags, rows, d_cls, E->stream);
+        rc = e == hipSuccess ? d2h(E, cls.data(), d_cls, n)
                              : fail(LBM_ERR_DEVICE, std::string("tile classes: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(d_tiles);
     (void)hipFree(d_cls);
     if (rc != LBM_OK) return rc;
     const int rest_rows = std::min(rows, 16);
-    std::vector<TileXY> bulk;
-    std::vector<Tile> rest;
-    for (int ty = 0; ty < tiles_y; ++ty)
-        for (int tx = 0; tx < tiles_x; ++tx) {
-            const int x0 = tx * 64, y0 = ty * rows;
-            if (cls[static_cast<size_t>(ty) * tiles_x + tx]) {
-                bulk.push_back(TileXY{x0, y0});
-            } else {
-                for (int y = y0; y < std::min(y0 + rows, d.ny); y += rest_rows) rest.push_back(Tile{0, y, x0, 0});
-            }
+    std::vector<Tile> bulk, rest;
+    for (size_t k = 0; k < n; ++k) {
+        const Tile& t = tiles[k];
+        if (cls[k]) {
+            bulk.push_back(t);
+        } else {
+            const int ny = E->lat[t.lattice].d.ny;
+            for (int y = t.y0; y < std::min(t.y0 + rows, ny); y += rest_rows) rest.push_back(Tile{t.lattice, y, t.x0, 0});
         }
+    }
     if ((rc = dev_alloc(E, &P.d, bulk.size())) != LBM_OK) return rc;
-    if ((rc = h2d(E, P.d, bulk.data(), bulk.size() * sizeof(TileXY))) != LBM_OK) return rc;
+    if ((rc = h2d(E, P.d, bulk.data(), bulk.size() * sizeof(Tile))) != LBM_OK) return rc;
     if ((rc = dev_alloc(E, &P.dr, rest.size())) != LBM_OK) return rc;
     if ((rc = h2d(E, P.dr, rest.data(), rest.size() * sizeof(Tile))) != LBM_OK) return rc;
     P.rows = rows;
@@ -689,18 +704,18 @@ const Ensemble::BulkPlan* current_plan(const Ensemble* E)
     return P.rows == E->rows ? &P : nullptr;
 }
 
-// One two-step launch of a batch: on a one-lattice handle with a plan, the bulk tiles and
-// then the others; else the grid of tiles (one lattice) or the ensemble's tile table.
+// One two-step launch of a batch: with a plan, the bulk tiles and the others beside them;
+// else the grid of tiles (one lattice) or the ensemble's tile table.
 hipError_t launch_pair(Ensemble* E, const StepArgs& a2, bool single, int blocks2)
 {
-    const Ensemble::BulkPlan* P = single ? current_plan(E) : nullptr;
+    const Ensemble::BulkPlan* P = current_plan(E);
     if (!P) return launch_step2(E->storage, E->rows, single, blocks2, a2, E->stream);
     StepArgs ab = a2;
-    ab.tile_list = P->d;
+    ab.bulk_tiles = P->d;
     StepArgs ar = a2;
     ar.tiles = P->dr;
     ar.n_tiles = P->n_rest;
-    if (P->n_rest == 0) return launch_step2_bulk(E->storage, P->rows, P->n_bulk, ab, E->stream);
+    if (P->n_rest == 0) return launch_step2_bulk(E->storage, P->rows, single, P->n_bulk, ab, E->stream);
     if (P->n_bulk == 0) return launch_step2(E->storage, P->rest_rows, false, P->n_rest, ar, E->stream);
     // the other tiles (few blocks, long general-path chains) on the side stream, started
     // first, beside the bulk tiles; both read the source buffer and write disjoint nodes of
@@ -710,7 +725,7 @@ hipError_t launch_pair(Ensemble* E, const StepArgs& a2, bool single, int blocks2
     if ((e = hipStreamWaitEvent(E->side, E->fork_ev, 0)) != hipSuccess) return e;
     if ((e = launch_step2(E->storage, P->rest_rows, false, P->n_rest, ar, E->side)) != hipSuccess) return e;
     if ((e = hipEventRecord(E->join_ev, E->side)) != hipSuccess) return e;
-    if ((e = launch_step2_bulk(E->storage, P->rows, P->n_bulk, ab, E->stream)) != hipSuccess) return e;
+    if ((e = launch_step2_bulk(E->storage, P->rows, single, P->n_bulk, ab, E->stream)) != hipSuccess) return e;
     return hipStreamWaitEvent(E->stream, E->join_ev, 0);
 }
 
@@ -1005,7 +1020,7 @@ int two_step_ok(Ensemble* E, int bx, int rows, int vec)
         return fail(LBM_ERR_INVALID, "two steps per launch take shapes (64, 4, 1), (64, 8, 1), (64, 16, 1) and "
                                      "(64, 32, 1)");
     if (rows == 32 && !bulk_eligible(E, 32))
-        return fail(LBM_ERR_INVALID, "two steps per launch on 64 x 32 tiles: one lattice in FP32 or a 16-bit form");
+        return fail(LBM_ERR_INVALID, "two steps per launch on 64 x 32 tiles: FP32 or a 16-bit form");
     for (const auto& H : E->lat) {
         const lbm_desc& d = H.d;
         if (E->lat.size() == 1 && d.outlet && d.nx % 64 == 1)

@@ -76,9 +76,5 @@ struct Tile {
     int32_t lattice, y0, x0, pad;
 };
 
-// Origin of a two-step tile of a one-lattice handle (tile lists).
-struct TileXY {
-    int32_t x0, y0;
-};
 
 }  // namespace lbm

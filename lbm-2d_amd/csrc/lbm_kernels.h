@@ -24,7 +24,7 @@ struct StepArgs {
     int32_t lattice0;
     int32_t tiles_x, tiles_y;   // tiles per row band, row bands
     int32_t options;            // LAUNCH_* bits
-    const TileXY* tile_list;    // two-step bulk launches: one tile origin per block
+    const Tile* bulk_tiles;     // two-step bulk launches: one tile per block
     // general-path nodes, taken by the first special_blocks blocks: the lattice's
     // boundary-link records, its edge nodes, then its written solids (list of
     // in-lattice indices y*pitch+x)
@@ -219,13 +219,16 @@ hipError_t launch_flush(const FlushArgs& f, hipStream_t s);
 // Two steps per launch: tiles of 64 x tile_rows (4, 8 or 16), 256 threads; single: one
 // lattice (StepArgs.L0, src, dst, tiles_x), else the ensemble's two-step tile table.
 hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a, hipStream_t s);
-// Two steps per launch on the bulk tiles of a one-lattice handle (StepArgs.tile_list: 64 x
-// tile_rows tiles, 8, 16 or 32 rows, whose node ring lies inside the lattice and holds no
-// general-path node); FP32 and the 16-bit forms.
+// Two steps per launch on the bulk tiles of one lattice (single: StepArgs.L0, src, dst) or of
+// an ensemble's lattices (StepArgs.lat, parity): StepArgs.bulk_tiles, 64 x tile_rows tiles
+// (8, 16 or 32 rows) whose node ring lies inside their lattice and holds no general-path
+// node; FP32 and the 16-bit forms.
 bool bulk_supported(int storage, int tile_rows);
-hipError_t launch_step2_bulk(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s);
-// out[t] = 1 for the bulk tiles among the tiles_x x tiles_y tiles of 64 x ty (row-major).
-hipError_t launch_tile_class(const uint8_t* flags, int nx, int ny, int pitch, int tiles_x, int tiles_y, int ty,
+hipError_t launch_step2_bulk(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a,
+                             hipStream_t s);
+// out[t] = 1 for the bulk tiles among the n_tiles tiles of 64 x ty (lattice descriptors on
+// the device).
+hipError_t launch_tile_class(const Tile* tiles, int n_tiles, const LatticeDev* lat, const uint8_t* flags, int ty,
                              uint8_t* out, hipStream_t s);
 // Blocks of block_x threads that take the general-path list nodes: n_links link
 // records (kListLanes per wave), then n_edges edge nodes (every lane).

@@ -1572,17 +1572,20 @@ __host__ __device__ constexpr int bulk_threads(int bytes_per_value, int ty)
     return 9 * 66 * (ty + 2) * bytes_per_value > 24 * 1024 ? 512 : 256;
 }
 
-// Steps k and k+1 of the bulk tiles of a one-lattice handle: 64 x TY tiles whose ring of
-// nodes (66 x (TY + 2)) lies inside the lattice and holds no general-path node -- no edge,
+// Steps k and k+1 of the bulk tiles of a lattice: 64 x TY tiles whose ring of nodes
+// (66 x (TY + 2)) lies inside the lattice and holds no general-path node -- no edge,
 // boundary link, solid, inlet, outlet or lid node -- so every node of the launch is a plain
-// pull and a collision.  The host lists them (a.tile_list, origins); the other tiles go to
-// step2_kernel through a list of their own, in a launch after this one.  Without the
-// general path the kernel keeps its values and pointers in registers: the pull addresses
-// are per-plane scalar bases plus one 32-bit lane offset, the ring's row and column come
-// from 24-bit integer arithmetic, and the stochastic-rounding choice is compiled in (SR).
-// Decoding, collision and encoding are group_update's, in stored units, so the results are
-// bit for bit those of every other kernel.
-template <int ST, int TY, bool SR>
+// pull and a collision.  The host lists them (a.bulk_tiles); the other tiles go to the
+// ensemble form of step2_kernel through a table of their own, on a second stream.  ENS:
+// the tiles belong to the lattices of an ensemble (descriptor and buffer parity per tile's
+// lattice, as step2_kernel's ensemble form), else to the one lattice of a.L0 (buffers
+// a.src, a.dst).  Without the general path the kernel keeps its values and pointers in
+// registers: the pull addresses are per-plane scalar bases plus one 32-bit lane offset,
+// the ring's row and column come from 24-bit integer arithmetic, and the
+// stochastic-rounding choice is compiled in (SR).  Decoding, collision and encoding are
+// group_update's, in stored units, so the results are bit for bit those of every other
+// kernel.
+template <int ST, int TY, bool SR, bool ENS>
 __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY)) void step2_bulk_kernel(StepArgs a)
 {
     using S = typename Form<ST>::store;
@@ -1593,19 +1596,32 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
     static_assert(P2 * NT == TX * TY, "whole tiles of NT threads");
     __shared__ S lds[9][WY][WX];
 
-    const LatticeDev& L = a.L0;
     const int nb = static_cast<int>(gridDim.x);
     int b = static_cast<int>(blockIdx.x);
     if (a.options & LAUNCH_XCD_ORDER) {
         const int q = nb >> 3, r = nb & 7, xcd = b & 7;
         b = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
     }
-    const TileXY t = a.tile_list[b];
+    const Tile t = a.bulk_tiles[b];
+    const int lat_id = ENS ? t.lattice : a.lattice0;
+    LatticeDev L;
+    const S* base_src;
+    S* base_dst;
+    if constexpr (ENS) {
+        L = a.lat[lat_id];
+        const int sp = a.par[lat_id] ^ (a.flips & 1);
+        base_src = static_cast<const S*>(sp ? a.buf1 : a.buf0);
+        base_dst = static_cast<S*>(a.dst ? a.dst : (sp ? a.buf0 : a.buf1));
+    } else {
+        L = a.L0;
+        base_src = static_cast<const S*>(a.src);
+        base_dst = static_cast<S*>(a.dst);
+    }
     // ring position (lx, ly) is lattice node (x0 - 1 + lx, y0 - 1 + ly): element ly * pitch
     // + lx from the ring origin (less than (TY + 2) pitch + 66, a 32-bit offset)
     const int64_t org = static_cast<int64_t>(t.y0 - 1) * L.pitch + (t.x0 - 1);
-    const S* src = static_cast<const S*>(a.src) + L.f_off + org;
-    S* dst = static_cast<S*>(a.dst) + L.f_off + org;
+    const S* src = base_src + L.f_off + org;
+    S* dst = base_dst + L.f_off + org;
     const uint16_t* wdb = a.wdist + L.cell_off + org;
     const uint32_t cell0 = static_cast<uint32_t>(L.cell_off + org);
     const int pitch = L.pitch;
@@ -1614,8 +1630,8 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
     const R sc = static_cast<R>(a.scale);
     uint32_t key0 = 0, key1 = 0;
     if constexpr (SR) {
-        key0 = step_key(a, a.lattice0, 0);
-        key1 = step_key(a, a.lattice0, 1);
+        key0 = step_key(a, lat_id, 0);
+        key1 = step_key(a, lat_id, 1);
     }
     const int tid = static_cast<int>(threadIdx.x);
     const int tx = tid & 63, ty0 = tid >> 6;
@@ -1748,28 +1764,29 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
             store_val(reinterpret_cast<S*>(reinterpret_cast<char*>(dst + j * L.plane) + static_cast<uint32_t>(o) * sizeof(S)),
                       v[j], nt);
     }
-    if (bad) atomicOr(a.nan_flags + a.lattice0, 1);
+    if (bad) atomicOr(a.nan_flags + lat_id, 1);
 }
 
-// Bulk tiles of a one-lattice handle's two-step launches: out[t] = 1 when tile t (64 x ty,
-// row-major over tiles_x columns) has its ring of nodes inside the lattice and no
-// general-path node in it (edge nodes are general-path nodes, so the ring's pulls stay
-// inside the lattice too).
-__global__ __launch_bounds__(256) void tile_class_kernel(const uint8_t* flags, int nx, int ny, int pitch,
-                                                         int tiles_x, int ty, uint8_t* out)
+// Bulk tiles of two-step launches: out[t] = 1 when tile t of the table (64 x ty from (x0,
+// y0) of its lattice) has its ring of nodes inside the lattice and no general-path node in
+// it (edge nodes are general-path nodes, so the ring's pulls stay inside the lattice too).
+__global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, const LatticeDev* lat,
+                                                         const uint8_t* flags, int ty, uint8_t* out)
 {
-    const int t = static_cast<int>(blockIdx.x);
-    const int x0 = (t % tiles_x) * 64, y0 = (t / tiles_x) * ty;
-    const bool inside = x0 >= 1 && x0 + 64 <= nx - 1 && y0 >= 1 && y0 + ty <= ny - 1;
+    const Tile t = tiles[blockIdx.x];
+    const LatticeDev& L = lat[t.lattice];
+    const int x0 = t.x0, y0 = t.y0;
+    const bool inside = x0 >= 1 && x0 + 64 <= L.nx - 1 && y0 >= 1 && y0 + ty <= L.ny - 1;
     int special = 0;
     if (inside) {
+        const uint8_t* f = flags + L.cell_off;
         for (int r = static_cast<int>(threadIdx.x); r < 66 * (ty + 2); r += 256) {
             const int ly = r / 66, lx = r - 66 * ly;
-            if (flags[static_cast<int64_t>(y0 - 1 + ly) * pitch + (x0 - 1 + lx)] & CELL_SPECIAL) special = 1;
+            if (f[static_cast<int64_t>(y0 - 1 + ly) * L.pitch + (x0 - 1 + lx)] & CELL_SPECIAL) special = 1;
         }
     }
     const int any = __syncthreads_or(special);
-    if (threadIdx.x == 0) out[t] = inside && !any ? 1 : 0;
+    if (threadIdx.x == 0) out[blockIdx.x] = inside && !any ? 1 : 0;
 }
 
 // ------------------------------------------------------------------ state / prologue
@@ -2601,22 +2618,24 @@ hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, c
 }
 
 namespace {
-template <int ST, int TY> hipError_t launch_bulk_t(int n_blocks, const StepArgs& a, hipStream_t s)
+template <int ST, int TY, bool ENS> hipError_t launch_bulk_e(int n_blocks, const StepArgs& a, hipStream_t s)
 {
     constexpr int nt = bulk_threads(sizeof(typename Form<ST>::store), TY);
     if (Form<ST>::shifted && a.sr)
-        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, Form<ST>::shifted>), dim3(n_blocks), dim3(nt), 0, s, a);
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, Form<ST>::shifted, ENS>), dim3(n_blocks), dim3(nt), 0, s, a);
     else
-        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, false>), dim3(n_blocks), dim3(nt), 0, s, a);
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, false, ENS>), dim3(n_blocks), dim3(nt), 0, s, a);
     return hipGetLastError();
 }
 
-template <int ST> hipError_t launch_bulk_st(int ty, int n_blocks, const StepArgs& a, hipStream_t s)
+template <int ST> hipError_t launch_bulk_st(int ty, bool single, int n_blocks, const StepArgs& a, hipStream_t s)
 {
     switch (ty) {
-    case 8: return launch_bulk_t<ST, 8>(n_blocks, a, s);
-    case 16: return launch_bulk_t<ST, 16>(n_blocks, a, s);
-    case 32: return launch_bulk_t<ST, 32>(n_blocks, a, s);
+    case 8: return single ? launch_bulk_e<ST, 8, false>(n_blocks, a, s) : launch_bulk_e<ST, 8, true>(n_blocks, a, s);
+    case 16:
+        return single ? launch_bulk_e<ST, 16, false>(n_blocks, a, s) : launch_bulk_e<ST, 16, true>(n_blocks, a, s);
+    case 32:
+        return single ? launch_bulk_e<ST, 32, false>(n_blocks, a, s) : launch_bulk_e<ST, 32, true>(n_blocks, a, s);
     }
     return hipErrorInvalidValue;
 }
@@ -2627,24 +2646,23 @@ bool bulk_supported(int storage, int tile_rows)
     return storage != ST_F64 && (tile_rows == 8 || tile_rows == 16 || tile_rows == 32);
 }
 
-hipError_t launch_step2_bulk(int storage, int tile_rows, int n_blocks, const StepArgs& a, hipStream_t s)
+hipError_t launch_step2_bulk(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a, hipStream_t s)
 {
     if (n_blocks <= 0) return hipSuccess;
     switch (storage) {
-    case ST_F32: return launch_bulk_st<ST_F32>(tile_rows, n_blocks, a, s);
-    case ST_F16S: return launch_bulk_st<ST_F16S>(tile_rows, n_blocks, a, s);
-    case ST_BF16S: return launch_bulk_st<ST_BF16S>(tile_rows, n_blocks, a, s);
+    case ST_F32: return launch_bulk_st<ST_F32>(tile_rows, single, n_blocks, a, s);
+    case ST_F16S: return launch_bulk_st<ST_F16S>(tile_rows, single, n_blocks, a, s);
+    case ST_BF16S: return launch_bulk_st<ST_BF16S>(tile_rows, single, n_blocks, a, s);
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_tile_class(const uint8_t* flags, int nx, int ny, int pitch, int tiles_x, int tiles_y, int ty,
+hipError_t launch_tile_class(const Tile* tiles, int n_tiles, const LatticeDev* lat, const uint8_t* flags, int ty,
                              uint8_t* out, hipStream_t s)
 {
-    const int64_t n = static_cast<int64_t>(tiles_x) * tiles_y;
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tile_class_kernel, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, flags, nx, ny, pitch,
-                       tiles_x, ty, out);
+    if (n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tile_class_kernel, dim3(static_cast<unsigned>(n_tiles)), dim3(256), 0, s, tiles, lat, flags,
+                       ty, out);
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ import pytest
 
 import lbm2d as G
 import oracle as O
-from cases import CASES, _hills, _mask_setup, build_gpu, build_oracle
+from cases import CASES, _hills, _mask_setup, apply_gpu_geometry, build_gpu, build_oracle, gpu_desc
 
 pytestmark = pytest.mark.gpu
 
@@ -124,8 +124,8 @@ def test_bulk_tiles_fp32_against_oracle():
 
 
 def test_bulk_plan_follows_geometry_and_refusals():
-    """A new obstacle rebuilds the tile classes; 64 x 32 tiles need the split (one lattice,
-    FP32 or a 16-bit form)."""
+    """A new obstacle rebuilds the tile classes; 64 x 32 tiles need the split (FP32 or a
+    16-bit form)."""
     c = BULK_CASES["cylinder"]
     ens, o = build_gpu(c, "f32")
     ens.set_launch(64, 16, 1, 5)
@@ -146,7 +146,77 @@ def test_bulk_plan_follows_geometry_and_refusals():
     with pytest.raises(G.LbmError):
         f64.set_launch(64, 32, 1, 5)
     f64.close()
-    two = G.Ensemble([G.desc_for_case(G.CYLINDER, 256, 128)] * 2, storage="f32")
+    two = G.Ensemble([G.desc_for_case(G.CYLINDER, 256, 128)] * 2, storage="f64")
     with pytest.raises(G.LbmError):
         two.set_launch(64, 32, 1, 5)
     two.close()
+
+
+# an ensemble of the scaled-up cases plus a channel whose last column block holds only the
+# outlet column (nx % 64 == 1: its two-step tiles are right-aligned)
+NARROW_BIG = dataclasses.replace(BULK_CASES["les"], name="les_big_321", nx=321, ny=130, pert=(100, 200),
+                                 setup=lambda s: s.place_cylinder(80, 65, 10),
+                                 gpu_geom=(("circle", 80, 65, 10, G.BB_MEI),))
+
+
+def _ens_cases():
+    return [BULK_CASES[n] for n in sorted(BULK_CASES)] + [NARROW_BIG]
+
+
+def _ensemble(cs, orcs, storage):
+    ens = G.Ensemble([gpu_desc(c) for c in cs], storage=storage)
+    for l, (c, o) in enumerate(zip(cs, orcs)):
+        apply_gpu_geometry(ens, l, c, o.obstacle.copy())
+        ens.upload(l, o.f.copy())
+    return ens
+
+
+@pytest.mark.parametrize("storage,sr", [("f32", False), ("f16s", True), ("bf16s", False)],
+                         ids=["f32", "f16s_sr", "bf16s"])
+def test_ensemble_bulk_tiles_equal_one_step_per_launch(storage, sr):
+    """An ensemble's two-step launches split into bulk and other tiles too: every lattice
+    holds exactly what the same ensemble holds stepped one step per launch (populations,
+    forces), and without stochastic rounding what the lattice holds stepped alone (the
+    random words of stochastic rounding follow a node's place in its handle)."""
+    cs = _ens_cases()
+    orcs = [build_oracle(c) for c in cs]
+    ens = _ensemble(cs, orcs, storage)
+    ref = _ensemble(cs, orcs, storage)
+    ref.set_launch(64, 1, 1, 0)
+    alone = []
+    if not sr:
+        for c, o in zip(cs, orcs):
+            a, _ = build_gpu(c, storage, o=o)
+            a.set_launch(64, 1, 1, 0)
+            alone.append(a)
+    else:
+        ens.set_stochastic_rounding(True, 5)
+        ref.set_stochastic_rounding(True, 5)
+    seen = 0
+    for rows, opt in ((16, 5), (8, 6), (32, 7)):
+        ens.set_launch(64, rows, 1, opt, graph_steps=10)
+        seen += ens.info().bulk_tiles
+        for n in (1, 5, 12):
+            ens.step(n)
+            ref.step(n)
+            for a in alone:
+                a.step(n)
+            for l in range(len(cs)):
+                got = ens.download(l)
+                np.testing.assert_array_equal(got, ref.download(l), err_msg=f"lattice {l} rows {rows} n {n}")
+                if alone:
+                    np.testing.assert_array_equal(got, alone[l].download(0), err_msg=f"alone {l} rows {rows} n {n}")
+    assert seen > 0
+    for l in range(len(cs)):
+        fe, fye = ens.forces(l)
+        fr, fyr = ref.forces(l)
+        np.testing.assert_array_equal(fe, fr)
+        np.testing.assert_array_equal(fye, fyr)
+        if alone:
+            fa, fya = alone[l].forces(0)
+            np.testing.assert_array_equal(fe, fa)
+        assert not ens.nan_flag(l)
+    for a in alone:
+        a.close()
+    ens.close()
+    ref.close()
