@@ -51,6 +51,7 @@ constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 constexpr int kHistoryRows = 4096;
 // Device time per candidate launch shape in lbm_autotune (ms).
 constexpr float kTuneMs = 1.5f;
+constexpr int kStripTiles = 16;  // bulk-tile strip width in 64-node tiles
 constexpr int kMinBatch = 32;  // accumulator blocks when graphs are shorter or off
 constexpr int64_t kStagingBytes = 256ll << 20;
 
@@ -685,6 +686,17 @@ int build_plan(Ensemble* E, int rows)
             for (int y = t.y0; y < std::min(t.y0 + rows, ny); y += rest_rows) rest.push_back(Tile{t.lattice, y, t.x0, 0});
         }
     }
+    // Bulk tiles in vertical strips of kStripTiles tile columns, row band after row band
+    // inside a strip: a tile's ring rows are read again by the tiles above and below it,
+    // which then follow a strip's width later (in L2) rather than a lattice row of tiles
+    // later.  Blocks go to the XCDs in contiguous ranges of this order (LAUNCH_XCD_ORDER).
+    std::stable_sort(bulk.begin(), bulk.end(), [](const Tile& p, const Tile& q) {
+        if (p.lattice != q.lattice) return p.lattice < q.lattice;
+        const int sp = p.x0 / (64 * kStripTiles), sq = q.x0 / (64 * kStripTiles);
+        if (sp != sq) return sp < sq;
+        if (p.y0 != q.y0) return p.y0 < q.y0;
+        return p.x0 < q.x0;
+    });
     if ((rc = dev_alloc(E, &P.d, bulk.size())) != LBM_OK) return rc;
     if ((rc = h2d(E, P.d, bulk.data(), bulk.size() * sizeof(Tile))) != LBM_OK) return rc;
     if ((rc = dev_alloc(E, &P.dr, rest.size())) != LBM_OK) return rc;
