@@ -154,21 +154,22 @@ struct Ensemble {
     Tile* d_tiles2 = nullptr;      // ensembles with two steps per launch: 64 x rows tiles
     size_t tiles2_cap = 0;
     int n_tiles2 = 0;
-    // two-step launches, per tile height (8, 16, 32): the bulk tiles (step2_bulk_kernel)
-    // and the others (the ensemble form of step2_kernel on 64 x rest_rows tiles, rest_rows
-    // = min(rows, 16)), classified from the flags
+    // two-step launches, per tile height (8, 16, 32): the bulk tiles (step2_bulk_kernel),
+    // the edge tiles (its edge form) and the others (the ensemble form of step2_kernel on
+    // 64 x rest_rows tiles, rest_rows = min(rows, 16)), classified from the flags
     struct BulkPlan {
         int rows = 0;  // 0: not built
         int rest_rows = 0;
         Tile* d = nullptr;     // bulk tiles
-        Tile* dr = nullptr;    // the others, as an ensemble tile table of lattice 0
-        int n_bulk = 0, n_rest = 0;
+        Tile* de = nullptr;    // edge tiles
+        Tile* dr = nullptr;    // the others, as an ensemble tile table
+        int n_bulk = 0, n_edge = 0, n_rest = 0;
     };
     BulkPlan plan[3];
-    // a second stream for the other tiles of a split launch, so that they run beside the
-    // bulk tiles (fork and join by events; graph capture records them as two branches)
-    hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // two more streams for the edge and the other tiles of a split launch, so that they run
+    // beside the bulk tiles (fork and join by events; graph capture records the branches)
+    hipStream_t side = nullptr, side2 = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, join2_ev = nullptr;
     int64_t* d_steps = nullptr;
     int32_t* d_par = nullptr;
     double* d_acc = nullptr;       // [acc_blocks][kAccCopies][total_slots][2]
@@ -620,10 +621,18 @@ bool bulk_eligible(const Ensemble* E, int rows)
     return true;
 }
 
+// Whether a lattice's edge tiles can take the bulk kernel's edge form: its pull addresses
+// are 32-bit byte offsets within a plane and 24-bit row products.
+bool edge_tiles_fit(const Ensemble* E, const HostLattice& H)
+{
+    return static_cast<int64_t>(H.d.ny + 1) * H.pitch * E->bpp < (int64_t(1) << 32) && H.d.ny < (1 << 24);
+}
+
 void drop_plans(Ensemble* E)
 {
     for (auto& p : E->plan) {
         if (p.d) (void)hipFree(p.d);
+        if (p.de) (void)hipFree(p.de);
         if (p.dr) (void)hipFree(p.dr);
         p = Ensemble::BulkPlan{};
     }
@@ -631,10 +640,11 @@ void drop_plans(Ensemble* E)
 
 // Classifies the 64 x rows two-step tiles of every lattice (rows of an ensemble lattice
 // with an outlet and nx % 64 == 1 right-aligned, as its two-step tile table) from the
-// device flags, and uploads the bulk tiles, and the other tiles cut to rest_rows rows as a
-// tile table for the ensemble form of the two-step kernel (which takes inlet, outlet and
-// periodic-edge nodes on its plain path and queues the remaining general-path nodes of a
-// tile: the other tiles are those holding such nodes).
+// device flags, and uploads the bulk tiles, the edge tiles, and the other tiles cut to
+// rest_rows rows as a tile table for the ensemble form of the two-step kernel (which takes
+// inlet, outlet and periodic-edge nodes on its plain path and queues the remaining
+// general-path nodes of a tile: the other tiles are those holding such nodes, or partial
+// tiles).
 int build_plan(Ensemble* E, int rows)
 {
     if (!E->side) {
@@ -642,12 +652,18 @@ int build_plan(Ensemble* E, int rows)
             E->side = nullptr;
             return fail(LBM_ERR_DEVICE, "hipStreamCreateWithFlags: side stream");
         }
+        if (hipStreamCreateWithFlags(&E->side2, hipStreamNonBlocking) != hipSuccess) {
+            E->side2 = nullptr;
+            return fail(LBM_ERR_DEVICE, "hipStreamCreateWithFlags: side stream");
+        }
         if (hipEventCreateWithFlags(&E->fork_ev, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&E->join_ev, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&E->join_ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&E->join2_ev, hipEventDisableTiming) != hipSuccess)
             return fail(LBM_ERR_DEVICE, "hipEventCreateWithFlags: fork / join events");
     }
     Ensemble::BulkPlan& P = E->plan[plan_slot(rows)];
     if (P.d) (void)hipFree(P.d);
+    if (P.de) (void)hipFree(P.de);
     if (P.dr) (void)hipFree(P.dr);
     P = Ensemble::BulkPlan{};
     std::vector<Tile> tiles;
@@ -676,11 +692,13 @@ int build_plan(Ensemble* E, int rows)
     (void)hipFree(d_cls);
     if (rc != LBM_OK) return rc;
     const int rest_rows = std::min(rows, 16);
-    std::vector<Tile> bulk, rest;
+    std::vector<Tile> bulk, edge, rest;
     for (size_t k = 0; k < n; ++k) {
         const Tile& t = tiles[k];
-        if (cls[k]) {
+        if (cls[k] == 1) {
             bulk.push_back(t);
+        } else if (cls[k] == 2 && edge_tiles_fit(E, E->lat[t.lattice])) {
+            edge.push_back(t);
         } else {
             const int ny = E->lat[t.lattice].d.ny;
             for (int y = t.y0; y < std::min(t.y0 + rows, ny); y += rest_rows) rest.push_back(Tile{t.lattice, y, t.x0, 0});
@@ -699,11 +717,14 @@ int build_plan(Ensemble* E, int rows)
     });
     if ((rc = dev_alloc(E, &P.d, bulk.size())) != LBM_OK) return rc;
     if ((rc = h2d(E, P.d, bulk.data(), bulk.size() * sizeof(Tile))) != LBM_OK) return rc;
+    if ((rc = dev_alloc(E, &P.de, edge.size())) != LBM_OK) return rc;
+    if ((rc = h2d(E, P.de, edge.data(), edge.size() * sizeof(Tile))) != LBM_OK) return rc;
     if ((rc = dev_alloc(E, &P.dr, rest.size())) != LBM_OK) return rc;
     if ((rc = h2d(E, P.dr, rest.data(), rest.size() * sizeof(Tile))) != LBM_OK) return rc;
     P.rows = rows;
     P.rest_rows = rest_rows;
     P.n_bulk = static_cast<int>(bulk.size());
+    P.n_edge = static_cast<int>(edge.size());
     P.n_rest = static_cast<int>(rest.size());
     return LBM_OK;
 }
@@ -716,29 +737,57 @@ const Ensemble::BulkPlan* current_plan(const Ensemble* E)
     return P.rows == E->rows ? &P : nullptr;
 }
 
-// One two-step launch of a batch: with a plan, the bulk tiles and the others beside them;
-// else the grid of tiles (one lattice) or the ensemble's tile table.
+// One two-step launch of a batch: with a plan, the bulk tiles, the edge tiles and the
+// others beside one another; else the grid of tiles (one lattice) or the ensemble's tile
+// table.
 hipError_t launch_pair(Ensemble* E, const StepArgs& a2, bool single, int blocks2)
 {
     const Ensemble::BulkPlan* P = current_plan(E);
     if (!P) return launch_step2(E->storage, E->rows, single, blocks2, a2, E->stream);
     StepArgs ab = a2;
     ab.bulk_tiles = P->d;
+    StepArgs ae = a2;
+    ae.bulk_tiles = P->de;
     StepArgs ar = a2;
     ar.tiles = P->dr;
     ar.n_tiles = P->n_rest;
-    if (P->n_rest == 0) return launch_step2_bulk(E->storage, P->rows, single, P->n_bulk, ab, E->stream);
-    if (P->n_bulk == 0) return launch_step2(E->storage, P->rest_rows, false, P->n_rest, ar, E->stream);
-    // the other tiles (few blocks, long general-path chains) on the side stream, started
-    // first, beside the bulk tiles; both read the source buffer and write disjoint nodes of
-    // the destination, and the next launch on the stream waits for both
+    // the kernels with tiles, the bulk tiles' first: that one on the handle's stream, the
+    // others (few blocks: the edge tiles, the other tiles' long general-path chains) on the
+    // side streams, beside it; all read the source buffer and write disjoint nodes of the
+    // destination, and the next launch on the stream waits for all.  The dispatcher keeps
+    // serving the kernel that reached it first: if the bulk kernel's thousands of blocks
+    // come first, the other kernels' blocks wait for its last ones and the launch runs as
+    // long as their chains after it (on the 6400 x 2400 lattice 205 against 60 us).  A
+    // workgroup with no work goes ahead of the bulk kernel on the handle's stream (graph
+    // replays start their parallel branches in no fixed order), so that the side streams'
+    // kernels are dispatched while it runs.
+    auto launch = [&](int kind, hipStream_t s) {
+        if (kind == 0) return launch_step2_bulk(E->storage, P->rows, single, false, P->n_bulk, ab, s);
+        if (kind == 1) return launch_step2_bulk(E->storage, P->rows, single, true, P->n_edge, ae, s);
+        return launch_step2(E->storage, P->rest_rows, false, P->n_rest, ar, s);
+    };
+    int kinds[3], n = 0;
+    if (P->n_bulk) kinds[n++] = 0;
+    if (P->n_edge) kinds[n++] = 1;
+    if (P->n_rest) kinds[n++] = 2;
+    if (n == 0) return hipSuccess;
     hipError_t e;
-    if ((e = hipEventRecord(E->fork_ev, E->stream)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(E->side, E->fork_ev, 0)) != hipSuccess) return e;
-    if ((e = launch_step2(E->storage, P->rest_rows, false, P->n_rest, ar, E->side)) != hipSuccess) return e;
-    if ((e = hipEventRecord(E->join_ev, E->side)) != hipSuccess) return e;
-    if ((e = launch_step2_bulk(E->storage, P->rows, single, P->n_bulk, ab, E->stream)) != hipSuccess) return e;
-    return hipStreamWaitEvent(E->stream, E->join_ev, 0);
+    if (n > 1) {
+        if ((e = hipEventRecord(E->fork_ev, E->stream)) != hipSuccess) return e;
+        hipStream_t sides[2] = {E->side, E->side2};
+        hipEvent_t joins[2] = {E->join_ev, E->join2_ev};
+        for (int k = 1; k < n; ++k) {
+            if ((e = hipStreamWaitEvent(sides[k - 1], E->fork_ev, 0)) != hipSuccess) return e;
+            if ((e = launch(kinds[k], sides[k - 1])) != hipSuccess) return e;
+            if ((e = hipEventRecord(joins[k - 1], sides[k - 1])) != hipSuccess) return e;
+        }
+        if ((e = launch_hold(E->stream)) != hipSuccess) return e;
+        if ((e = launch(kinds[0], E->stream)) != hipSuccess) return e;
+        for (int k = 1; k < n; ++k)
+            if ((e = hipStreamWaitEvent(E->stream, joins[k - 1], 0)) != hipSuccess) return e;
+        return hipSuccess;
+    }
+    return launch(kinds[0], E->stream);
 }
 
 int commit(Ensemble* E)
@@ -1588,12 +1637,13 @@ int lbm_destroy(lbm_handle h)
         if (p) (void)hipFree(p);
     lbm::drop_plans(E);
     for (auto& H : E->lat) lbm::drop_new_geometry(H);
-    if (E->side) {
-        (void)hipStreamSynchronize(E->side);
-        (void)hipStreamDestroy(E->side);
-    }
-    if (E->fork_ev) (void)hipEventDestroy(E->fork_ev);
-    if (E->join_ev) (void)hipEventDestroy(E->join_ev);
+    for (hipStream_t s : {E->side, E->side2})
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    for (hipEvent_t ev : {E->fork_ev, E->join_ev, E->join2_ev})
+        if (ev) (void)hipEventDestroy(ev);
     if (E->own_stream) (void)hipStreamDestroy(E->stream);
     delete h;
     return LBM_OK;
@@ -2417,6 +2467,7 @@ int lbm_get_info(lbm_handle h, lbm_info* out)
     if (const Ensemble::BulkPlan* P = current_plan(h)) {
         r.bulk_tiles = P->n_bulk;
         r.other_tiles = P->n_rest;
+        r.edge_tiles = P->n_edge;
     }
     *out = r;
     return LBM_OK;

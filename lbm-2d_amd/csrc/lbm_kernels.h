@@ -222,12 +222,16 @@ hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, c
 // Two steps per launch on the bulk tiles of one lattice (single: StepArgs.L0, src, dst) or of
 // an ensemble's lattices (StepArgs.lat, parity): StepArgs.bulk_tiles, 64 x tile_rows tiles
 // (8, 16 or 32 rows) whose node ring lies inside their lattice and holds no general-path
-// node; FP32 and the 16-bit forms.
+// node; FP32 and the 16-bit forms.  edge: StepArgs.bulk_tiles holds edge tiles instead
+// (class 2 of launch_tile_class), whose lattices' planes take 32-bit byte offsets.
 bool bulk_supported(int storage, int tile_rows);
-hipError_t launch_step2_bulk(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a,
+hipError_t launch_step2_bulk(int storage, int tile_rows, bool single, bool edge, int n_blocks, const StepArgs& a,
                              hipStream_t s);
-// out[t] = 1 for the bulk tiles among the n_tiles tiles of 64 x ty (lattice descriptors on
-// the device).
+// One workgroup that does nothing (orders the kernels of a split launch's streams).
+hipError_t launch_hold(hipStream_t s);
+// Classes of the n_tiles tiles of 64 x ty (lattice descriptors on the device): out[t] = 1
+// for a bulk tile, 2 for an edge tile (a whole tile whose ring holds no general-path node
+// but inlet, outlet and periodic-edge nodes), 0 for the others.
 hipError_t launch_tile_class(const Tile* tiles, int n_tiles, const LatticeDev* lat, const uint8_t* flags, int ty,
                              uint8_t* out, hipStream_t s);
 // Blocks of block_x threads that take the general-path list nodes: n_links link

@@ -512,11 +512,12 @@ __device__ __forceinline__ void gather_general(const Ctx<ST>& c, int px, int py,
 }
 
 // Zou/He velocity inlet on deviations (src/lbm.hpp:21-34 / 40-58); w1=w3, w5=w7=w6=w8
-// so the closure keeps its form, and w0+w2+w4+2(w3+w6+w7) = 1.
-template <typename R> __device__ __forceinline__ void zou_he(R d[9], R u)
+// so the closure keeps its form, and w0+w2+w4+2(w3+w6+w7) = 1.  one: 1, or the storage
+// scale 2^k for deviations in stored units (every term then scales exactly by 2^k).
+template <typename R> __device__ __forceinline__ void zou_he(R d[9], R u, R one = R(1))
 {
 #pragma clang fp contract(off)
-    const R rho = (R(1) + d[0] + d[2] + d[4] + R(2) * (d[3] + d[6] + d[7])) / (R(1) - u);
+    const R rho = (one + d[0] + d[2] + d[4] + R(2) * (d[3] + d[6] + d[7])) / (R(1) - u);
     d[1] = d[3] + R(2.0 / 3.0) * rho * u;
     d[5] = d[7] + R(1.0 / 6.0) * rho * u;
     d[8] = d[6] + R(1.0 / 6.0) * rho * u;
@@ -533,8 +534,9 @@ template <int ST> __device__ __forceinline__ int gathered_x(const LatticeDev& L,
 }
 
 // The velocity inlet at row y of a fluid node on x = 0: Zou/He with u_in, or the
-// parabolic profile on rows y >= inlet_y0 (src/lbm.hpp:21-58).
-template <typename R> __device__ __forceinline__ void inlet_closure(const LatticeDev& L, int y, R d[9])
+// parabolic profile on rows y >= inlet_y0 (src/lbm.hpp:21-58).  one: as zou_he's (the
+// equilibrium inlet's deviations are multiplied by it, an exact identity for 1).
+template <typename R> __device__ __forceinline__ void inlet_closure(const LatticeDev& L, int y, R d[9], R one = R(1))
 {
 #pragma clang fp contract(off)
     if (L.inlet == INLET_EQUILIBRIUM) {
@@ -543,14 +545,14 @@ template <typename R> __device__ __forceinline__ void inlet_closure(const Lattic
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             const R eu = R(CX[i]) * u;
-            d[i] = wgt<R>(i) * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * u * u);
+            d[i] = (wgt<R>(i) * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * u * u)) * one;
         }
     } else if (L.inlet == INLET_ZOU_HE) {
-        zou_he<R>(d, static_cast<R>(L.u_in));
+        zou_he<R>(d, static_cast<R>(L.u_in), one);
     } else if (y >= L.inlet_y0) {
         const double yy = static_cast<double>(y - L.inlet_y0);
         const double h = static_cast<double>(L.inlet_h);
-        zou_he<R>(d, static_cast<R>(L.u_in * 4.0 * yy * (h - yy) / (h * h)));
+        zou_he<R>(d, static_cast<R>(L.u_in * 4.0 * yy * (h - yy) / (h * h)), one);
     }
 }
 
@@ -1585,7 +1587,17 @@ __host__ __device__ constexpr int bulk_threads(int bytes_per_value, int ty)
 // stochastic-rounding choice is compiled in (SR).  Decoding, collision and encoding are
 // group_update's, in stored units, so the results are bit for bit those of every other
 // kernel.
-template <int ST, int TY, bool SR, bool ENS>
+//
+// EDGE: the edge tiles instead (a.bulk_tiles holds them) -- whole 64 x TY tiles of their
+// lattice whose ring holds, besides plain nodes, only inlet and outlet column nodes
+// (CELL_COLUMN) and nodes whose general path reduces to wrapped pulls (wrap_only).  Ring
+// positions wrap around periodic edges and are skipped off the others (only an inlet
+// node's closed slots read there); an inlet node closes its populations in stored units
+// (inlet_closure scaled by 2^k: the bits of the closure in physical units, scaled
+// exactly); an outlet node takes the pulls of the node before it (the outflow copy).  Pull
+// addresses are whole node offsets within a plane, from the three source rows and columns
+// of the node, wrapped.
+template <int ST, int TY, bool SR, bool ENS, bool EDGE>
 __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY)) void step2_bulk_kernel(StepArgs a)
 {
     using S = typename Form<ST>::store;
@@ -1618,8 +1630,9 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
         base_dst = static_cast<S*>(a.dst);
     }
     // ring position (lx, ly) is lattice node (x0 - 1 + lx, y0 - 1 + ly): element ly * pitch
-    // + lx from the ring origin (less than (TY + 2) pitch + 66, a 32-bit offset)
-    const int64_t org = static_cast<int64_t>(t.y0 - 1) * L.pitch + (t.x0 - 1);
+    // + lx from the ring origin (less than (TY + 2) pitch + 66, a 32-bit offset).  EDGE:
+    // the origin is the plane's first node, offsets are node offsets in the plane
+    const int64_t org = EDGE ? 0 : static_cast<int64_t>(t.y0 - 1) * L.pitch + (t.x0 - 1);
     const S* src = base_src + L.f_off + org;
     S* dst = base_dst + L.f_off + org;
     const uint16_t* wdb = a.wdist + L.cell_off + org;
@@ -1636,18 +1649,42 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
     const int tid = static_cast<int>(threadIdx.x);
     const int tx = tid & 63, ty0 = tid >> 6;
     constexpr int RS = NT / 64;  // tile rows per step-2 iteration
+    // offset of the tile node at ring position (lx, ly) from the origin
+    auto tile_off = [&](int lx, int ly) -> uint32_t {
+        if constexpr (EDGE)
+            return __umul24(t.y0 - 1 + ly, pitch) + static_cast<uint32_t>(t.x0 - 1 + lx);
+        else
+            return __umul24(ly, pitch) + static_cast<uint32_t>(lx);
+    };
+    // EDGE: lattice node (gx, gy) of ring position (lx, ly), wrapped around every edge;
+    // false off a non-periodic one
+    auto ring_xy = [&](int lx, int ly, int& gx, int& gy) {
+        gx = t.x0 - 1 + lx;
+        gy = t.y0 - 1 + ly;
+        bool in = true;
+        if (gx < 0 || gx >= L.nx) {
+            in = L.x_edge == EDGE_PERIODIC;
+            gx = gx < 0 ? gx + L.nx : gx - L.nx;
+        }
+        if (gy < 0 || gy >= L.ny) {
+            in = in && L.y_edge == EDGE_PERIODIC;
+            gy = gy < 0 ? gy + L.ny : gy - L.ny;
+        }
+        return in;
+    };
 
     // the second step's wall distances, fetched with the first step's loads
     uint16_t wd2[P2];
 #pragma unroll
     for (int k = 0; k < P2; ++k) {
         wd2[k] = 0xffffu;
-        if (les) wd2[k] = load_at(wdb, static_cast<uint32_t>(((ty0 + RS * k + 1) * pitch + tx + 1) * 2));
+        if (les) wd2[k] = load_at(wdb, tile_off(tx + 1, ty0 + RS * k + 1) * 2u);
     }
 
     // decode, collide, encode one node in place (stored units, as group_update; the shift
-    // additions are skipped without a shift, where they are exact identities)
-    auto update = [&](S v[9], uint16_t wraw, uint32_t key, uint32_t cell, bool& bad) {
+    // additions are skipped without a shift, where they are exact identities).  inlet
+    // (EDGE): the node is an inlet node of row y
+    auto update = [&](S v[9], uint16_t wraw, uint32_t key, uint32_t cell, bool inlet, int y, bool& bad) {
         R d[9];
 #pragma unroll
         for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec_raw(v[j]);
@@ -1656,6 +1693,9 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
 #pragma unroll
                 for (int j = 0; j < 9; ++j) d[j] += L.shift_s[j];
             }
+        }
+        if constexpr (EDGE) {
+            if (inlet) inlet_closure<R>(L, y, d, sc);
         }
         R wd = R(0);
         if (les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
@@ -1701,11 +1741,31 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
     using W = typename std::conditional<sizeof(S) == 2, uint32_t, S>::type;
     auto fetch = [&](int r, W v[9], uint32_t& wraw) {
         const int ly = ring_row(r), lx = r - 66 * ly;
-        const uint32_t o = __umul24(ly, pitch) + lx;  // pitch < 2^24
+        if constexpr (EDGE) {
+            int gx, gy;
+            ring_xy(lx, ly, gx, gy);
+            // the gathered column (x - 1 on the outlet column) and the source rows and
+            // columns around it, wrapped (across an open edge too: an inlet node's sources
+            // there are replaced by its closure, and the address stays in the plane)
+            const int px = (L.outlet && gx == L.nx - 1) ? gx - 1 : gx;
+            const uint32_t xl = px == 0 ? L.nx - 1 : px - 1, xr = px == L.nx - 1 ? 0 : px + 1;
+            const uint32_t rd = __umul24(gy == 0 ? L.ny - 1 : gy - 1, pitch), r0 = __umul24(gy, pitch),
+                           ru = __umul24(gy == L.ny - 1 ? 0 : gy + 1, pitch);
 #pragma unroll
-        for (int j = 0; j < 9; ++j) v[j] = load_at(src + j * L.plane - (CY[j] * pitch + CX[j]), o * sizeof(S));
-        wraw = 0xffffu;
-        if (les) wraw = load_at(wdb, o * 2u);
+            for (int j = 0; j < 9; ++j) {
+                const uint32_t row = CY[j] == 1 ? rd : (CY[j] == 0 ? r0 : ru);
+                const uint32_t col = CX[j] == 1 ? xl : (CX[j] == 0 ? static_cast<uint32_t>(px) : xr);
+                v[j] = load_at(src + j * L.plane, (row + col) * sizeof(S));
+            }
+            wraw = 0xffffu;
+            if (les) wraw = load_at(wdb, (r0 + gx) * 2u);
+        } else {
+            const uint32_t o = __umul24(ly, pitch) + lx;  // pitch < 2^24
+#pragma unroll
+            for (int j = 0; j < 9; ++j) v[j] = load_at(src + j * L.plane - (CY[j] * pitch + CX[j]), o * sizeof(S));
+            wraw = 0xffffu;
+            if (les) wraw = load_at(wdb, o * 2u);
+        }
     };
 
     bool bad = false;
@@ -1728,12 +1788,18 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
         if (k + PF < P1) fetch(r + NT * PF < NR ? r + NT * PF : NR - 1, nxt, wnxt);
         if (r < NR) {
             const int ly = ring_row(r), lx = r - 66 * ly;
-            S v[9];
+            int gx = 0, gy = 0;
+            const bool in = EDGE ? ring_xy(lx, ly, gx, gy) : true;
+            if (in) {
+                S v[9];
 #pragma unroll
-            for (int j = 0; j < 9; ++j) v[j] = static_cast<S>(q[0][j]);
-            update(v, static_cast<uint16_t>(wq[0]), key0, cell0 + __umul24(ly, pitch) + lx, bad);
+                for (int j = 0; j < 9; ++j) v[j] = static_cast<S>(q[0][j]);
+                const uint32_t cell = EDGE ? cell0 + __umul24(gy, pitch) + static_cast<uint32_t>(gx)
+                                           : cell0 + __umul24(ly, pitch) + static_cast<uint32_t>(lx);
+                update(v, static_cast<uint16_t>(wq[0]), key0, cell, EDGE && gx == 0 && L.inlet != INLET_NONE, gy, bad);
 #pragma unroll
-            for (int j = 0; j < 9; ++j) lds[j][ly][lx] = v[j];
+                for (int j = 0; j < 9; ++j) lds[j][ly][lx] = v[j];
+            }
         }
 #pragma unroll
         for (int p = 0; p + 1 < PF; ++p) {
@@ -1750,26 +1816,38 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
 #pragma unroll 1
     for (int k = 0; k < P2; ++k) {
         const int lx = tx + 1, ly = ty0 + RS * k + 1;
-        const int o = static_cast<int>(__umul24(ly, pitch)) + lx;
+        const uint32_t o = tile_off(lx, ly);
         uint16_t wraw = wd2[0];
 #pragma unroll
         for (int q = 1; q < P2; ++q)
             if (k == q) wraw = wd2[q];
+        // EDGE: the outlet column pulls those of x - 1 (the ring holds them), an inlet node
+        // closes its populations
+        int slx = lx, y = 0;
+        bool inlet = false;
+        if constexpr (EDGE) {
+            const int x = t.x0 + tx;
+            y = t.y0 + ly - 1;
+            if (L.outlet && x == L.nx - 1) slx = lx - 1;
+            inlet = x == 0 && L.inlet != INLET_NONE;
+        }
         S v[9];
 #pragma unroll
-        for (int j = 0; j < 9; ++j) v[j] = lds[j][ly - CY[j]][lx - CX[j]];
-        update(v, wraw, key1, cell0 + static_cast<uint32_t>(o), bad);
+        for (int j = 0; j < 9; ++j) v[j] = lds[j][ly - CY[j]][slx - CX[j]];
+        update(v, wraw, key1, cell0 + o, inlet, y, bad);
 #pragma unroll
         for (int j = 0; j < 9; ++j)
-            store_val(reinterpret_cast<S*>(reinterpret_cast<char*>(dst + j * L.plane) + static_cast<uint32_t>(o) * sizeof(S)),
-                      v[j], nt);
+            store_val(reinterpret_cast<S*>(reinterpret_cast<char*>(dst + j * L.plane) + o * sizeof(S)), v[j], nt);
     }
     if (bad) atomicOr(a.nan_flags + lat_id, 1);
 }
 
-// Bulk tiles of two-step launches: out[t] = 1 when tile t of the table (64 x ty from (x0,
-// y0) of its lattice) has its ring of nodes inside the lattice and no general-path node in
-// it (edge nodes are general-path nodes, so the ring's pulls stay inside the lattice too).
+// Classes of two-step tiles (64 x ty from (x0, y0) of its lattice): out[t] = 1 (bulk) when
+// the tile's ring of nodes lies inside the lattice and holds no general-path node (edge
+// nodes are general-path nodes, so the ring's pulls stay inside the lattice too); 2 (edge)
+// when the tile lies whole in its lattice and every node of its ring, wrapped around
+// periodic edges and skipped off the others, is plain, an inlet or outlet column node or a
+// node whose general path reduces to wrapped pulls; else 0.
 __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, const LatticeDev* lat,
                                                          const uint8_t* flags, int ty, uint8_t* out)
 {
@@ -1777,16 +1855,36 @@ __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, cons
     const LatticeDev& L = lat[t.lattice];
     const int x0 = t.x0, y0 = t.y0;
     const bool inside = x0 >= 1 && x0 + 64 <= L.nx - 1 && y0 >= 1 && y0 + ty <= L.ny - 1;
-    int special = 0;
-    if (inside) {
+    const bool whole = x0 >= 0 && x0 + 64 <= L.nx && y0 >= 0 && y0 + ty <= L.ny;
+    int special = 0, general = 0;
+    if (whole) {
         const uint8_t* f = flags + L.cell_off;
         for (int r = static_cast<int>(threadIdx.x); r < 66 * (ty + 2); r += 256) {
             const int ly = r / 66, lx = r - 66 * ly;
-            if (f[static_cast<int64_t>(y0 - 1 + ly) * L.pitch + (x0 - 1 + lx)] & CELL_SPECIAL) special = 1;
+            int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+            bool in = true;
+            if (gx < 0 || gx >= L.nx) {
+                in = L.x_edge == EDGE_PERIODIC;
+                gx = gx < 0 ? gx + L.nx : gx - L.nx;
+            }
+            if (gy < 0 || gy >= L.ny) {
+                in = in && L.y_edge == EDGE_PERIODIC;
+                gy = gy < 0 ? gy + L.ny : gy - L.ny;
+            }
+            if (!in) {
+                special = 1;
+                continue;
+            }
+            const uint8_t fl = f[static_cast<int64_t>(gy) * L.pitch + gx];
+            if (fl & CELL_SPECIAL) {
+                special = 1;
+                if (!(fl & CELL_COLUMN) && !wrap_only(L, fl, gx, gy)) general = 1;
+            }
         }
     }
     const int any = __syncthreads_or(special);
-    if (threadIdx.x == 0) out[blockIdx.x] = inside && !any ? 1 : 0;
+    const int gen = __syncthreads_or(general);
+    if (threadIdx.x == 0) out[blockIdx.x] = !whole ? 0 : (inside && !any ? 1 : (gen ? 0 : 2));
 }
 
 // ------------------------------------------------------------------ state / prologue
@@ -2441,6 +2539,11 @@ __global__ __launch_bounds__(256) void wall_cols_kernel(uint16_t* wd, int nx, in
 
 // Advances the per-lattice step counters after a batch of launches; an odd number of
 // launches (a two-step launch swaps the buffers once) leaves the state in the other buffer.
+// No work: a split two-step launch runs it on the handle's stream between the fork and the
+// bulk tiles' kernel, so that the kernels of the other streams, launched at the fork, reach
+// the dispatcher first.
+__global__ void hold_kernel() {}
+
 __global__ void add_steps_kernel(int64_t* steps, int32_t* par, int32_t begin, int32_t count, int64_t inc,
                                  int32_t flips)
 {
@@ -2618,24 +2721,30 @@ hipError_t launch_step2(int storage, int tile_rows, bool single, int n_blocks, c
 }
 
 namespace {
-template <int ST, int TY, bool ENS> hipError_t launch_bulk_e(int n_blocks, const StepArgs& a, hipStream_t s)
+template <int ST, int TY, bool ENS, bool EDGE> hipError_t launch_bulk_e(int n_blocks, const StepArgs& a, hipStream_t s)
 {
     constexpr int nt = bulk_threads(sizeof(typename Form<ST>::store), TY);
     if (Form<ST>::shifted && a.sr)
-        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, Form<ST>::shifted, ENS>), dim3(n_blocks), dim3(nt), 0, s, a);
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, Form<ST>::shifted, ENS, EDGE>), dim3(n_blocks), dim3(nt), 0, s, a);
     else
-        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, false, ENS>), dim3(n_blocks), dim3(nt), 0, s, a);
+        hipLaunchKernelGGL((step2_bulk_kernel<ST, TY, false, ENS, EDGE>), dim3(n_blocks), dim3(nt), 0, s, a);
     return hipGetLastError();
 }
 
-template <int ST> hipError_t launch_bulk_st(int ty, bool single, int n_blocks, const StepArgs& a, hipStream_t s)
+template <int ST, int TY, bool EDGE> hipError_t launch_bulk_t(bool single, int n_blocks, const StepArgs& a, hipStream_t s)
+{
+    return single ? launch_bulk_e<ST, TY, false, EDGE>(n_blocks, a, s) : launch_bulk_e<ST, TY, true, EDGE>(n_blocks, a, s);
+}
+
+template <int ST>
+hipError_t launch_bulk_st(int ty, bool single, bool edge, int n_blocks, const StepArgs& a, hipStream_t s)
 {
     switch (ty) {
-    case 8: return single ? launch_bulk_e<ST, 8, false>(n_blocks, a, s) : launch_bulk_e<ST, 8, true>(n_blocks, a, s);
+    case 8: return edge ? launch_bulk_t<ST, 8, true>(single, n_blocks, a, s) : launch_bulk_t<ST, 8, false>(single, n_blocks, a, s);
     case 16:
-        return single ? launch_bulk_e<ST, 16, false>(n_blocks, a, s) : launch_bulk_e<ST, 16, true>(n_blocks, a, s);
+        return edge ? launch_bulk_t<ST, 16, true>(single, n_blocks, a, s) : launch_bulk_t<ST, 16, false>(single, n_blocks, a, s);
     case 32:
-        return single ? launch_bulk_e<ST, 32, false>(n_blocks, a, s) : launch_bulk_e<ST, 32, true>(n_blocks, a, s);
+        return edge ? launch_bulk_t<ST, 32, true>(single, n_blocks, a, s) : launch_bulk_t<ST, 32, false>(single, n_blocks, a, s);
     }
     return hipErrorInvalidValue;
 }
@@ -2646,15 +2755,22 @@ bool bulk_supported(int storage, int tile_rows)
     return storage != ST_F64 && (tile_rows == 8 || tile_rows == 16 || tile_rows == 32);
 }
 
-hipError_t launch_step2_bulk(int storage, int tile_rows, bool single, int n_blocks, const StepArgs& a, hipStream_t s)
+hipError_t launch_step2_bulk(int storage, int tile_rows, bool single, bool edge, int n_blocks, const StepArgs& a,
+                             hipStream_t s)
 {
     if (n_blocks <= 0) return hipSuccess;
     switch (storage) {
-    case ST_F32: return launch_bulk_st<ST_F32>(tile_rows, single, n_blocks, a, s);
-    case ST_F16S: return launch_bulk_st<ST_F16S>(tile_rows, single, n_blocks, a, s);
-    case ST_BF16S: return launch_bulk_st<ST_BF16S>(tile_rows, single, n_blocks, a, s);
+    case ST_F32: return launch_bulk_st<ST_F32>(tile_rows, single, edge, n_blocks, a, s);
+    case ST_F16S: return launch_bulk_st<ST_F16S>(tile_rows, single, edge, n_blocks, a, s);
+    case ST_BF16S: return launch_bulk_st<ST_BF16S>(tile_rows, single, edge, n_blocks, a, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_hold(hipStream_t s)
+{
+    hipLaunchKernelGGL(hold_kernel, dim3(1), dim3(64), 0, s);
+    return hipGetLastError();
 }
 
 hipError_t launch_tile_class(const Tile* tiles, int n_tiles, const LatticeDev* lat, const uint8_t* flags, int ty,

@@ -54,7 +54,8 @@ class LbmInfo(C.Structure):
         ("les_cells", C.c_int64), ("bytes_per_pop", C.c_int32), ("n_lattices", C.c_int32),
         ("device_bytes", C.c_int64), ("block_x", C.c_int32), ("rows", C.c_int32),
         ("nodes_per_thread", C.c_int32), ("launch_options", C.c_int32),
-        ("bulk_tiles", C.c_int32), ("other_tiles", C.c_int32),
+        ("bulk_tiles", C.c_int32), ("other_tiles", C.c_int32), ("edge_tiles", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -352,3 +352,110 @@ def build_gpu_thermal(tc: ThermalCase, storage: str):
     if code is not None:
         ens.set_thermal_rules(0, code, rules)
     return ens, o
+
+
+# ---- classes of two-step tiles (lbm_kernels.hip tile_class_kernel), restated on the mask
+#
+# Node flags as the geometry pass sets them: a solid node, a fluid node with a solid
+# neighbour (boundary links; neighbours wrap across periodic edges only) and every edge
+# node are general-path nodes.  Inlet and outlet column nodes (CELL_COLUMN, rows 1..ny-2 of
+# an open-x lattice): the inlet's from row 1 (Zou/He, equilibrium) or inlet_y0 (parabolic),
+# unless solid or with links; the outlet's unless it or the node before it is.  A node
+# whose general path reduces to wrapped pulls (wrap_only) lies on periodic edges only and
+# is no solid, link, inlet, outlet or lid node.
+
+_CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+_CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+
+
+def node_kinds(obstacle, d):
+    """(special, allowed) masks: general-path nodes, and the nodes an edge tile may hold
+    (plain, column or wrap-only)."""
+    solid = np.asarray(obstacle).astype(bool)
+    ny, nx = solid.shape
+    xp, yp = d.x_edge == G.EDGE_PERIODIC, d.y_edge == G.EDGE_PERIODIC
+    links = np.zeros_like(solid)
+    for cx, cy in zip(_CX[1:], _CY[1:]):
+        s = np.roll(np.roll(solid, -cy, axis=0), -cx, axis=1)
+        if not xp:
+            s[:, -1 if cx == 1 else 0] &= cx == 0
+        if not yp:
+            s[-1 if cy == 1 else 0, :] &= cy == 0
+        links |= s
+    links &= ~solid
+    edge = np.zeros_like(solid)
+    edge[0, :] = edge[-1, :] = edge[:, 0] = edge[:, -1] = True
+    special = solid | links | edge
+    blocked = solid | links
+    column = np.zeros_like(solid)
+    y = np.arange(ny)
+    if d.x_edge == G.EDGE_OPEN and nx >= 4:
+        if d.inlet in (G.INLET_ZOU_HE, G.INLET_EQUILIBRIUM):
+            start = 1
+        elif d.inlet == G.INLET_PARABOLIC:
+            start = max(1, d.inlet_y0)
+        else:
+            start = ny
+        r = (y >= start) & (y <= ny - 2)
+        column[r, 0] = ~blocked[r, 0]
+        if d.outlet:
+            r = (y >= 1) & (y <= ny - 2)
+            column[r, nx - 1] = ~blocked[r, nx - 1] & ~blocked[r, nx - 2]
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    wrap = ~blocked & ~column
+    if not xp:
+        wrap &= (xx != 0) & (xx != nx - 1)
+    if not yp:
+        wrap &= (yy != 0) & (yy != ny - 1)
+    if d.lid:
+        wrap &= yy != ny - 1
+    if d.inlet != G.INLET_NONE:
+        wrap &= xx != 0
+    if d.outlet:
+        wrap &= xx != nx - 1
+    return special, ~special | column | wrap
+
+
+def ring_nodes(d, nx, ny, x0, y0, rows):
+    """Lattice rows and columns of a tile's ring (66 x (rows + 2) from (x0 - 1, y0 - 1)),
+    wrapped around every edge, and the mask of positions on the lattice or across a
+    periodic edge."""
+    ys, xs = y0 - 1 + np.arange(rows + 2), x0 - 1 + np.arange(66)
+    iny = (ys >= 0) & (ys < ny) | (d.y_edge == G.EDGE_PERIODIC)
+    inx = (xs >= 0) & (xs < nx) | (d.x_edge == G.EDGE_PERIODIC)
+    return ys % ny, xs % nx, iny[:, None] & inx[None, :]
+
+
+def tile_classes(obstacle, d, rows, x_start=0):
+    """[(x0, y0, cls)] of a lattice's 64 x rows two-step tiles: 1 bulk, 2 edge, 0 other."""
+    special, allowed = node_kinds(obstacle, d)
+    ny, nx = special.shape
+    out = []
+    for y0 in range(0, ny, rows):
+        for x0 in range(x_start, nx, 64):
+            inside = x0 >= 1 and x0 + 64 <= nx - 1 and y0 >= 1 and y0 + rows <= ny - 1
+            whole = x0 >= 0 and x0 + 64 <= nx and y0 + rows <= ny
+            cls = 0
+            if inside and not special[y0 - 1:y0 + rows + 1, x0 - 1:x0 + 65].any():
+                cls = 1
+            elif whole:
+                ys, xs, inm = ring_nodes(d, nx, ny, x0, y0, rows)
+                if allowed[np.ix_(ys, xs)][inm].all():
+                    cls = 2
+            out.append((x0, y0, cls))
+    return out
+
+
+def split_counts(obstacle, d, rows, x_start=0):
+    """(bulk, edge, other) tile counts of a split two-step launch: the other tiles are cut
+    into 64 x min(rows, 16) tiles."""
+    ny = np.asarray(obstacle).shape[0]
+    bulk = edge = other = 0
+    for x0, y0, cls in tile_classes(obstacle, d, rows, x_start):
+        if cls == 1:
+            bulk += 1
+        elif cls == 2:
+            edge += 1
+        else:
+            other += -(-min(rows, ny - y0) // min(rows, 16))
+    return bulk, edge, other

@@ -1,8 +1,9 @@
-"""Two steps per launch with bulk tiles: on a one-lattice handle the 64 x rows tiles whose
-node ring lies inside the lattice and holds no general-path node go to the bulk kernel,
+"""Two steps per launch with bulk tiles: the 64 x rows tiles whose node ring lies inside
+the lattice and holds no general-path node go to the bulk kernel, the whole tiles whose
+ring holds besides plain nodes only inlet, outlet and periodic-edge nodes to its edge form,
 the others to the general two-step kernel.  The split must not change a bit: populations
 and force totals equal one step per launch, in every storage form, with and without
-stochastic rounding, for every boundary kind next to the bulk tiles."""
+stochastic rounding, for every boundary kind next to the bulk and edge tiles."""
 from __future__ import annotations
 
 import dataclasses
@@ -12,11 +13,13 @@ import pytest
 
 import lbm2d as G
 import oracle as O
-from cases import CASES, _hills, _mask_setup, apply_gpu_geometry, build_gpu, build_oracle, gpu_desc
+from cases import (CASES, _hills, _mask_setup, _step_geometry, apply_gpu_geometry, build_gpu, build_oracle, gpu_desc,
+                   split_counts)
 
 pytestmark = pytest.mark.gpu
 
 HILLS_BIG = _hills(384, 144)
+STEP_BIG = _step_geometry(384, 144)
 
 # the small cases scaled up until most of their tiles are bulk tiles
 BULK_CASES = {
@@ -35,40 +38,9 @@ BULK_CASES = {
                                    gpu_geom=(("circle", 112, 80, 12, G.BB_BOUZIDI), ("moving", 0.01, 112, 80))),
     "hills": dataclasses.replace(CASES["hills"], name="hills_big", nx=384, ny=144, setup=_mask_setup(HILLS_BIG)),
     "cavity": dataclasses.replace(CASES["cavity"], name="cavity_big", nx=256, ny=256),
+    # parabolic inlet from row ny/3 (the rows below it are general-path nodes), periodic y
+    "step": dataclasses.replace(CASES["step"], name="step_big", nx=384, ny=144, setup=_mask_setup(STEP_BIG)),
 }
-
-
-def expected_split(obstacle, x_periodic, y_periodic, rows):
-    """Bulk and other tiles of 64 x rows tiles from the mask: a node needs the general
-    path when it is solid, on an edge, or fluid next to a solid (periodic edges wrap)."""
-    ny, nx = obstacle.shape
-    solid = obstacle.astype(bool)
-    near = np.zeros_like(solid)
-    for cx, cy in ((1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)):
-        s = np.roll(np.roll(solid, -cy, axis=0), -cx, axis=1)
-        if not x_periodic:
-            if cx == 1:
-                s[:, -1] = False
-            if cx == -1:
-                s[:, 0] = False
-        if not y_periodic:
-            if cy == 1:
-                s[-1, :] = False
-            if cy == -1:
-                s[0, :] = False
-        near |= s
-    special = solid | near
-    special[0, :] = special[-1, :] = True
-    special[:, 0] = special[:, -1] = True
-    bulk = other = 0
-    for y0 in range(0, ny, rows):
-        for x0 in range(0, nx, 64):
-            inside = x0 >= 1 and x0 + 64 <= nx - 1 and y0 >= 1 and y0 + rows <= ny - 1
-            if inside and not special[y0 - 1:y0 + rows + 1, x0 - 1:x0 + 65].any():
-                bulk += 1
-            else:
-                other += -(-(min(rows, ny - y0)) // min(rows, 16))
-    return bulk, other
 
 
 FORMS = [("f32", False), ("f16s", False), ("f16s", True), ("bf16s", False), ("bf16s", True)]
@@ -85,20 +57,23 @@ def test_bulk_tiles_bitwise_equal_one_step_per_launch(name, storage, sr):
         b.set_stochastic_rounding(True, 77)
     a.set_launch(64, 1, 1, 0)
     desc = a.info()
-    assert desc.bulk_tiles == 0
-    d = G.desc_for_case(c.case, c.nx, c.ny)
-    seen = 0
+    assert (desc.bulk_tiles, desc.edge_tiles, desc.other_tiles) == (0, 0, 0)
+    d = gpu_desc(c)
+    seen = seen_edge = 0
     for rows, opt in ((16, 5), (32, 7), (8, 6), (32, 4), (16, 7)):
         b.set_launch(64, rows, 1, opt, graph_steps=10)
         info = b.info()
-        exp = expected_split(o.obstacle, d.x_edge == G.EDGE_PERIODIC, d.y_edge == G.EDGE_PERIODIC, rows)
-        assert (info.bulk_tiles, info.other_tiles) == exp, f"rows {rows}"
+        exp = split_counts(o.obstacle, d, rows)
+        assert (info.bulk_tiles, info.edge_tiles, info.other_tiles) == exp, f"rows {rows}"
         seen += info.bulk_tiles
+        seen_edge += info.edge_tiles
         for n in (1, 5, 12):
             a.step(n)
             b.step(n)
             np.testing.assert_array_equal(b.download(0), a.download(0), err_msg=f"rows {rows} options {opt} n {n}")
     assert seen > 0
+    # edge tiles wherever an edge is periodic or carries an inlet or outlet column
+    assert (seen_edge > 0) == (d.x_edge == G.EDGE_PERIODIC or d.y_edge == G.EDGE_PERIODIC or d.outlet != 0), seen_edge
     fa, fya = a.forces(0)
     fb, fyb = b.forces(0)
     np.testing.assert_array_equal(fb, fa)
@@ -130,16 +105,17 @@ def test_bulk_plan_follows_geometry_and_refusals():
     ens, o = build_gpu(c, "f32")
     ens.set_launch(64, 16, 1, 5)
     ens.step(2)
-    d = G.desc_for_case(c.case, c.nx, c.ny)
+    d = gpu_desc(c)
     first = ens.info().bulk_tiles
     m = o.obstacle.copy()
     m[40:120, 300:310] = 1
+    m[0:30, 440:448] = 1  # on the outlet column: its tiles leave the edge class
     ens.set_obstacle(0, m)
     ens.upload(0, o.f.copy())
     ens.step(2)
     info = ens.info()
-    exp = expected_split(m, d.x_edge == G.EDGE_PERIODIC, d.y_edge == G.EDGE_PERIODIC, 16)
-    assert (info.bulk_tiles, info.other_tiles) == exp
+    exp = split_counts(m, d, 16)
+    assert (info.bulk_tiles, info.edge_tiles, info.other_tiles) == exp
     assert info.bulk_tiles < first
     ens.close()
     f64, _ = build_gpu(c, "f64")
@@ -192,10 +168,11 @@ def test_ensemble_bulk_tiles_equal_one_step_per_launch(storage, sr):
     else:
         ens.set_stochastic_rounding(True, 5)
         ref.set_stochastic_rounding(True, 5)
-    seen = 0
+    seen = seen_edge = 0
     for rows, opt in ((16, 5), (8, 6), (32, 7)):
         ens.set_launch(64, rows, 1, opt, graph_steps=10)
         seen += ens.info().bulk_tiles
+        seen_edge += ens.info().edge_tiles
         for n in (1, 5, 12):
             ens.step(n)
             ref.step(n)
@@ -206,7 +183,7 @@ def test_ensemble_bulk_tiles_equal_one_step_per_launch(storage, sr):
                 np.testing.assert_array_equal(got, ref.download(l), err_msg=f"lattice {l} rows {rows} n {n}")
                 if alone:
                     np.testing.assert_array_equal(got, alone[l].download(0), err_msg=f"alone {l} rows {rows} n {n}")
-    assert seen > 0
+    assert seen > 0 and seen_edge > 0
     for l in range(len(cs)):
         fe, fye = ens.forces(l)
         fr, fyr = ref.forces(l)

@@ -245,3 +245,68 @@ def test_bulk_ring_row_and_offsets():
     assert (rows + 3) * pitch * bpp < 2 ** 31 and pitch < 2 ** 24
     # the largest offset a ring load forms, in bytes
     assert ((rows + 1) * pitch + 65 + pitch + 1) * bpp < 2 ** 32
+
+
+# ---- edge tiles of two-step launches (step2_bulk_kernel's edge form) ---------------------
+#
+# An edge tile lies whole in its lattice; its ring positions wrap around every edge, those
+# off a non-periodic edge are skipped in the first step (nothing written to their LDS
+# slots).  A ring node's pulls come from the gathered column (x - 1 on the outlet column)
+# and the rows and columns around it, wrapped, so every pull is a lattice node and every
+# address a node offset within a plane.  In the second step a tile node reads the LDS ring
+# at (column - cx, row - cy), the outlet node at the column before it: those reads must land
+# on ring positions written in the first step, except an inlet node's slots whose sources
+# lie off the lattice (the three slots Zou/He replaces; every slot for the equilibrium
+# inlet).
+
+def _edge_cases():
+    from cases import _hills, _step_geometry
+    yy, xx = np.mgrid[0:160, 0:448]
+    out = [(G.desc_for_case(G.CYLINDER, 448, 160), ((xx - 112) ** 2 + (yy - 81) ** 2 <= 144).astype(np.uint8)),
+           (G.desc_for_case(G.STEP, 384, 144), _step_geometry(384, 144)),
+           (G.desc_for_case(G.PERIODIC_HILLS, 384, 144), _hills(384, 144))]
+    # the equilibrium inlet, periodic y edges
+    d = G.desc_for_case(G.CYLINDER, 320, 96)
+    d.inlet = G.INLET_EQUILIBRIUM
+    out.append((d, np.zeros((96, 320), np.uint8)))
+    return out
+
+
+@pytest.mark.parametrize("rows", [8, 16, 32])
+def test_edge_tile_pulls_and_ring_reads(rows):
+    from cases import node_kinds, ring_nodes, tile_classes
+    n_edge = 0
+    for d, m in _edge_cases():
+        ny, nx = m.shape
+        special, allowed = node_kinds(m, d)
+        for x0, y0, cls in tile_classes(m, d, rows):
+            if cls != 2:
+                continue
+            n_edge += 1
+            assert x0 >= 0 and x0 + 64 <= nx and y0 >= 0 and y0 + rows <= ny
+            ys, xs, inm = ring_nodes(d, nx, ny, x0, y0, rows)
+            gy, gx = np.meshgrid(ys, xs, indexing="ij")
+            # step 1: the pulls of every ring position (skipped ones included: their loads
+            # are issued) are lattice nodes
+            px = np.where(d.outlet & (gx == nx - 1), gx - 1, gx)
+            for j in range(9):
+                sx = np.where(px - CX[j] < 0, nx - 1, np.where(px - CX[j] >= nx, 0, px - CX[j]))
+                sy = (gy - CY[j]) % ny
+                assert sx.min() >= 0 and sx.max() < nx and sy.min() >= 0 and sy.max() < ny
+                # off-lattice sources are read only by inlet nodes (replaced slots)
+                off = (px - CX[j] < 0) | (px - CX[j] >= nx)
+                if d.x_edge != G.EDGE_PERIODIC:
+                    assert not (off & inm & (gx != 0)).any()
+            # step 2: LDS reads of the tile nodes
+            for ty in range(rows):
+                for tx in range(64):
+                    x, y = x0 + tx, y0 + ty
+                    lx, ly = tx + 1, ty + 1
+                    slx = lx - 1 if (d.outlet and x == nx - 1) else lx
+                    inlet = x == 0 and d.inlet != G.INLET_NONE
+                    for j in range(9):
+                        rx, ry = slx - CX[j], ly - CY[j]
+                        assert 0 <= rx <= 65 and 0 <= ry <= rows + 1
+                        if not inm[ry, rx]:
+                            assert inlet and (d.inlet == G.INLET_EQUILIBRIUM or j in (1, 5, 8)), (x, y, j)
+    assert n_edge > 0
