@@ -104,10 +104,10 @@ typedef struct lbm_info {
     int32_t nodes_per_thread;
     int32_t launch_options;
     int32_t bulk_tiles;       /* two steps per launch split by tile class: tiles taken   */
-    int32_t other_tiles;      /* by the bulk kernel and by the general one, and the edge */
-    int32_t edge_tiles;       /* tiles (inlet, outlet, periodic edges) taken by the bulk */
-    int32_t reserved;         /* kernel's edge form (0, 0, 0: not split)                 */
-} lbm_info;
+    int32_t other_tiles;      /* by the bulk kernel and by the general one, the edge     */
+    int32_t edge_tiles;       /* tiles (inlet, outlet, periodic edges) taken by its edge */
+    int32_t empty_tiles;      /* form, and tiles of unread solid nodes left out (0: not  */
+} lbm_info;                   /* split)                                                  */
 
 /* Message of the last failing call on this host thread. */
 const char* lbm_last_error(void);

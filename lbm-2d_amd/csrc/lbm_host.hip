@@ -164,6 +164,7 @@ struct Ensemble {
         Tile* de = nullptr;    // edge tiles
         Tile* dr = nullptr;    // the others, as an ensemble tile table
         int n_bulk = 0, n_edge = 0, n_rest = 0;
+        int n_empty = 0;       // tiles of solid nodes nothing reads: left out
     };
     BulkPlan plan[3];
     // two more streams for the edge and the other tiles of a split launch, so that they run
@@ -699,6 +700,8 @@ int build_plan(Ensemble* E, int rows)
             bulk.push_back(t);
         } else if (cls[k] == 2 && edge_tiles_fit(E, E->lat[t.lattice])) {
             edge.push_back(t);
+        } else if (cls[k] == 3) {
+            ++P.n_empty;
         } else {
             const int ny = E->lat[t.lattice].d.ny;
             for (int y = t.y0; y < std::min(t.y0 + rows, ny); y += rest_rows) rest.push_back(Tile{t.lattice, y, t.x0, 0});
@@ -2468,6 +2471,7 @@ int lbm_get_info(lbm_handle h, lbm_info* out)
         r.bulk_tiles = P->n_bulk;
         r.other_tiles = P->n_rest;
         r.edge_tiles = P->n_edge;
+        r.empty_tiles = P->n_empty;
     }
     *out = r;
     return LBM_OK;

@@ -231,7 +231,8 @@ hipError_t launch_step2_bulk(int storage, int tile_rows, bool single, bool edge,
 hipError_t launch_hold(hipStream_t s);
 // Classes of the n_tiles tiles of 64 x ty (lattice descriptors on the device): out[t] = 1
 // for a bulk tile, 2 for an edge tile (a whole tile whose ring holds no general-path node
-// but inlet, outlet and periodic-edge nodes), 0 for the others.
+// but inlet, outlet and periodic-edge nodes), 3 for a tile of solid nodes nothing reads, 0
+// for the others.
 hipError_t launch_tile_class(const Tile* tiles, int n_tiles, const LatticeDev* lat, const uint8_t* flags, int ty,
                              uint8_t* out, hipStream_t s);
 // Blocks of block_x threads that take the general-path list nodes: n_links link

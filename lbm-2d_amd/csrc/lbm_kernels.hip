@@ -1847,7 +1847,9 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
 // nodes are general-path nodes, so the ring's pulls stay inside the lattice too); 2 (edge)
 // when the tile lies whole in its lattice and every node of its ring, wrapped around
 // periodic edges and skipped off the others, is plain, an inlet or outlet column node or a
-// node whose general path reduces to wrapped pulls; else 0.
+// node whose general path reduces to wrapped pulls; 3 (empty) when every node of the tile is
+// a solid node nothing reads (no value to compute, no force: the tile can be left out);
+// else 0.
 __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, const LatticeDev* lat,
                                                          const uint8_t* flags, int ty, uint8_t* out)
 {
@@ -1856,7 +1858,14 @@ __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, cons
     const int x0 = t.x0, y0 = t.y0;
     const bool inside = x0 >= 1 && x0 + 64 <= L.nx - 1 && y0 >= 1 && y0 + ty <= L.ny - 1;
     const bool whole = x0 >= 0 && x0 + 64 <= L.nx && y0 >= 0 && y0 + ty <= L.ny;
-    int special = 0, general = 0;
+    int special = 0, general = 0, live = 0;
+    for (int r = static_cast<int>(threadIdx.x); r < 64 * ty; r += 256) {
+        const int x = x0 + (r & 63), y = y0 + (r >> 6);
+        if (x >= 0 && x < L.nx && y < L.ny) {
+            const uint8_t fl = flags[L.cell_off + static_cast<int64_t>(y) * L.pitch + x];
+            if (!(fl & CELL_SOLID) || (fl & CELL_WRITE)) live = 1;
+        }
+    }
     if (whole) {
         const uint8_t* f = flags + L.cell_off;
         for (int r = static_cast<int>(threadIdx.x); r < 66 * (ty + 2); r += 256) {
@@ -1884,7 +1893,8 @@ __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, cons
     }
     const int any = __syncthreads_or(special);
     const int gen = __syncthreads_or(general);
-    if (threadIdx.x == 0) out[blockIdx.x] = !whole ? 0 : (inside && !any ? 1 : (gen ? 0 : 2));
+    const int alive = __syncthreads_or(live);
+    if (threadIdx.x == 0) out[blockIdx.x] = !alive ? 3 : !whole ? 0 : (inside && !any ? 1 : (gen ? 0 : 2));
 }
 
 // ------------------------------------------------------------------ state / prologue

@@ -55,7 +55,7 @@ class LbmInfo(C.Structure):
         ("device_bytes", C.c_int64), ("block_x", C.c_int32), ("rows", C.c_int32),
         ("nodes_per_thread", C.c_int32), ("launch_options", C.c_int32),
         ("bulk_tiles", C.c_int32), ("other_tiles", C.c_int32), ("edge_tiles", C.c_int32),
-        ("reserved", C.c_int32),
+        ("empty_tiles", C.c_int32),
     ]
 
 

@@ -426,9 +426,11 @@ def ring_nodes(d, nx, ny, x0, y0, rows):
     return ys % ny, xs % nx, iny[:, None] & inx[None, :]
 
 
-def tile_classes(obstacle, d, rows, x_start=0):
-    """[(x0, y0, cls)] of a lattice's 64 x rows two-step tiles: 1 bulk, 2 edge, 0 other."""
+def tile_classes(obstacle, d, rows, x_start=0, written_solids=False):
+    """[(x0, y0, cls)] of a lattice's 64 x rows two-step tiles: 1 bulk, 2 edge, 3 empty
+    (every node solid and, on a lattice without Bouzidi links, never written), 0 other."""
     special, allowed = node_kinds(obstacle, d)
+    solid = np.asarray(obstacle).astype(bool)
     ny, nx = special.shape
     out = []
     for y0 in range(0, ny, rows):
@@ -436,7 +438,9 @@ def tile_classes(obstacle, d, rows, x_start=0):
             inside = x0 >= 1 and x0 + 64 <= nx - 1 and y0 >= 1 and y0 + rows <= ny - 1
             whole = x0 >= 0 and x0 + 64 <= nx and y0 + rows <= ny
             cls = 0
-            if inside and not special[y0 - 1:y0 + rows + 1, x0 - 1:x0 + 65].any():
+            if not written_solids and solid[y0:y0 + rows, max(x0, 0):x0 + 64].all():
+                cls = 3
+            elif inside and not special[y0 - 1:y0 + rows + 1, x0 - 1:x0 + 65].any():
                 cls = 1
             elif whole:
                 ys, xs, inm = ring_nodes(d, nx, ny, x0, y0, rows)
@@ -446,16 +450,18 @@ def tile_classes(obstacle, d, rows, x_start=0):
     return out
 
 
-def split_counts(obstacle, d, rows, x_start=0):
-    """(bulk, edge, other) tile counts of a split two-step launch: the other tiles are cut
-    into 64 x min(rows, 16) tiles."""
+def split_counts(obstacle, d, rows, x_start=0, written_solids=False):
+    """(bulk, edge, other, empty) tile counts of a split two-step launch: the other tiles
+    are cut into 64 x min(rows, 16) tiles."""
     ny = np.asarray(obstacle).shape[0]
-    bulk = edge = other = 0
-    for x0, y0, cls in tile_classes(obstacle, d, rows, x_start):
+    bulk = edge = other = empty = 0
+    for x0, y0, cls in tile_classes(obstacle, d, rows, x_start, written_solids):
         if cls == 1:
             bulk += 1
         elif cls == 2:
             edge += 1
+        elif cls == 3:
+            empty += 1
         else:
             other += -(-min(rows, ny - y0) // min(rows, 16))
-    return bulk, edge, other
+    return bulk, edge, other, empty

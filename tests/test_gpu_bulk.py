@@ -38,9 +38,18 @@ BULK_CASES = {
                                    gpu_geom=(("circle", 112, 80, 12, G.BB_BOUZIDI), ("moving", 0.01, 112, 80))),
     "hills": dataclasses.replace(CASES["hills"], name="hills_big", nx=384, ny=144, setup=_mask_setup(HILLS_BIG)),
     "cavity": dataclasses.replace(CASES["cavity"], name="cavity_big", nx=256, ny=256),
+    # a disc holding whole 64 x 8 / 64 x 16 / 64 x 32 tiles of solid nodes: empty tiles
+    "disc": dataclasses.replace(CASES["les"], name="disc_big", nx=448, ny=160, pert=(100, 300),
+                                setup=lambda s: s.place_cylinder(170, 80, 52),
+                                gpu_geom=(("circle", 170, 80, 52, G.BB_MEI),)),
     # parabolic inlet from row ny/3 (the rows below it are general-path nodes), periodic y
     "step": dataclasses.replace(CASES["step"], name="step_big", nx=384, ny=144, setup=_mask_setup(STEP_BIG)),
 }
+
+
+def _written_solids(c):
+    """Lattices with Bouzidi links write every solid node (no tile of them is empty)."""
+    return any(g[0] == "circle" and g[4] == G.BB_BOUZIDI for g in c.gpu_geom)
 
 
 FORMS = [("f32", False), ("f16s", False), ("f16s", True), ("bf16s", False), ("bf16s", True)]
@@ -57,14 +66,15 @@ def test_bulk_tiles_bitwise_equal_one_step_per_launch(name, storage, sr):
         b.set_stochastic_rounding(True, 77)
     a.set_launch(64, 1, 1, 0)
     desc = a.info()
-    assert (desc.bulk_tiles, desc.edge_tiles, desc.other_tiles) == (0, 0, 0)
+    assert (desc.bulk_tiles, desc.edge_tiles, desc.other_tiles, desc.empty_tiles) == (0, 0, 0, 0)
     d = gpu_desc(c)
+    written = _written_solids(c)
     seen = seen_edge = 0
     for rows, opt in ((16, 5), (32, 7), (8, 6), (32, 4), (16, 7)):
         b.set_launch(64, rows, 1, opt, graph_steps=10)
         info = b.info()
-        exp = split_counts(o.obstacle, d, rows)
-        assert (info.bulk_tiles, info.edge_tiles, info.other_tiles) == exp, f"rows {rows}"
+        exp = split_counts(o.obstacle, d, rows, written_solids=written)
+        assert (info.bulk_tiles, info.edge_tiles, info.other_tiles, info.empty_tiles) == exp, f"rows {rows}"
         seen += info.bulk_tiles
         seen_edge += info.edge_tiles
         for n in (1, 5, 12):
@@ -115,7 +125,7 @@ def test_bulk_plan_follows_geometry_and_refusals():
     ens.step(2)
     info = ens.info()
     exp = split_counts(m, d, 16)
-    assert (info.bulk_tiles, info.edge_tiles, info.other_tiles) == exp
+    assert (info.bulk_tiles, info.edge_tiles, info.other_tiles, info.empty_tiles) == exp
     assert info.bulk_tiles < first
     ens.close()
     f64, _ = build_gpu(c, "f64")
