@@ -21,9 +21,9 @@ population, the u16 wall distance with LES), divided by that kernel's average la
 duration measured here with HIP events on the library's stream (torch events recorded on
 the stream the library launches on): the difference between replays of hipGraphs of 40
 and of 20 steps, which differ by exactly 10 two-step launches (or 20 one-step launches),
-against 8 TB/s.  A two-step launch of this lattice is the bulk-tile kernel with the kernel
-of the other tiles (edges, obstacle surfaces) running beside it on a second stream; the
-launch time covers both.  ``traffic`` is the HBM bytes per launch read from the committed
+against 8 TB/s.  A two-step launch of this lattice is the bulk-tile kernel with the edge
+tiles (the bulk kernel's edge form) and the other tiles (obstacle surfaces, corners) running
+beside it on side streams; the launch time covers all three.  ``traffic`` is the HBM bytes per launch read from the committed
 rocprofv3 PMC summary of the same workload and launch shape (null when the autotuner
 picks a shape the summary does not hold).
 
@@ -296,7 +296,8 @@ def bench_large(geom, form, steps, warmup, torch, stream, launch="", graph_steps
            "GLUPS": info.cells * steps / t / 1e9, "ms_per_step": 1e3 * t / steps, "seconds": t,
            "bytes_per_step": b, "launch": list(shape), "cells": info.cells, "fluid_cells": info.fluid_cells,
            "les_cells": info.les_cells, "setup_s": t_setup,
-           "bulk_tiles": info.bulk_tiles, "other_tiles": info.other_tiles}
+           "bulk_tiles": info.bulk_tiles, "edge_tiles": info.edge_tiles, "other_tiles": info.other_tiles,
+           "empty_tiles": info.empty_tiles}
     two = bool(shape[3] & 4)
     if roofline:
         lms, per = kernel_launch_ms(e, torch, two)
@@ -567,11 +568,13 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": profiled_traffic(head["launch"]),
-            "kernel": (f"step2_bulk_kernel<1, {head['launch'][1]}, false> with step2_kernel<1, "
-                       f"{min(head['launch'][1], 16)}, false> on the other tiles" if two and head["bulk_tiles"] else
+            "kernel": (f"step2_bulk_kernel<1, {head['launch'][1]}, false, false, false> with its edge form "
+                       f"(..., true) on the edge tiles and step2_kernel<1, {min(head['launch'][1], 16)}, false> "
+                       f"on the other tiles" if two and head["bulk_tiles"] else
                        f"step2_kernel<1, {head['launch'][1]}, true>" if two else
                        "step_kernel<1, {}, {}, {}, true>".format(*head["launch"][:3])),
-            "bulk_tiles": head["bulk_tiles"], "other_tiles": head["other_tiles"],
+            "bulk_tiles": head["bulk_tiles"], "edge_tiles": head["edge_tiles"],
+            "other_tiles": head["other_tiles"], "empty_tiles": head["empty_tiles"],
             "steps_per_launch": head["steps_per_launch"],
             "launch_ms": head["launch_ms"],
             "bytes_per_launch": bytes_launch,
