@@ -1588,11 +1588,12 @@ __host__ __device__ constexpr int bulk_threads(int bytes_per_value, int ty)
 // group_update's, in stored units, so the results are bit for bit those of every other
 // kernel.
 //
-// EDGE: the edge tiles instead (a.bulk_tiles holds them) -- whole 64 x TY tiles of their
-// lattice whose ring holds, besides plain nodes, only inlet and outlet column nodes
-// (CELL_COLUMN) and nodes whose general path reduces to wrapped pulls (wrap_only).  Ring
-// positions wrap around periodic edges and are skipped off the others (only an inlet
-// node's closed slots read there); an inlet node closes its populations in stored units
+// EDGE: the edge tiles instead (a.bulk_tiles holds them) -- 64 x TY tiles from a node of
+// their lattice (cut off at its last column and row) whose ring holds, besides plain nodes,
+// only inlet and outlet column nodes (CELL_COLUMN) and nodes whose general path reduces to
+// wrapped pulls (wrap_only).  Ring positions wrap around periodic edges and are skipped off
+// the others (only an inlet node's closed slots read there) and more than one node off the
+// lattice (no tile node reads there); an inlet node closes its populations in stored units
 // (inlet_closure scaled by 2^k: the bits of the closure in physical units, scaled
 // exactly); an outlet node takes the pulls of the node before it (the outflow copy).  Pull
 // addresses are whole node offsets within a plane, from the three source rows and columns
@@ -1656,29 +1657,33 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
         else
             return __umul24(ly, pitch) + static_cast<uint32_t>(lx);
     };
-    // EDGE: lattice node (gx, gy) of ring position (lx, ly), wrapped around every edge;
-    // false off a non-periodic one
+    // EDGE: lattice node (gx, gy) of ring position (lx, ly), wrapped around every edge and
+    // kept inside the lattice; false off a non-periodic edge or more than one node off
     auto ring_xy = [&](int lx, int ly, int& gx, int& gy) {
         gx = t.x0 - 1 + lx;
         gy = t.y0 - 1 + ly;
-        bool in = true;
+        bool in = gx <= L.nx && gy <= L.ny;
         if (gx < 0 || gx >= L.nx) {
-            in = L.x_edge == EDGE_PERIODIC;
+            in = in && L.x_edge == EDGE_PERIODIC;
             gx = gx < 0 ? gx + L.nx : gx - L.nx;
         }
         if (gy < 0 || gy >= L.ny) {
             in = in && L.y_edge == EDGE_PERIODIC;
             gy = gy < 0 ? gy + L.ny : gy - L.ny;
         }
+        gx = min(gx, L.nx - 1);
+        gy = min(gy, L.ny - 1);
         return in;
     };
+    // EDGE: whether the tile node at ring position (lx, ly) lies in the lattice
+    auto tile_node = [&](int lx, int ly) { return !EDGE || (t.x0 - 1 + lx < L.nx && t.y0 - 1 + ly < L.ny); };
 
     // the second step's wall distances, fetched with the first step's loads
     uint16_t wd2[P2];
 #pragma unroll
     for (int k = 0; k < P2; ++k) {
         wd2[k] = 0xffffu;
-        if (les) wd2[k] = load_at(wdb, tile_off(tx + 1, ty0 + RS * k + 1) * 2u);
+        if (les && tile_node(tx + 1, ty0 + RS * k + 1)) wd2[k] = load_at(wdb, tile_off(tx + 1, ty0 + RS * k + 1) * 2u);
     }
 
     // decode, collide, encode one node in place (stored units, as group_update; the shift
@@ -1816,6 +1821,7 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
 #pragma unroll 1
     for (int k = 0; k < P2; ++k) {
         const int lx = tx + 1, ly = ty0 + RS * k + 1;
+        if (!tile_node(lx, ly)) continue;
         const uint32_t o = tile_off(lx, ly);
         uint16_t wraw = wd2[0];
 #pragma unroll
@@ -1845,9 +1851,10 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
 // Classes of two-step tiles (64 x ty from (x0, y0) of its lattice): out[t] = 1 (bulk) when
 // the tile's ring of nodes lies inside the lattice and holds no general-path node (edge
 // nodes are general-path nodes, so the ring's pulls stay inside the lattice too); 2 (edge)
-// when the tile lies whole in its lattice and every node of its ring, wrapped around
-// periodic edges and skipped off the others, is plain, an inlet or outlet column node or a
-// node whose general path reduces to wrapped pulls; 3 (empty) when every node of the tile is
+// when the tile starts on a lattice node and every node of its ring, wrapped around
+// periodic edges and skipped off the others and more than one node off the lattice, is
+// plain, an inlet or outlet column node or a node whose general path reduces to wrapped
+// pulls (and an outlet node is not the tile's first column); 3 (empty) when every node of the tile is
 // a solid node nothing reads (no value to compute, no force: the tile can be left out);
 // else 0.
 __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, const LatticeDev* lat,
@@ -1857,7 +1864,9 @@ __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, cons
     const LatticeDev& L = lat[t.lattice];
     const int x0 = t.x0, y0 = t.y0;
     const bool inside = x0 >= 1 && x0 + 64 <= L.nx - 1 && y0 >= 1 && y0 + ty <= L.ny - 1;
-    const bool whole = x0 >= 0 && x0 + 64 <= L.nx && y0 >= 0 && y0 + ty <= L.ny;
+    // edge tiles start on a lattice node (x0, y0 >= 0) and may be cut off at the last column
+    // and row; an outlet node takes the ring column before it, which must lie in the ring
+    const bool whole = x0 >= 0 && y0 >= 0 && !(L.outlet && x0 == L.nx - 1);
     int special = 0, general = 0, live = 0;
     for (int r = static_cast<int>(threadIdx.x); r < 64 * ty; r += 256) {
         const int x = x0 + (r & 63), y = y0 + (r >> 6);
@@ -1871,9 +1880,9 @@ __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, cons
         for (int r = static_cast<int>(threadIdx.x); r < 66 * (ty + 2); r += 256) {
             const int ly = r / 66, lx = r - 66 * ly;
             int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-            bool in = true;
+            bool in = gx <= L.nx && gy <= L.ny;
             if (gx < 0 || gx >= L.nx) {
-                in = L.x_edge == EDGE_PERIODIC;
+                in = in && L.x_edge == EDGE_PERIODIC;
                 gx = gx < 0 ? gx + L.nx : gx - L.nx;
             }
             if (gy < 0 || gy >= L.ny) {
@@ -1884,6 +1893,8 @@ __global__ __launch_bounds__(256) void tile_class_kernel(const Tile* tiles, cons
                 special = 1;
                 continue;
             }
+            gx = min(gx, L.nx - 1);
+            gy = min(gy, L.ny - 1);
             const uint8_t fl = f[static_cast<int64_t>(gy) * L.pitch + gx];
             if (fl & CELL_SPECIAL) {
                 special = 1;

@@ -419,10 +419,10 @@ def node_kinds(obstacle, d):
 def ring_nodes(d, nx, ny, x0, y0, rows):
     """Lattice rows and columns of a tile's ring (66 x (rows + 2) from (x0 - 1, y0 - 1)),
     wrapped around every edge, and the mask of positions on the lattice or across a
-    periodic edge."""
+    periodic edge, at most one node off it."""
     ys, xs = y0 - 1 + np.arange(rows + 2), x0 - 1 + np.arange(66)
-    iny = (ys >= 0) & (ys < ny) | (d.y_edge == G.EDGE_PERIODIC)
-    inx = (xs >= 0) & (xs < nx) | (d.x_edge == G.EDGE_PERIODIC)
+    iny = ((ys >= 0) & (ys < ny) | (d.y_edge == G.EDGE_PERIODIC)) & (ys >= -1) & (ys <= ny)
+    inx = ((xs >= 0) & (xs < nx) | (d.x_edge == G.EDGE_PERIODIC)) & (xs >= -1) & (xs <= nx)
     return ys % ny, xs % nx, iny[:, None] & inx[None, :]
 
 
@@ -436,7 +436,8 @@ def tile_classes(obstacle, d, rows, x_start=0, written_solids=False):
     for y0 in range(0, ny, rows):
         for x0 in range(x_start, nx, 64):
             inside = x0 >= 1 and x0 + 64 <= nx - 1 and y0 >= 1 and y0 + rows <= ny - 1
-            whole = x0 >= 0 and x0 + 64 <= nx and y0 + rows <= ny
+            # edge tiles start on a lattice node; an outlet node needs the column before it
+            whole = x0 >= 0 and not (d.outlet and x0 == nx - 1)
             cls = 0
             if not written_solids and solid[y0:y0 + rows, max(x0, 0):x0 + 64].all():
                 cls = 3

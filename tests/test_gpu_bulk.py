@@ -38,6 +38,10 @@ BULK_CASES = {
                                    gpu_geom=(("circle", 112, 80, 12, G.BB_BOUZIDI), ("moving", 0.01, 112, 80))),
     "hills": dataclasses.replace(CASES["hills"], name="hills_big", nx=384, ny=144, setup=_mask_setup(HILLS_BIG)),
     "cavity": dataclasses.replace(CASES["cavity"], name="cavity_big", nx=256, ny=256),
+    # last tile column and row band cut off by the lattice (partial edge tiles)
+    "partial": dataclasses.replace(CASES["les"], name="les_300x150", nx=300, ny=150, pert=(100, 250),
+                                   setup=lambda s: s.place_cylinder(100, 75, 12),
+                                   gpu_geom=(("circle", 100, 75, 12, G.BB_MEI),)),
     # a disc holding whole 64 x 8 / 64 x 16 / 64 x 32 tiles of solid nodes: empty tiles
     "disc": dataclasses.replace(CASES["les"], name="disc_big", nx=448, ny=160, pert=(100, 300),
                                 setup=lambda s: s.place_cylinder(170, 80, 52),

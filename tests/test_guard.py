@@ -249,8 +249,9 @@ def test_bulk_ring_row_and_offsets():
 
 # ---- edge tiles of two-step launches (step2_bulk_kernel's edge form) ---------------------
 #
-# An edge tile lies whole in its lattice; its ring positions wrap around every edge, those
-# off a non-periodic edge are skipped in the first step (nothing written to their LDS
+# An edge tile starts on a node of its lattice and may be cut off at its last column and
+# row; its ring positions wrap around every edge, those off a non-periodic edge or more than
+# one node off the lattice are skipped in the first step (nothing written to their LDS
 # slots).  A ring node's pulls come from the gathered column (x - 1 on the outlet column)
 # and the rows and columns around it, wrapped, so every pull is a lattice node and every
 # address a node offset within a plane.  In the second step a tile node reads the LDS ring
@@ -269,6 +270,12 @@ def _edge_cases():
     d = G.desc_for_case(G.CYLINDER, 320, 96)
     d.inlet = G.INLET_EQUILIBRIUM
     out.append((d, np.zeros((96, 320), np.uint8)))
+    # widths and heights that cut the last tiles off, narrower and lower than a tile and
+    # its ring, with an outlet column alone in the last tile (nx % 64 == 1)
+    for nx, ny in ((300, 100), (65, 30), (129, 41), (40, 7)):
+        yy, xx = np.mgrid[0:ny, 0:nx]
+        out.append((G.desc_for_case(G.CYLINDER, nx, ny), ((xx - nx // 3) ** 2 + (yy - ny // 2) ** 2 <= 4).astype(np.uint8)))
+        out.append((G.desc_for_case(G.PERIODIC_HILLS, nx, ny), np.zeros((ny, nx), np.uint8)))
     return out
 
 
@@ -283,9 +290,15 @@ def test_edge_tile_pulls_and_ring_reads(rows):
             if cls != 2:
                 continue
             n_edge += 1
-            assert x0 >= 0 and x0 + 64 <= nx and y0 >= 0 and y0 + rows <= ny
+            assert x0 >= 0 and y0 >= 0
             ys, xs, inm = ring_nodes(d, nx, ny, x0, y0, rows)
-            gy, gx = np.meshgrid(ys, xs, indexing="ij")
+            # the kernel's wrap (once) and clamp of the ring positions: inside the lattice
+            uy, ux = np.meshgrid(y0 - 1 + np.arange(rows + 2), x0 - 1 + np.arange(66), indexing="ij")
+            gx = np.minimum(np.where(ux < 0, ux + nx, np.where(ux >= nx, ux - nx, ux)), nx - 1)
+            gy = np.minimum(np.where(uy < 0, uy + ny, np.where(uy >= ny, uy - ny, uy)), ny - 1)
+            assert gx.min() >= 0 and gy.min() >= 0
+            assert np.array_equal(gx[inm], np.broadcast_to(xs, gx.shape)[inm])
+            assert np.array_equal(gy[inm], np.broadcast_to(ys[:, None], gy.shape)[inm])
             # step 1: the pulls of every ring position (skipped ones included: their loads
             # are issued) are lattice nodes
             px = np.where(d.outlet & (gx == nx - 1), gx - 1, gx)
@@ -301,6 +314,8 @@ def test_edge_tile_pulls_and_ring_reads(rows):
             for ty in range(rows):
                 for tx in range(64):
                     x, y = x0 + tx, y0 + ty
+                    if x >= nx or y >= ny:
+                        continue  # not a node: the kernel skips it
                     lx, ly = tx + 1, ty + 1
                     slx = lx - 1 if (d.outlet and x == nx - 1) else lx
                     inlet = x == 0 and d.inlet != G.INLET_NONE
