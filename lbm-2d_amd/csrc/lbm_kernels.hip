@@ -1574,6 +1574,67 @@ __host__ __device__ constexpr int bulk_threads(int bytes_per_value, int ty)
     return 9 * 66 * (ty + 2) * bytes_per_value > 24 * 1024 ? 512 : 256;
 }
 
+// Decode, collide, encode one node of a bulk or edge tile in place (stored units, as
+// group_update; the shift additions are skipped without a shift, where they are exact
+// identities).  inlet (EDGE): the node is an inlet node of row y.
+template <int ST, bool SR, bool EDGE>
+__device__ __forceinline__ void bulk_update(typename Form<ST>::store v[9], const LatticeDev& L,
+                                            typename Form<ST>::real sc, bool les, uint16_t wraw, uint32_t key,
+                                            uint32_t cell, bool inlet, int y, bool& bad)
+{
+    using S = typename Form<ST>::store;
+    using R = typename Form<ST>::real;
+    R d[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec_raw(v[j]);
+    if constexpr (Form<ST>::shifted) {
+        if (L.shifted) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) d[j] += L.shift_s[j];
+        }
+    }
+    if constexpr (EDGE) {
+        if (inlet) inlet_closure<R>(L, y, d, sc);
+    }
+    R wd = R(0);
+    if (les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
+    collide<R>(d, L, wd, sc);
+    if (!(d[0] == d[0])) bad = true;
+    if constexpr (Form<ST>::shifted) {
+        if (L.shifted) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) d[j] += L.nshift_s[j];
+        }
+    }
+    if constexpr (SR) {
+        // enc_node_raw's words and conversions, two populations per register (each
+        // conversion writes one half and keeps the other)
+        uint32_t h = mix32(key ^ (cell * 0x9e3779b1u));
+#pragma unroll
+        for (int j = 0; j < 9; j += 2) {
+            // the pair's register starts as the first random word (dead after its use)
+            const uint32_t r0 = next_rand(h, j);
+            uint32_t pk = cvt_sr_half<ST, false>(r0, d[j], r0);
+            if (j < 8) pk = cvt_sr_half<ST, true>(pk, d[j + 1], next_rand(h, j + 1));
+            v[j] = static_cast<S>(pk);
+            if (j < 8) v[j + 1] = static_cast<S>(pk >> 16);
+        }
+    } else if constexpr (Form<ST>::shifted) {
+        // round to nearest even, two conversions per instruction (v_cvt_pk_f16_f32,
+        // v_cvt_pk_bf16_f32): the bits of enc_raw's one at a time
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+            const uint32_t pk = cvt_pk_rne<ST>(d[j], d[j + 1]);
+            v[j] = static_cast<S>(pk);
+            v[j + 1] = static_cast<S>(pk >> 16);
+        }
+        v[8] = Form<ST>::enc_raw(d[8]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) v[j] = Form<ST>::enc_raw(d[j]);
+    }
+}
+
 // Steps k and k+1 of the bulk tiles of a lattice: 64 x TY tiles whose ring of nodes
 // (66 x (TY + 2)) lies inside the lattice and holds no general-path node -- no edge,
 // boundary link, solid, inlet, outlet or lid node -- so every node of the launch is a plain
@@ -1686,59 +1747,8 @@ __global__ __launch_bounds__(bulk_threads(sizeof(typename Form<ST>::store), TY))
         if (les && tile_node(tx + 1, ty0 + RS * k + 1)) wd2[k] = load_at(wdb, tile_off(tx + 1, ty0 + RS * k + 1) * 2u);
     }
 
-    // decode, collide, encode one node in place (stored units, as group_update; the shift
-    // additions are skipped without a shift, where they are exact identities).  inlet
-    // (EDGE): the node is an inlet node of row y
     auto update = [&](S v[9], uint16_t wraw, uint32_t key, uint32_t cell, bool inlet, int y, bool& bad) {
-        R d[9];
-#pragma unroll
-        for (int j = 0; j < 9; ++j) d[j] = Form<ST>::dec_raw(v[j]);
-        if constexpr (Form<ST>::shifted) {
-            if (L.shifted) {
-#pragma unroll
-                for (int j = 0; j < 9; ++j) d[j] += L.shift_s[j];
-            }
-        }
-        if constexpr (EDGE) {
-            if (inlet) inlet_closure<R>(L, y, d, sc);
-        }
-        R wd = R(0);
-        if (les) wd = wraw == 0xffffu ? R(1e9) : static_cast<R>(wraw);
-        collide<R>(d, L, wd, sc);
-        if (!(d[0] == d[0])) bad = true;
-        if constexpr (Form<ST>::shifted) {
-            if (L.shifted) {
-#pragma unroll
-                for (int j = 0; j < 9; ++j) d[j] += L.nshift_s[j];
-            }
-        }
-        if constexpr (SR) {
-            // enc_node_raw's words and conversions, two populations per register (each
-            // conversion writes one half and keeps the other)
-            uint32_t h = mix32(key ^ (cell * 0x9e3779b1u));
-#pragma unroll
-            for (int j = 0; j < 9; j += 2) {
-                // the pair's register starts as the first random word (dead after its use)
-                const uint32_t r0 = next_rand(h, j);
-                uint32_t pk = cvt_sr_half<ST, false>(r0, d[j], r0);
-                if (j < 8) pk = cvt_sr_half<ST, true>(pk, d[j + 1], next_rand(h, j + 1));
-                v[j] = static_cast<S>(pk);
-                if (j < 8) v[j + 1] = static_cast<S>(pk >> 16);
-            }
-        } else if constexpr (Form<ST>::shifted) {
-            // round to nearest even, two conversions per instruction (v_cvt_pk_f16_f32,
-            // v_cvt_pk_bf16_f32): the bits of enc_raw's one at a time
-#pragma unroll
-            for (int j = 0; j < 8; j += 2) {
-                const uint32_t pk = cvt_pk_rne<ST>(d[j], d[j + 1]);
-                v[j] = static_cast<S>(pk);
-                v[j + 1] = static_cast<S>(pk >> 16);
-            }
-            v[8] = Form<ST>::enc_raw(d[8]);
-        } else {
-#pragma unroll
-            for (int j = 0; j < 9; ++j) v[j] = Form<ST>::enc_raw(d[j]);
-        }
+        bulk_update<ST, SR, EDGE>(v, L, sc, les, wraw, key, cell, inlet, y, bad);
     };
 
     // pulls and wall distance of ring position r (r < NR), one register each (kept apart
