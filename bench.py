@@ -459,8 +459,8 @@ def main():
     ap.add_argument("--launch", default="", help="fix the step-kernel launch: bx,rows,nodes,options "
                                                    "(default: autotune)")
     ap.add_argument("--graph-steps", type=int, default=100,
-                    help="steps per hipGraph (even; 100 is the library's default: a batch of two-step launches "
-                         "ends with two one-step launches, so short graphs pay that tail more often)")
+                    help="steps per hipGraph (even; 100 is the library's default: an odd batch of two-step "
+                         "launches ends with a one-step launch)")
     ap.add_argument("--no-forms", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)

@@ -240,8 +240,10 @@ int lbm_step_noalloc(lbm_handle h, int32_t n);
 /* Two steps per launch: tiles of 64 x rows nodes (shape (64, rows, 1), rows 4, 8 or 16)
  * with a one-node ring, the intermediate state in LDS.  Every lattice of the handle
  * (an ensemble too; not thermal handles, not free-slip y edges, and on a one-lattice
- * handle not an outlet with nx % 64 == 1); a batch ends with one-step launches.  The
- * results are those of one step per launch, bit for bit. */
+ * handle not an outlet with nx % 64 == 1); a batch of an odd number of steps ends with a
+ * one-step launch, and after an even one the calls that read the state recompute the
+ * populations of the step before the last first (one one-step launch, a temporary copy of
+ * the population buffer).  The results are those of one step per launch, bit for bit. */
 #define LBM_LAUNCH_TWO_STEPS 4
 
 /* Launch shape of the step kernel (threads per block, rows per block, consecutive

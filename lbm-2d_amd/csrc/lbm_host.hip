@@ -171,6 +171,9 @@ struct Ensemble {
     // beside the bulk tiles (fork and join by events; graph capture records the branches)
     hipStream_t side = nullptr, side2 = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr, join2_ev = nullptr;
+    // the last batch ended with a two-step launch: the buffer that is not current holds
+    // C_{k-2} instead of C_{k-1} (complete_prev)
+    bool prev_pending = false;
     int64_t* d_steps = nullptr;
     int32_t* d_par = nullptr;
     double* d_acc = nullptr;       // [acc_blocks][kAccCopies][total_slots][2]
@@ -1065,13 +1068,21 @@ bool one_lattice(Ensemble* E, int l) { return l >= 0 || E->lat.size() == 1; }
 
 // Two-step launches in a batch of n steps (LAUNCH_TWO_STEPS: a one-lattice handle, or
 // every lattice of an ensemble; one lattice of an ensemble alone steps one step per
-// launch); the batch always ends with a one-step launch, so the last stored C precedes
-// the state.
+// launch); an odd batch ends with a one-step launch.  A batch that ends with a two-step
+// launch leaves C_{k-1} unstored (its intermediate state stays in LDS): prev_pending, and
+// complete_prev recomputes it before anything reads it.
 int two_step_pairs(Ensemble* E, int n, int l)
 {
     if (E->thermal || !(E->options & LAUNCH_TWO_STEPS) || n < 2) return 0;
     if (l >= 0 && E->lat.size() > 1) return 0;
-    return (n - 1) / 2;
+    return n / 2;
+}
+
+// Whether a batch of n steps ends with a two-step launch.
+bool ends_on_pair(Ensemble* E, int n, int l)
+{
+    const int p = two_step_pairs(E, n, l);
+    return p > 0 && 2 * p == n;
 }
 
 // Whether two steps per launch apply: tile 64 x (4, 8 or 16) with 256 threads (block_x 64,
@@ -1217,6 +1228,69 @@ int launch_batch(Ensemble* E, int n, int l, double* acc_base = nullptr, double* 
     LBM_HIP(launch_flush(f, E->stream));
     const int begin = l >= 0 ? l : 0, count = l >= 0 ? 1 : static_cast<int>(E->lat.size());
     LBM_HIP(launch_add_steps(E->d_steps, E->d_par, begin, count, n, swaps, E->stream));
+    return LBM_OK;
+}
+
+// Completes the state after a batch that ended with a two-step launch: C_{k-1}, which the
+// two-step kernel kept in LDS, is recomputed from C_{k-2} (still in the buffer that is not
+// current) by one one-step launch, from a copy of that buffer into it, with the random words
+// of its step and dummy force and NaN accumulators -- the bits of the two-step launch's first
+// step (every launch form rounds alike).  Downloads, macros, masses, uploads, parameter,
+// geometry and rounding changes, single-lattice stepping and the autotuner call it first.
+int complete_prev(Ensemble* E)
+{
+    if (!E->prev_pending) return LBM_OK;
+    void* T = nullptr;
+    const int64_t held = E->device_bytes;
+    int rc = alloc_population(E, &T, "a copy of the previous population buffer");
+    E->device_bytes = held;  // freed before returning
+    if (rc != LBM_OK) return rc;
+    int s1[4];
+    one_step_shape(E, s1);
+    StepArgs a = base_args(E);
+    int blocks;
+    const bool single = grid_for(E, -1, a, blocks, s1[0], s1[1], s1[2]);
+    a.options = s1[3];
+    a.acc = E->d_dummy_acc;
+    a.nf = E->d_dummy_nf;
+    a.nan_flags = E->d_dummy_nan;
+    a.step_arg = -2;  // the device step counts are k + 1: step k - 1's keys
+    hipError_t e = hipSuccess;
+    if (single) {
+        const int par = E->lat[0].par;
+        e = hipMemcpyAsync(T, E->buf[1 - par], static_cast<size_t>(E->pool_elems) * E->bpp, hipMemcpyDeviceToDevice,
+                           E->stream);
+        a.src = T;
+        a.dst = E->buf[1 - par];
+        if (e == hipSuccess) e = launch_step(E->storage, s1[0], s1[1], s1[2], true, blocks, a, E->stream);
+    } else {
+        // per buffer parity p: the lattices of parity p read their C_{k-2} from T and write
+        // C_{k-1} into buf[1 - p]; the others read buf[1 - p] and write into T (unused)
+        for (int p = 0; p < 2 && e == hipSuccess; ++p) {
+            bool any = false;
+            for (size_t l = 0; l < E->lat.size() && e == hipSuccess; ++l) {
+                const HostLattice& H = E->lat[l];
+                if (H.par != p) continue;
+                any = true;
+                const size_t off = static_cast<size_t>(H.f_off) * E->bpp;
+                const size_t bytes = static_cast<size_t>(9) * E->host_lat[l].plane * E->bpp;
+                e = hipMemcpyAsync(static_cast<char*>(T) + off, static_cast<const char*>(E->buf[1 - p]) + off, bytes,
+                                   hipMemcpyDeviceToDevice, E->stream);
+            }
+            if (!any || e != hipSuccess) continue;
+            a.buf0 = p == 0 ? T : E->buf[0];
+            a.buf1 = p == 1 ? T : E->buf[1];
+            a.dst = nullptr;
+            a.flips = 0;
+            e = launch_step(E->storage, s1[0], s1[1], s1[2], false, blocks, a, E->stream);
+        }
+    }
+    const hipError_t se = hipStreamSynchronize(E->stream);
+    (void)hipFree(T);
+    if (e != hipSuccess || se != hipSuccess)
+        return fail(LBM_ERR_DEVICE, std::string("completing the state after two-step launches: ") +
+                                        hipGetErrorString(e != hipSuccess ? e : se));
+    E->prev_pending = false;
     return LBM_OK;
 }
 
@@ -1654,6 +1728,10 @@ int lbm_destroy(lbm_handle h)
 
 int lbm_set_obstacle(lbm_handle h, int32_t l, const uint8_t* mask)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!mask) return fail(LBM_ERR_INVALID, "null mask");
@@ -1674,6 +1752,10 @@ int lbm_set_obstacle(lbm_handle h, int32_t l, const uint8_t* mask)
 int lbm_place_obstacles(lbm_handle h, int32_t l, const double* circles, int32_t n_circles, const double* poly_xy,
                         const int32_t* poly_n, int32_t n_polys)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = lattice_ok(h, l);
     if (rc) return rc;
@@ -1773,6 +1855,10 @@ int lbm_get_obstacle(lbm_handle h, int32_t l, uint8_t* mask)
 
 int lbm_set_storage_shift(lbm_handle h, int32_t l, double u, double v)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (h->storage != lbm::ST_F16S && h->storage != lbm::ST_BF16S)
@@ -1788,6 +1874,10 @@ int lbm_set_storage_shift(lbm_handle h, int32_t l, double u, double v)
 
 int lbm_set_stochastic_rounding(lbm_handle h, int32_t on, uint32_t seed)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     if (!h) return fail(LBM_ERR_INVALID, "null handle");
     if (h->storage != lbm::ST_F16S && h->storage != lbm::ST_BF16S)
         return fail(LBM_ERR_INVALID, "stochastic rounding applies to the 16-bit storage forms");
@@ -1799,6 +1889,10 @@ int lbm_set_stochastic_rounding(lbm_handle h, int32_t on, uint32_t seed)
 
 int lbm_set_bb_circle(lbm_handle h, int32_t l, double cx, double cy, double r, int32_t scheme)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (scheme < 0 || scheme > 2) return fail(LBM_ERR_INVALID, "bad scheme");
@@ -1815,6 +1909,10 @@ int lbm_set_bb_circle(lbm_handle h, int32_t l, double cx, double cy, double r, i
 
 int lbm_set_bb_polygon(lbm_handle h, int32_t l, const double* xy, int32_t n, int32_t scheme)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!xy || n < 0 || scheme < 0 || scheme > 2) return fail(LBM_ERR_INVALID, "bad polygon arguments");
@@ -1829,6 +1927,10 @@ int lbm_set_bb_polygon(lbm_handle h, int32_t l, const double* xy, int32_t n, int
 
 int lbm_set_moving_wall(lbm_handle h, int32_t l, double omega, double rcx, double rcy)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     auto& H = h->lat[l];
@@ -1843,6 +1945,10 @@ int lbm_set_moving_wall(lbm_handle h, int32_t l, double omega, double rcx, doubl
 
 int lbm_set_force_radius(lbm_handle h, int32_t l, double cx, double cy, double r)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     auto& H = h->lat[l];
@@ -1857,6 +1963,10 @@ int lbm_set_force_radius(lbm_handle h, int32_t l, double cx, double cy, double r
 
 int lbm_set_links(lbm_handle h, int32_t l, const lbm_link* links, int32_t n)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (n < 0 || (n > 0 && !links)) return fail(LBM_ERR_INVALID, "bad links");
@@ -1869,6 +1979,10 @@ int lbm_set_links(lbm_handle h, int32_t l, const lbm_link* links, int32_t n)
 
 int lbm_set_force_slots(lbm_handle h, int32_t l, int32_t n_slots)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (n_slots < 1 || n_slots > 32767) return fail(LBM_ERR_INVALID, "n_slots out of range");
@@ -1882,6 +1996,10 @@ int lbm_set_force_slots(lbm_handle h, int32_t l, int32_t n_slots)
 
 int lbm_set_params(lbm_handle h, int32_t l, double tau, double u_in)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!(tau > 0.0)) return fail(LBM_ERR_INVALID, "tau must be positive");
@@ -1960,6 +2078,10 @@ int load_state(Ensemble* E, int l, const double* f, double rho, double u, double
 
 int lbm_upload(lbm_handle h, int32_t l, const double* f)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!f) return fail(LBM_ERR_INVALID, "null population array");
@@ -1968,6 +2090,10 @@ int lbm_upload(lbm_handle h, int32_t l, const double* f)
 
 int lbm_init_equilibrium(lbm_handle h, int32_t l, double rho, double u, double v)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!(rho > 0.0)) return fail(LBM_ERR_INVALID, "density must be positive");
@@ -1976,6 +2102,10 @@ int lbm_init_equilibrium(lbm_handle h, int32_t l, double rho, double u, double v
 
 int lbm_perturb_velocity(lbm_handle h, int32_t l, int32_t x0, int32_t x1, const double* dv)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = lattice_ok(h, l);
     if (rc) return rc;
@@ -2013,6 +2143,10 @@ int lbm_perturb_velocity(lbm_handle h, int32_t l, int32_t x0, int32_t x1, const 
 
 int lbm_macros(lbm_handle h, int32_t l, int32_t y0, int32_t y1, double* rho, double* u, double* v)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!h->lat[l].uploaded) return fail(LBM_ERR_INVALID, "no state uploaded");
@@ -2022,6 +2156,10 @@ int lbm_macros(lbm_handle h, int32_t l, int32_t y0, int32_t y1, double* rho, dou
 
 int lbm_download(lbm_handle h, int32_t l, double* f)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!f) return fail(LBM_ERR_INVALID, "null output array");
@@ -2031,6 +2169,10 @@ int lbm_download(lbm_handle h, int32_t l, double* f)
 
 int lbm_mass(lbm_handle h, int32_t l, double* mass)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     int rc = lbm::lattice_ok(h, l);
     if (rc) return rc;
     if (!mass) return fail(LBM_ERR_INVALID, "null output");
@@ -2078,6 +2220,7 @@ int lbm_step(lbm_handle h, int32_t n)
             H.steps += batch;
             H.par ^= flips & 1;
         }
+        E->prev_pending = ends_on_pair(E, batch, -1);
         n -= batch;
     }
     return LBM_OK;
@@ -2128,6 +2271,7 @@ int lbm_step_noalloc(lbm_handle h, int32_t n)
             H.steps += batch;
             H.par ^= flips & 1;
         }
+        E->prev_pending = ends_on_pair(E, batch, -1);
         n -= batch;
     }
     return LBM_OK;
@@ -2135,6 +2279,10 @@ int lbm_step_noalloc(lbm_handle h, int32_t n)
 
 int lbm_debug_poison(lbm_handle h, int32_t l)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = lattice_ok(h, l);
     if (rc) return rc;
@@ -2152,6 +2300,10 @@ int lbm_debug_poison(lbm_handle h, int32_t l)
 
 int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = lattice_ok(h, l);
     if (rc) return rc;
@@ -2166,6 +2318,7 @@ int lbm_step_lattice(lbm_handle h, int32_t l, int32_t n)
         if ((rc = launch_batch(E, batch, l)) != LBM_OK) return rc;
         H.steps += batch;
         H.par ^= batch_flips(E, batch, l) & 1;
+        E->prev_pending = ends_on_pair(E, batch, l);
         n -= batch;
     }
     return LBM_OK;
@@ -2209,6 +2362,10 @@ int lbm_set_launch(lbm_handle h, int32_t bx, int32_t rows, int32_t nodes_per_thr
 int lbm_autotune(lbm_handle h, int32_t* best_bx, int32_t* best_rows, int32_t* best_nodes, int32_t* best_options,
                  float* best_ms)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     if (!h) return fail(LBM_ERR_INVALID, "null handle");
     Ensemble* E = h;
@@ -2487,6 +2644,10 @@ int lbm_sync(lbm_handle h)
 
 int lbm_set_thermal(lbm_handle h, int32_t l, double omega_k, double beta, double g_buoyancy, double t_ref)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = lattice_ok(h, l);
     if (rc) return rc;
@@ -2523,6 +2684,10 @@ int lbm_set_thermal(lbm_handle h, int32_t l, double omega_k, double beta, double
 int lbm_set_thermal_rules(lbm_handle h, int32_t l, const uint8_t* code, const lbm_thermal_rule* rules,
                           int32_t n_rules)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = thermal_ok(h, l);
     if (rc) return rc;
@@ -2560,6 +2725,10 @@ int lbm_set_thermal_rules(lbm_handle h, int32_t l, const uint8_t* code, const lb
 
 int lbm_upload_thermal(lbm_handle h, int32_t l, const double* g)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = thermal_ok(h, l);
     if (rc) return rc;
@@ -2574,6 +2743,10 @@ int lbm_upload_thermal(lbm_handle h, int32_t l, const double* g)
 
 int lbm_download_thermal(lbm_handle h, int32_t l, double* g)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = thermal_ok(h, l);
     if (rc) return rc;
@@ -2584,6 +2757,10 @@ int lbm_download_thermal(lbm_handle h, int32_t l, double* g)
 
 int lbm_temperature(lbm_handle h, int32_t l, int32_t y0, int32_t y1, double* t)
 {
+    if (h && h->prev_pending) {
+        const int cp = lbm::complete_prev(h);
+        if (cp != LBM_OK) return cp;
+    }
     using namespace lbm;
     int rc = thermal_ok(h, l);
     if (rc) return rc;

@@ -1167,8 +1167,9 @@ __device__ __forceinline__ int node_values(const LatticeDev& L, const StepArgs& 
 // buffer (ring nodes wrap on periodic edges), keeps it in LDS in the stored form, then
 // computes C_{k+1} on the tile from LDS and writes it to the destination.  Every
 // arithmetic step is the one-step kernel's, so results are bit for bit the same; C_k
-// itself is never written, so a batch ends with a one-step launch (downloads re-gather
-// the state from the last stored C).  Forces of step k go to acc, of step k+1 to acc2.
+// itself is never written: after a batch that ends with a two-step launch the host
+// recomputes it before anything reads the state (the state after step k+1 is gathered from
+// C_k).  Forces of step k go to acc, of step k+1 to acc2.
 //
 // Ensemble launches (SINGLE false) take each block's lattice and tile origin from the
 // two-step tile table and the buffers from the lattice's parity; a tile may start left
