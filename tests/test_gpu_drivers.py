@@ -11,8 +11,8 @@ significant digits).  With FP64 storage the GPU step equals the reference's to ~
 so printed values agree to one unit of their last digit; with LBM2D_STORAGE=f32 the
 drivers run the FP32 form and Cd / Cl agree to 1e-4.
 
-The binaries are built from the reference sources where those are present (this
-container); the GPU box runs the prebuilt ones.
+The binaries are built from the reference sources where those are present; a machine
+without them runs the binaries built beforehand.
 """
 import json
 import math

@@ -1,9 +1,9 @@
 """The configs' boundary wording in the oracle, pinned by what the mathematics fixes.
 
-BASELINE's cylinder configs ask for an equilibrium inlet at u = 0.1 and free-slip top /
-bottom edges where the reference has Zou/He and periodic y (survey Appendix A); the
-oracle carries both as an option (orc_set_spec) with no reference counterpart, so they
-are pinned here by properties:
+The benchmark's cylinder workloads are worded with an equilibrium inlet at u = 0.1 and
+free-slip top / bottom edges where the reference has Zou/He and periodic y
+(src/lbm.hpp:21-34, 468-469); the oracle carries both as an option (orc_set_spec) with no
+reference counterpart, so they are pinned here by properties:
 
 * method of images: a free-slip lattice of height N equals the upper half of a periodic
   lattice of height 2N holding the mirrored state (populations with their y velocity
