@@ -249,3 +249,33 @@ def test_state_after_a_batch_ending_on_two_steps(storage, sr):
     assert not b.nan_flag(0)
     a.close()
     b.close()
+
+
+def test_ensemble_state_after_even_batches_with_mixed_buffer_parity():
+    """An ensemble whose lattices hold their state in different buffers (one lattice stepped
+    alone an odd number of times) and an even batch of two-step launches: the recomputed
+    previous populations of every lattice, and the state after more steps, equal one step
+    per launch bit for bit."""
+    cs = _ens_cases()
+    orcs = [build_oracle(c) for c in cs]
+    ens = _ensemble(cs, orcs, "f32")
+    ref = _ensemble(cs, orcs, "f32")
+    ref.set_launch(64, 1, 1, 0)
+    ens.set_launch(64, 16, 1, 5, graph_steps=10)
+    for e in (ens, ref):
+        e.step_lattice(2, 3)
+        e.step(4)
+    for l in range(len(cs)):
+        np.testing.assert_array_equal(ens.download(l), ref.download(l), err_msg=f"lattice {l}")
+    for e in (ens, ref):
+        e.step(10)
+        e.step_lattice(1, 1)
+        e.step(2)
+    for l in range(len(cs)):
+        np.testing.assert_array_equal(ens.download(l), ref.download(l), err_msg=f"lattice {l} later")
+        fe, fye = ens.forces(l)
+        fr, fyr = ref.forces(l)
+        np.testing.assert_array_equal(fe, fr)
+        assert not ens.nan_flag(l)
+    ens.close()
+    ref.close()
