@@ -279,3 +279,36 @@ def test_ensemble_state_after_even_batches_with_mixed_buffer_parity():
         assert not ens.nan_flag(l)
     ens.close()
     ref.close()
+
+
+@pytest.mark.parametrize("storage,sr", [("f32", False), ("f16s", True), ("bf16s", False)],
+                         ids=["f32", "f16s_sr", "bf16s"])
+def test_equilibrium_inlet_edge_tiles(storage, sr):
+    """The equilibrium inlet (every slot of an inlet node replaced by feq(1, u_in, 0)) with
+    periodic y edges: its column's tiles are edge tiles, whose closure runs in stored units;
+    populations and forces equal one step per launch bit for bit."""
+    c = BULK_CASES["les"]
+    o = build_oracle(c)
+    d = gpu_desc(c)
+    d.inlet = G.INLET_EQUILIBRIUM
+    hs = []
+    for launch in ((64, 1, 1, 0), (64, 16, 1, 5)):
+        e = G.Ensemble([d], storage=storage)
+        apply_gpu_geometry(e, 0, c, o.obstacle.copy())
+        e.upload(0, o.f.copy())
+        if sr:
+            e.set_stochastic_rounding(True, 3)
+        e.set_launch(*launch, graph_steps=10)
+        hs.append(e)
+    a, b = hs
+    assert b.info().edge_tiles > 0
+    for n in (3, 10, 8):
+        a.step(n)
+        b.step(n)
+        np.testing.assert_array_equal(b.download(0), a.download(0), err_msg=f"n {n}")
+    fa, _ = a.forces(0)
+    fb, _ = b.forces(0)
+    np.testing.assert_array_equal(fb, fa)
+    assert not b.nan_flag(0)
+    a.close()
+    b.close()
